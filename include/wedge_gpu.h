@@ -1,0 +1,196 @@
+/*
+ * wedge_gpu.h -- C-ABI of the MI355X (gfx950) pull-engine phases.
+ *
+ * This is the drop-in boundary for the reference's phase operators
+ * (include/phases.h:27-88, include/wedge.h:23-27).  The reference passes data
+ * through process globals (include/graphdata.h:31-110) and returns nothing;
+ * here every device buffer belongs to a context, host pointers are borrowed
+ * only for the duration of a call, and every entry point returns an int status
+ * (0 = WG_OK, negative = WG_E_*).  Nothing in this library exits or aborts.
+ *
+ * Calls on one context are serialised on the context's HIP stream; a context
+ * must be used by one host thread at a time.  Scalars that the reference
+ * returns through graph_reduce_buffer (source/phases.c:26-50) come back through
+ * optional out-parameters: passing a non-NULL pointer makes the call wait for
+ * the device and copy the value; passing NULL leaves the value on the device
+ * and returns without waiting.
+ */
+#ifndef WEDGE_GPU_H
+#define WEDGE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wg_ctx wg_ctx;
+
+/* ---- status codes --------------------------------------------------------- */
+#define WG_OK           0
+#define WG_E_INVAL     -1  /* bad argument */
+#define WG_E_NOMEM     -2  /* device or host allocation failed */
+#define WG_E_HIP       -3  /* a HIP runtime call failed (see wg_last_hip_error) */
+#define WG_E_STATE     -4  /* call not valid in the context's current state */
+#define WG_E_RANGE     -5  /* vertex id or size out of range */
+#define WG_E_IO        -6  /* file could not be opened / read / written */
+#define WG_E_FORMAT    -7  /* input file malformed (size check, ordering) */
+#define WG_E_OVERFLOW  -8  /* integer distance exceeded 32 bits */
+#define WG_E_NODEV     -9  /* no HIP device */
+
+/* ---- enums ---------------------------------------------------------------- */
+#define WG_WEIGHT_NONE 0
+#define WG_WEIGHT_F64  1
+#define WG_WEIGHT_U32  2
+
+#define WG_APP_NONE 0
+#define WG_APP_BFS  1
+#define WG_APP_CC   2
+#define WG_APP_PR   3
+#define WG_APP_SSSP 4
+
+#define WG_MODE_DENSE    0  /* reference: perform_edge_pull_phase_X */
+#define WG_MODE_FRONTIER 1  /* reference: perform_edge_pull_phase_X_with_wedge_frontier */
+
+/* Default frontier granularity: 2^4 = 16 edges per Wedge bit, the reference's
+ * 4 edge vectors of 4 lanes per bit (include/configuration.h:19-24). */
+#define WG_DEFAULT_BLOCK_LOG2 4
+#define WG_MIN_BLOCK_LOG2 3
+#define WG_MAX_BLOCK_LOG2 10
+
+/* Kernel ids for wg_profile_get. */
+#define WG_K_PR_TILE        0  /* fused PageRank edge pull + vertex phase, edge tiles */
+#define WG_K_PR_SPAN        1  /* PageRank destinations spanning several tiles */
+#define WG_K_PR_VERTEX      2  /* stand-alone vertex phase */
+#define WG_K_MIN_TILE       3  /* CC / SSSP dense pull */
+#define WG_K_MIN_BLOCKS     4  /* CC / SSSP frontier pull */
+#define WG_K_BFS_DENSE      5
+#define WG_K_BFS_HEAVY      6
+#define WG_K_BFS_BLOCKS     7
+#define WG_K_WEDGE_BUILD    8
+#define WG_K_ADVANCE        9
+#define WG_K_PR_ACC        10  /* stand-alone PageRank edge pull (accumulators) */
+#define WG_K_COUNT         11
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Opens a context on HIP device `device` with its own stream. */
+int wg_open(wg_ctx** out, int device);
+void wg_close(wg_ctx* ctx);
+const char* wg_error_string(int code);
+/* Last hipError_t seen by the context (0 if none). */
+int wg_last_hip_error(const wg_ctx* ctx);
+/* Waits for all work queued on the context's stream. */
+int wg_sync(wg_ctx* ctx);
+
+/* Uploads a destination-grouped CSC and builds every derived device structure:
+ * out-degrees, per-edge destination column, the source-to-edge index (the
+ * reference's CSRI), the PageRank edge tiles.  offsets has V+1 entries with
+ * offsets[V] == E < 2^32; sources has E entries, ascending within each
+ * destination.  weights (E entries) is read according to weight_kind.
+ * Replaces graph_data_read_from_file + graph_data_initialize_data_structures
+ * (source/graphdata.c:946-1133, :648-942) for the device. */
+int wg_load_csc(wg_ctx* ctx, uint64_t num_vertices, uint64_t num_edges,
+                const uint32_t* offsets, const uint32_t* sources,
+                const void* weights, int weight_kind);
+
+/* Number of vertices / edges loaded. */
+int wg_graph_info(const wg_ctx* ctx, uint64_t* num_vertices, uint64_t* num_edges);
+/* Out-degree array (V entries) as computed on the device. */
+int wg_read_outdegrees(wg_ctx* ctx, uint32_t* host_out);
+/* Out-degree of one vertex (the reference reads graph_vertex_outdegrees[root]
+ * to seed num_edges_active, source/execution_bfs.c:133). */
+int wg_vertex_outdegree(wg_ctx* ctx, uint64_t vertex, uint32_t* out);
+
+/* ---- per-application initialisation (execution_initialize_*) -------------- */
+
+/* source/execution_pr.c:73-85: contrib = (1/N)/od', accumulators = 0.
+ * contrib_fp32 != 0 stores contributions in FP32 (accumulation stays FP64). */
+int wg_init_pr(wg_ctx* ctx, double damping, int contrib_fp32);
+/* source/execution_bfs.c:84-118: root visited at depth 0, root in frontier. */
+int wg_init_bfs(wg_ctx* ctx, uint64_t root);
+/* source/execution_cc.c:69-79: label[v] = v. */
+int wg_init_cc(wg_ctx* ctx);
+/* source/execution_sssp.c:70-89: dist = +inf, dist[root] = 0, root active.
+ * Distances are FP64 for WG_WEIGHT_F64 graphs, uint32 for WG_WEIGHT_U32. */
+int wg_init_sssp(wg_ctx* ctx, uint64_t root);
+
+/* ---- PageRank phases ------------------------------------------------------ */
+
+/* perform_edge_pull_phase_pr (source/phase_edge_pull_pr.c:28-116) followed by
+ * edge_pull_op_merge_with_merge_buffer (source/phases.c:69-114) and the
+ * reduce-buffer combine (source/phases.c:40-50): accumulators[v] = sum of
+ * contributions over in-edges; total_rank = sum of all accumulators. */
+int wg_edge_pull_pr(wg_ctx* ctx, double* total_rank);
+/* perform_vertex_phase_pr (source/phase_vertex_pr.c:26-77), using the
+ * total_rank of the preceding wg_edge_pull_pr. */
+int wg_vertex_pr(wg_ctx* ctx);
+/* One whole PageRank iteration (edge pull + vertex phase) as one fused tile
+ * kernel plus a tile-spanning fix-up; the total rank the vertex phase needs is
+ * carried from the previous iteration.  total_rank (optional) is the sum of
+ * the accumulators of this iteration. */
+int wg_pr_iteration(wg_ctx* ctx, double* total_rank);
+/* Final ranks: rank[v] = contrib[v] * od'[v] (source/execution_pr.c:39-42). */
+int wg_read_ranks(wg_ctx* ctx, double* host_ranks);
+
+/* ---- traversal phases ----------------------------------------------------- */
+
+/* perform_edge_pull_phase_bfs (source/phase_edge_pull_bfs.c:109-210) for
+ * WG_MODE_DENSE and perform_edge_pull_phase_bfs_with_wedge_frontier (:212-332)
+ * for WG_MODE_FRONTIER.  edges_activated = sum of out-degrees of the vertices
+ * discovered (the reference's num_edges_changed summed over threads). */
+int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated);
+/* perform_edge_pull_phase_cc[_with_wedge_frontier]
+ * (source/phase_edge_pull_cc.c:53-164, :166-297) plus the idempotent merge
+ * (source/phases.c:54-65). */
+int wg_edge_pull_cc(wg_ctx* ctx, int mode, uint64_t* edges_activated);
+/* perform_edge_pull_phase_sssp[_with_wedge_frontier]
+ * (source/phase_edge_pull_sssp.c:57-169, :171-306) plus the idempotent merge.
+ * Returns WG_E_OVERFLOW if an integer distance reached 2^32 - 1. */
+int wg_edge_pull_sssp(wg_ctx* ctx, int mode, uint64_t* edges_activated);
+
+/* Edges and vertices activated by the last pull whose edges_activated
+ * out-parameter was non-NULL (values already on the host; no device wait). */
+int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices);
+
+/* wedge_generate_frontier (source/wedge.c:63-155): marks every 2^block_log2
+ * edge block that holds an out-edge of a vertex activated by the last pull,
+ * and queues those blocks for the next WG_MODE_FRONTIER pull. */
+int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2);
+/* wedge_activate_single_vertex (source/wedge.c:52-59). */
+int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int block_log2);
+/* End of an iteration: the vertices activated by the last pull become the
+ * current frontier and the next-frontier bitmask is cleared (the reference's
+ * active/snapshot swap and memset, source/execution_bfs.c:326-338 and
+ * source/execution_cc.c:243-277).  frontier_vertices (optional) = number of
+ * vertices in the new frontier. */
+int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices);
+
+/* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
+ * the root holds (0 << 32) | root. */
+int wg_read_bfs(wg_ctx* ctx, uint64_t* host_packed);
+/* CC result: uint32 minimum label per vertex. */
+int wg_read_cc(wg_ctx* ctx, uint32_t* host_labels);
+/* SSSP result: double (F64 graphs, +inf unreached) or uint32 (U32 graphs,
+ * 0xFFFFFFFF unreached). */
+int wg_read_sssp(wg_ctx* ctx, void* host_dist);
+
+/* Activity counters of the current state: vertices reached (BFS: visited;
+ * SSSP: finite distance) and the sum of their out-degrees. */
+int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Enables per-launch HIP-event timing of every kernel on the context stream. */
+int wg_profile_enable(wg_ctx* ctx, int on);
+/* Launch count and summed device time of kernel `kernel_id` since enabling
+ * or the last reset (waits for the stream). */
+int wg_profile_get(wg_ctx* ctx, int kernel_id, uint64_t* launches, double* total_ms);
+int wg_profile_reset(wg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WEDGE_GPU_H */

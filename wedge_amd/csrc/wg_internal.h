@@ -1,0 +1,185 @@
+// wg_internal.h -- context layout and helpers shared by the HIP translation units.
+//
+// Device layout of a loaded graph (all 32-bit indices, one HBM pool):
+//   off[V+1]        CSC offsets by destination
+//   src[E]          source of each edge, ascending within a destination
+//   dst[E]          destination of each edge (read only by frontier-mode pulls,
+//                   which touch only the queued edge blocks)
+//   w[E]            weights: double (SSSP FP64) or uint32 (SSSP integer)
+//   od[V]           out-degree
+//   csri_off[V+1],  source -> positions of its out-edges in src[]
+//   csri_pos[E]     (the reference's CSRI, source/graphdata.c:493-585, but holding
+//                   edge positions so any block granularity can be derived)
+//   tile_v[T+1],    edge tiles for the dense pulls: a light tile owns whole
+//   tile_e[T+1]     destinations [tile_v[t], tile_v[t+1]) with <= TILE_EDGES
+//                   edges and <= TILE_VERTS vertices; a destination with more
+//                   than TILE_EDGES in-edges is cut into heavy tiles of
+//                   TILE_EDGES edges that all carry tile_v[t] = that vertex
+//   heavy_v[H], heavy_t[H]  heavy destinations and their first tile
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "wedge_gpu.h"
+
+#define WG_WAVE 64
+#define WG_BLOCK 256
+#define WG_IPT 8                              // edges per thread in a tile
+#define WG_TILE_EDGES (WG_BLOCK * WG_IPT)     // 2048
+#define WG_TILE_VERTS 2048                    // destinations per light tile
+#define WG_BFS_SEQ 16                         // edges a lane scans alone in dense BFS
+#define WG_BFS_HEAVY 2048                     // in-degree above which dense BFS splits a vertex
+
+// Device-side counters, one struct per context.
+struct wg_dev_counters {
+    unsigned long long activity;        // sum of out-degrees of vertices activated this pull
+    unsigned long long activated;       // number of vertices activated this pull
+    unsigned long long frontier_verts;  // popcount of the frontier made current by advance
+    unsigned int queue_len;             // edge blocks queued for the next frontier pull
+    unsigned int heavy_len;             // high out-degree vertices deferred by the frontier build
+    unsigned int overflow;              // integer SSSP distance reached 2^32-1
+    unsigned int ticket;                // last-block-done ticket (PR span kernel)
+    double S[2];                        // PR: sum of ranks of vertices with out-degree > 0
+    double total_rank;                  // PR: sum of accumulators of the last iteration
+    unsigned long long reached_verts;
+    unsigned long long reached_edges;
+};
+
+struct wg_prof {
+    int enabled;
+    uint64_t launches[WG_K_COUNT];
+    double total_ms[WG_K_COUNT];
+    // pending event pairs
+    int npending;
+    int cap;
+    hipEvent_t* ev;   // 2 * cap
+    int* kid;         // cap
+};
+
+struct wg_ctx {
+    int device;
+    hipStream_t stream;
+    int last_hip_error;
+    int num_cu;
+
+    // graph
+    uint64_t V, E;
+    int wkind;
+    uint32_t* off;
+    uint32_t* src;
+    uint32_t* dst;
+    void* w;
+    uint32_t* od;
+    uint32_t* csri_off;
+    uint32_t* csri_pos;
+    uint32_t ntiles;
+    uint32_t* tile_v;
+    uint32_t* tile_e;
+    uint32_t nheavy;
+    uint32_t* heavy_v;
+    uint32_t* heavy_t;
+    uint32_t nbfs_heavy;      // dense BFS chunks of vertices with in-degree > WG_BFS_HEAVY
+    uint32_t* bfs_heavy_v;
+    uint32_t* bfs_heavy_e;    // chunk start edge (chunk end = min(start + WG_BFS_HEAVY, off[v+1]))
+    uint32_t n_out_pos;       // vertices with out-degree > 0
+
+    // application state
+    int app;
+    void* prop[2];            // PR contributions (double buffer) / BFS packed / CC labels / SSSP dist
+    int pr_cur;               // PR: index of the current contribution buffer
+    int pr_parity;            // PR: index of S[] holding the current iteration's sum
+    int pr_fp32;
+    double damping;
+    double* acc;              // PR accumulators (stand-alone edge phase)
+    double* tile_part;        // PR: per-tile partial sums of heavy destinations
+    double* blk_part;         // PR: per-block partial sums for S
+    uint32_t pr_grid;         // blocks of the tile kernel
+    uint32_t span_grid;       // blocks of the span kernel
+    uint64_t bm_words;        // 64-bit words per vertex bitmask
+    uint64_t* bm_cur;         // BFS: current frontier
+    uint64_t* bm_next;        // activated by the running pull
+    uint64_t* bm_visited;     // BFS: visited before this level
+    uint32_t* wedge;          // Wedge frontier, one bit per edge block
+    uint64_t wedge_words;     // 32-bit words allocated (finest granularity)
+    int wedge_log2;           // granularity of the queued blocks
+    uint32_t* queue;          // queued edge blocks
+    uint32_t* heavy_list;     // frontier build: deferred high out-degree vertices
+    uint64_t queue_cap;
+    wg_dev_counters* ctr;     // device
+    wg_dev_counters* hctr;    // pinned host mirror
+    int level;                // BFS level of the running pull (current frontier depth)
+    int iter;                 // iterations run since init
+
+    wg_prof prof;
+};
+
+// ---- error plumbing --------------------------------------------------------
+
+#define WG_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t _e = (call);                             \
+        if (_e != hipSuccess) {                             \
+            if (ctx)                                        \
+                (ctx)->last_hip_error = (int)_e;            \
+            return WG_E_HIP;                                \
+        }                                                   \
+    } while (0)
+
+// Launch bracket for per-kernel timing.
+int wg_prof_begin(wg_ctx* ctx, int kid);
+int wg_prof_end(wg_ctx* ctx, int kid, int slot);
+
+static inline int wg_check_launch(wg_ctx* ctx)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->last_hip_error = (int)e;
+        return WG_E_HIP;
+    }
+    return WG_OK;
+}
+
+static inline uint32_t wg_div_up(uint64_t a, uint64_t b)
+{
+    return (uint32_t)((a + b - 1) / b);
+}
+
+// Number of blocks for a grid-stride kernel: enough to fill the chip several
+// times over (256 CUs in 8 XCDs) but no more than the work needs.
+static inline uint32_t wg_grid(const wg_ctx* ctx, uint64_t work_items, uint32_t items_per_block, uint32_t per_cu)
+{
+    uint64_t need = (work_items + items_per_block - 1) / items_per_block;
+    uint64_t cap = (uint64_t)ctx->num_cu * per_cu;
+    if (need > cap)
+        need = cap;
+    if (need < 1)
+        need = 1;
+    return (uint32_t)need;
+}
+
+// ---- device helpers ---------------------------------------------------------
+
+__device__ __forceinline__ int wg_lane() { return (int)(threadIdx.x & (WG_WAVE - 1)); }
+
+__device__ __forceinline__ unsigned long long wg_ballot(int pred)
+{
+    return __ballot(pred);
+}
+
+template <typename T>
+__device__ __forceinline__ T wg_wave_sum(T v)
+{
+    for (int o = WG_WAVE / 2; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, WG_WAVE);
+    return v;
+}
+
+// internal entry points shared across translation units
+int wg_pr_reset(wg_ctx* ctx);
+int wg_free_app_state(wg_ctx* ctx);
+int wg_alloc_bitmasks(wg_ctx* ctx);
+int wg_clear_u64(wg_ctx* ctx, uint64_t* p, uint64_t words);
+int wg_fill_u32(wg_ctx* ctx, uint32_t* p, uint64_t n, uint32_t value);
+int wg_fill_u64(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t value);

@@ -1,0 +1,992 @@
+// wg_traverse.hip -- BFS, CC and SSSP edge pulls, the Wedge frontier, and the
+// end-of-iteration frontier handling on gfx950.
+//
+// Reference phases: source/phase_edge_pull_bfs.c:109-332,
+// source/phase_edge_pull_cc.c:53-297, source/phase_edge_pull_sssp.c:57-306,
+// source/wedge.c:37-155, the idempotent merge source/phases.c:54-65, and the
+// frontier swap/clear in source/execution_{bfs,cc,sssp}.c.
+//
+// Vertex state on the device:
+//   BFS   packed[v] = (depth << 32) | parent, UINT64_MAX while unvisited.  The
+//         parent is the smallest in-neighbour on the previous level in both
+//         pull modes (dense: first hit in ascending source order; frontier:
+//         atomicMin of the packed word), so the result does not depend on the
+//         mode sequence.  The reference keeps only a parent (its largest
+//         active source within the first matching edge vector).
+//   CC    label[v] uint32, min-label propagation in place.
+//   SSSP  FP64 distances as IEEE bit patterns in uint64 (non-negative doubles
+//         order like their bit patterns, so atomicMin on the bits is the FP64
+//         minimum) or uint32 integer distances.
+// Bitmasks (one bit per vertex, 64-bit words): cur (BFS frontier), next
+// (activated by the running pull: the reference's active bitmask with its
+// cond_set_bit semantics), visited (BFS, visited before the running level).
+// The Wedge frontier is one bit per edge block of 2^b consecutive CSC edges;
+// marking a block also appends it to a queue, and the frontier pull walks the
+// queue, so its cost follows the active blocks instead of the bitmask size.
+#include "wg_internal.h"
+
+__device__ __forceinline__ bool test_bit64(const uint64_t* __restrict__ bm, uint32_t i)
+{
+    return (bm[i >> 6] >> (i & 63)) & 1ull;
+}
+
+// Per-thread activation tally: activated vertices and their out-edges.
+struct act_tally {
+    unsigned long long edges;
+    unsigned long long verts;
+};
+
+// Marks vertex v activated (the reference's cond_set_bit + out-degree count,
+// source/phase_edge_pull_cc.c:119-123); counts it if this call set the bit.
+__device__ __forceinline__ void activate(uint64_t* __restrict__ next, const uint32_t* __restrict__ od, uint32_t v,
+                                         act_tally& t)
+{
+    const unsigned long long m = 1ull << (v & 63);
+    const unsigned long long old = atomicOr((unsigned long long*)&next[v >> 6], m);
+    if (!(old & m)) {
+        t.edges += od[v];
+        t.verts += 1ull;
+    }
+}
+
+__device__ __forceinline__ void add_count(unsigned long long a, unsigned long long* __restrict__ dst)
+{
+    a = wg_wave_sum(a);
+    if (wg_lane() == 0 && a)
+        atomicAdd(dst, a);
+}
+
+__device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* __restrict__ c)
+{
+    add_count(t.edges, &c->activity);
+    add_count(t.verts, &c->activated);
+}
+
+// ---- counters -------------------------------------------------------------------
+
+#define RST_ACTIVITY 1u
+#define RST_QUEUE 2u
+#define RST_FRONTIER 4u
+#define RST_OVERFLOW 8u
+#define RST_HEAVY 16u
+
+__global__ void k_reset(wg_dev_counters* c, unsigned int flags)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (flags & RST_ACTIVITY) {
+            c->activity = 0ull;
+            c->activated = 0ull;
+        }
+        if (flags & RST_QUEUE) {
+            c->queue_len = 0u;
+            c->heavy_len = 0u;
+        }
+        if (flags & RST_FRONTIER)
+            c->frontier_verts = 0ull;
+        if (flags & RST_OVERFLOW)
+            c->overflow = 0u;
+        if (flags & RST_HEAVY)
+            c->heavy_len = 0u;
+    }
+}
+
+static int reset_counters(wg_ctx* ctx, unsigned int flags)
+{
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr, flags);
+    return wg_check_launch(ctx);
+}
+
+// ---- BFS -------------------------------------------------------------------------
+
+// Dense bottom-up level: one wave per 64 consecutive vertices.  A lane scans
+// up to WG_BFS_SEQ in-edges of its own vertex; vertices with more edges are
+// finished by the whole wave, 64 edges per step, stopping at the first hit.
+// Vertices with more than WG_BFS_HEAVY in-edges are left to k_bfs_heavy.
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                                                         const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                                         uint64_t* __restrict__ packed, uint32_t V, uint32_t newdepth,
+                                                         wg_dev_counters* __restrict__ ctr)
+{
+    const int lane = wg_lane();
+    const uint64_t nwords = ((uint64_t)V + 63) / 64;
+    const uint64_t nwaves = (uint64_t)gridDim.x * (WG_BLOCK / WG_WAVE);
+    act_tally act = { 0ull, 0ull };
+    for (uint64_t w = (uint64_t)blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
+        const uint64_t vis = visited[w];
+        if (vis == ~0ull)
+            continue;
+        const uint32_t v = (uint32_t)(w * 64 + lane);
+        bool todo = (v < V) && !((vis >> lane) & 1ull);
+        uint32_t s = 0, e = 0;
+        if (todo) {
+            s = off[v];
+            e = off[v + 1];
+        }
+        const uint32_t deg = e - s;
+        todo = todo && deg > 0 && deg <= WG_BFS_HEAVY;
+        if (wg_ballot(todo) == 0ull)
+            continue;
+        int64_t parent = -1;
+        if (todo) {
+            const uint32_t lim = (deg < WG_BFS_SEQ) ? e : s + WG_BFS_SEQ;
+            for (uint32_t p = s; p < lim; ++p) {
+                const uint32_t u = src[p];
+                if (test_bit64(cur, u)) {
+                    parent = u;
+                    break;
+                }
+            }
+        }
+        unsigned long long need = wg_ballot(todo && parent < 0 && deg > WG_BFS_SEQ);
+        while (need) {
+            const int l = __ffsll((long long)need) - 1;
+            need &= need - 1ull;
+            const uint32_t ls = __shfl(s, l) + WG_BFS_SEQ;
+            const uint32_t le = __shfl(e, l);
+            int64_t found = -1;
+            for (uint32_t p = ls; p < le && found < 0; p += WG_WAVE) {
+                const uint32_t q = p + lane;
+                uint32_t u = 0;
+                bool hit = false;
+                if (q < le) {
+                    u = src[q];
+                    hit = test_bit64(cur, u);
+                }
+                const unsigned long long hm = wg_ballot(hit);
+                if (hm) {
+                    const int fl = __ffsll((long long)hm) - 1;
+                    found = (int64_t)__shfl(u, fl);
+                }
+            }
+            if (lane == l && found >= 0)
+                parent = found;
+        }
+        const bool disc = parent >= 0;
+        const unsigned long long dm = wg_ballot(disc);
+        if (disc) {
+            packed[v] = ((uint64_t)newdepth << 32) | (uint64_t)(uint32_t)parent;
+            act.edges += od[v];
+            act.verts += 1ull;
+        }
+        if (lane == 0 && dm)
+            atomicOr((unsigned long long*)&next[w], dm);
+    }
+    add_tally(act, ctr);
+}
+
+// Dense level for one chunk of WG_BFS_HEAVY in-edges of a high in-degree vertex.
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
+                                                         uint32_t n, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
+                                                         const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
+                                                         uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
+                                                         uint32_t newdepth, wg_dev_counters* __restrict__ ctr)
+{
+    const int lane = wg_lane();
+    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
+    act_tally act = { 0ull, 0ull };
+    for (uint32_t c = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; c < n; c += nwaves) {
+        const uint32_t v = hv[c];
+        if (test_bit64(visited, v))
+            continue;
+        const uint32_t cs = he[c];
+        const uint32_t vend = off[v + 1];
+        const uint32_t ce = (cs + WG_BFS_HEAVY < vend) ? cs + WG_BFS_HEAVY : vend;
+        // a lower chunk already found a parent smaller than anything here
+        const uint64_t pk = packed[v];
+        if ((uint32_t)(pk >> 32) == newdepth && (uint32_t)pk < src[cs])
+            continue;
+        int64_t found = -1;
+        for (uint32_t p = cs; p < ce && found < 0; p += WG_WAVE) {
+            const uint32_t q = p + lane;
+            uint32_t u = 0;
+            bool hit = false;
+            if (q < ce) {
+                u = src[q];
+                hit = test_bit64(cur, u);
+            }
+            const unsigned long long hm = wg_ballot(hit);
+            if (hm) {
+                const int fl = __ffsll((long long)hm) - 1;
+                found = (int64_t)__shfl(u, fl);
+            }
+        }
+        if (found >= 0 && lane == 0) {
+            atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)(uint32_t)found);
+            activate(next, od, v, act);
+        }
+    }
+    add_tally(act, ctr);
+}
+
+// Lane -> (queued block, edge) mapping shared by the frontier pulls.
+struct block_walk {
+    uint32_t per_wave;   // blocks per wave step
+    uint32_t lane_bits;  // log2 of lanes per block
+    uint32_t steps;      // 64-edge steps per block
+};
+
+__device__ __forceinline__ block_walk make_walk(int blog2)
+{
+    block_walk b;
+    b.lane_bits = blog2 < 6 ? (uint32_t)blog2 : 6u;
+    b.per_wave = 64u >> b.lane_bits;
+    b.steps = blog2 > 6 ? (1u << (blog2 - 6)) : 1u;
+    return b;
+}
+
+// Frontier-mode BFS level over the queued edge blocks.
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restrict__ queue,
+                                                          wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                                          uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
+                                                          uint32_t newdepth)
+{
+    const int lane = wg_lane();
+    const uint32_t qlen = ctr->queue_len;
+    const block_walk bw = make_walk(blog2);
+    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
+    act_tally act = { 0ull, 0ull };
+    for (uint32_t q0 = (blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE) * bw.per_wave; q0 < qlen;
+         q0 += nwaves * bw.per_wave) {
+        const uint32_t q = q0 + ((uint32_t)lane >> bw.lane_bits);
+        const bool qok = q < qlen;
+        const uint32_t blk = qok ? queue[q] : 0u;
+        if (qok && ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) == 0u)
+            wedge[blk >> 5] = 0u; // consume the word; the queue drives the work
+        for (uint32_t st = 0; st < bw.steps; ++st) {
+            const uint64_t p64 = ((uint64_t)blk << blog2) + (uint64_t)st * 64u + ((uint32_t)lane & ((1u << bw.lane_bits) - 1u));
+            const bool ok = qok && p64 < E;
+            const uint32_t p = (uint32_t)p64;
+            bool cand = false;
+            uint32_t u = 0, v = 0;
+            if (ok) {
+                u = src[p];
+                v = dst[p];
+                cand = test_bit64(cur, u) && !test_bit64(visited, v);
+            }
+            // only the first candidate of a run of equal destinations within
+            // one block needs the atomic: sources ascend within a destination
+            // (lanes of different blocks are not compared: queued blocks come
+            // in no particular order)
+            const uint32_t pv = __shfl_up(v, 1);
+            const unsigned long long cm = wg_ballot(cand);
+            const bool prev_same = ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) != 0u &&
+                                   ((cm >> (lane - 1)) & 1ull) && pv == v;
+            if (cand && !prev_same) {
+                atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)u);
+                activate(next, od, v, act);
+            }
+        }
+    }
+    add_tally(act, ctr);
+}
+
+// End of a BFS level: visited |= next, the old frontier is cleared (it becomes
+// the next 'next'), and the new frontier is counted.
+__global__ void k_bfs_advance(uint64_t* __restrict__ visited, const uint64_t* __restrict__ next,
+                              uint64_t* __restrict__ oldcur, uint64_t nwords, unsigned long long* __restrict__ fv)
+{
+    unsigned long long cnt = 0;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t n = next[w];
+        if (n) {
+            visited[w] |= n;
+            cnt += (unsigned long long)__popcll(n);
+        }
+        if (oldcur[w])
+            oldcur[w] = 0ull;
+    }
+    add_count(cnt, fv);
+}
+
+__global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, unsigned long long* __restrict__ fv)
+{
+    unsigned long long cnt = 0;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t b = bm[w];
+        if (b) {
+            cnt += (unsigned long long)__popcll(b);
+            bm[w] = 0ull;
+        }
+    }
+    add_count(cnt, fv);
+}
+
+// ---- CC / SSSP (min propagation) -------------------------------------------------
+
+struct PolCC {
+    typedef uint32_t T;
+    static constexpr uint32_t INF = 0xffffffffu;
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        (void)w;
+        (void)p;
+        (void)ovf;
+        return prop[u];
+    }
+};
+
+struct PolSF { // FP64 distances as bit patterns
+    typedef unsigned long long T;
+    static constexpr unsigned long long INF = 0x7FF0000000000000ull;
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        (void)ovf;
+        const T du = prop[u];
+        if (du >= INF)
+            return INF;
+        const double s = __longlong_as_double((long long)du) + ((const double*)w)[p];
+        return (T)__double_as_longlong(s);
+    }
+};
+
+struct PolSU { // 32-bit integer distances
+    typedef uint32_t T;
+    static constexpr uint32_t INF = 0xffffffffu;
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        const uint32_t du = prop[u];
+        if (du == INF)
+            return INF;
+        const uint32_t s = du + ((const uint32_t*)w)[p];
+        if (s < du || s == INF) {
+            *ovf = 1u;
+            return INF;
+        }
+        return s;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ T atomic_min_t(T* p, T v);
+template <>
+__device__ __forceinline__ uint32_t atomic_min_t<uint32_t>(uint32_t* p, uint32_t v)
+{
+    return atomicMin(p, v);
+}
+template <>
+__device__ __forceinline__ unsigned long long atomic_min_t<unsigned long long>(unsigned long long* p, unsigned long long v)
+{
+    return atomicMin(p, v);
+}
+
+template <class P>
+__device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint64_t* __restrict__ next,
+                                          const uint32_t* __restrict__ od, uint32_t v, typename P::T m, act_tally& t)
+{
+    if (m >= prop[v])
+        return;
+    const typename P::T old = atomic_min_t<typename P::T>(&prop[v], m);
+    if (old > m)
+        activate(next, od, v, t);
+}
+
+// Dense pull over the edge tiles: per-destination minimum in LDS, then one
+// global update per changed destination.
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                        const void* __restrict__ w, typename P::T* __restrict__ prop,
+                                                        const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
+                                                        const uint32_t* __restrict__ tile_e, uint32_t ntiles,
+                                                        uint64_t* __restrict__ next, wg_dev_counters* __restrict__ ctr,
+                                                        unsigned int* __restrict__ ovf)
+{
+    typedef typename P::T T;
+    __shared__ uint32_t s_off[WG_TILE_VERTS + 1];
+    __shared__ T s_min[WG_TILE_VERTS];
+    const int tid = threadIdx.x;
+    act_tally act = { 0ull, 0ull };
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint32_t v0 = tile_v[t];
+        const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
+        const uint32_t a = e0 + (uint32_t)tid * WG_IPT;
+        const bool heavy = (off[v0 + 1] - off[v0]) > WG_TILE_EDGES;
+        const uint32_t nv = heavy ? 1u : tile_v[t + 1] - v0;
+        for (uint32_t i = tid; i <= nv; i += WG_BLOCK)
+            s_off[i] = heavy ? (i == 0 ? e0 : e1) : off[v0 + i];
+        for (uint32_t i = tid; i < nv; i += WG_BLOCK)
+            s_min[i] = P::INF;
+        __syncthreads();
+        if (a < e1) {
+            T g[WG_IPT];
+#pragma unroll
+            for (int j = 0; j < WG_IPT; ++j) {
+                const uint32_t e = a + j;
+                g[j] = (e < e1) ? P::val(prop, w, src[e], e, ovf) : P::INF;
+            }
+            uint32_t lo = 0, hi = nv - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= a)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            uint32_t d = lo;
+            uint32_t dend = s_off[d + 1];
+            T run = P::INF;
+#pragma unroll
+            for (int j = 0; j < WG_IPT; ++j) {
+                const uint32_t e = a + j;
+                if (e < e1) {
+                    run = g[j] < run ? g[j] : run;
+                    if (e + 1 == dend) {
+                        if (run != P::INF)
+                            atomic_min_t<T>(&s_min[d], run);
+                        run = P::INF;
+                        ++d;
+                        while (d < nv && s_off[d + 1] == s_off[d])
+                            ++d;
+                        dend = (d < nv) ? s_off[d + 1] : 0xffffffffu;
+                    }
+                }
+            }
+            if (run != P::INF && d < nv)
+                atomic_min_t<T>(&s_min[d], run);
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < nv; i += WG_BLOCK) {
+            const T m = s_min[i];
+            if (m != P::INF)
+                min_apply<P>(prop, next, od, v0 + i, m, act);
+        }
+        __syncthreads();
+    }
+    add_tally(act, ctr);
+}
+
+// Frontier-mode pull over the queued edge blocks.
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(const uint32_t* __restrict__ queue,
+                                                          wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                                          const void* __restrict__ w, typename P::T* __restrict__ prop,
+                                                          const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
+                                                          uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf)
+{
+    typedef typename P::T T;
+    const int lane = wg_lane();
+    const uint32_t qlen = ctr->queue_len;
+    const block_walk bw = make_walk(blog2);
+    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
+    act_tally act = { 0ull, 0ull };
+    for (uint32_t q0 = (blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE) * bw.per_wave; q0 < qlen;
+         q0 += nwaves * bw.per_wave) {
+        const uint32_t q = q0 + ((uint32_t)lane >> bw.lane_bits);
+        const bool qok = q < qlen;
+        const uint32_t blk = qok ? queue[q] : 0u;
+        if (qok && ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) == 0u)
+            wedge[blk >> 5] = 0u;
+        for (uint32_t st = 0; st < bw.steps; ++st) {
+            const uint64_t p64 = ((uint64_t)blk << blog2) + (uint64_t)st * 64u + ((uint32_t)lane & ((1u << bw.lane_bits) - 1u));
+            if (qok && p64 < E) {
+                const uint32_t p = (uint32_t)p64;
+                const T m = P::val(prop, w, src[p], p, ovf);
+                if (m != P::INF)
+                    min_apply<P>(prop, next, od, dst[p], m, act);
+            }
+        }
+    }
+    add_tally(act, ctr);
+}
+
+// ---- Wedge frontier ---------------------------------------------------------------
+
+__device__ __forceinline__ void mark_block(uint32_t blk, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
+                                           wg_dev_counters* __restrict__ ctr)
+{
+    const uint32_t m = 1u << (blk & 31);
+    uint32_t* wp = &wedge[blk >> 5];
+    if (*wp & m)
+        return;
+    const uint32_t old = atomicOr(wp, m);
+    if (!(old & m)) {
+        const uint32_t at = atomicAdd(&ctr->queue_len, 1u);
+        queue[at] = blk;
+    }
+}
+
+#define WG_WEDGE_LANE_MAX 32u
+
+// source/wedge.c:63-155: every set vertex marks the blocks of its out-edges.
+__global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __restrict__ active, uint64_t nwords,
+                                                           const uint32_t* __restrict__ csri_off,
+                                                           const uint32_t* __restrict__ csri_pos, int blog2,
+                                                           uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
+                                                           uint32_t* __restrict__ heavy, wg_dev_counters* __restrict__ ctr)
+{
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t bits = active[w];
+        while (bits) {
+            const uint32_t u = (uint32_t)(w * 64 + (uint64_t)(__ffsll((long long)bits) - 1));
+            bits &= bits - 1ull;
+            const uint32_t s = csri_off[u], e = csri_off[u + 1];
+            if (e - s > WG_WEDGE_LANE_MAX) {
+                heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
+                continue;
+            }
+            for (uint32_t p = s; p < e; ++p)
+                mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+        }
+    }
+}
+
+// High out-degree vertices deferred by k_wedge_build: one wave per vertex.
+__global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy,
+                                                           const uint32_t* __restrict__ csri_off,
+                                                           const uint32_t* __restrict__ csri_pos, int blog2,
+                                                           uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
+                                                           wg_dev_counters* __restrict__ ctr)
+{
+    const uint32_t n = ctr->heavy_len;
+    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
+    for (uint32_t i = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
+        const uint32_t u = heavy[i];
+        const uint32_t s = csri_off[u], e = csri_off[u + 1];
+        for (uint32_t p = s + (uint32_t)wg_lane(); p < e; p += WG_WAVE)
+            mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+    }
+}
+
+// source/wedge.c:52-59
+__global__ void k_wedge_single(uint32_t u, const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
+                               int blog2, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
+                               wg_dev_counters* __restrict__ ctr)
+{
+    const uint32_t s = csri_off[u], e = csri_off[u + 1];
+    for (uint32_t p = s + blockIdx.x * blockDim.x + threadIdx.x; p < e; p += gridDim.x * blockDim.x)
+        mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+}
+
+__global__ void k_reached_bfs(const uint64_t* __restrict__ packed, const uint32_t* __restrict__ od, uint64_t V,
+                              wg_dev_counters* __restrict__ c)
+{
+    unsigned long long nv = 0, ne = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        if (packed[v] != ~0ull) {
+            nv += 1;
+            ne += od[v];
+        }
+    add_count(nv, &c->reached_verts);
+    add_count(ne, &c->reached_edges);
+}
+
+template <typename T>
+__global__ void k_reached_min(const T* __restrict__ prop, T inf, const uint32_t* __restrict__ od, uint64_t V,
+                              wg_dev_counters* __restrict__ c)
+{
+    unsigned long long nv = 0, ne = 0;
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        if (prop[v] < inf) {
+            nv += 1;
+            ne += od[v];
+        }
+    add_count(nv, &c->reached_verts);
+    add_count(ne, &c->reached_edges);
+}
+
+__global__ void k_set_u64(uint64_t* p, uint64_t i, uint64_t v)
+{
+    p[i] = v;
+}
+
+__global__ void k_or_bit(uint64_t* bm, uint32_t v)
+{
+    bm[v >> 6] |= 1ull << (v & 63);
+}
+
+__global__ void k_iota_u32(uint32_t* p, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
+// ---- host side ----------------------------------------------------------------------
+
+static int read_counters(wg_ctx* ctx)
+{
+    WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, sizeof(wg_dev_counters), hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
+static int clear_wedge_state(wg_ctx* ctx)
+{
+    int rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u);
+    if (rc)
+        return rc;
+    rc = reset_counters(ctx, RST_QUEUE);
+    ctx->wedge_log2 = -1;
+    return rc;
+}
+
+static int init_common(wg_ctx* ctx, int app)
+{
+    if (!ctx || !ctx->off)
+        return WG_E_STATE;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    wg_free_app_state(ctx);
+    int rc;
+    if ((rc = wg_clear_u64(ctx, ctx->bm_cur, ctx->bm_words)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
+        (rc = wg_clear_u64(ctx, ctx->bm_visited, ctx->bm_words)) || (rc = clear_wedge_state(ctx)) ||
+        (rc = reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW)))
+        return rc;
+    ctx->app = app;
+    ctx->level = 0;
+    ctx->iter = 0;
+    return WG_OK;
+}
+
+extern "C" int wg_init_bfs(wg_ctx* ctx, uint64_t root)
+{
+    int rc = init_common(ctx, WG_APP_BFS);
+    if (rc)
+        return rc;
+    if (root >= ctx->V)
+        return WG_E_RANGE;
+    WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
+    if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, ~0ull)))
+        return rc;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], root, root);
+    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_cur, (uint32_t)root);
+    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_visited, (uint32_t)root);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
+extern "C" int wg_init_cc(wg_ctx* ctx)
+{
+    int rc = init_common(ctx, WG_APP_CC);
+    if (rc)
+        return rc;
+    WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint32_t)));
+    hipLaunchKernelGGL(k_iota_u32, dim3(wg_grid(ctx, ctx->V, 1024, 8)), dim3(256), 0, ctx->stream, (uint32_t*)ctx->prop[0],
+                       ctx->V);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
+extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
+{
+    if (!ctx || !ctx->off || ctx->wkind == WG_WEIGHT_NONE)
+        return WG_E_STATE;
+    int rc = init_common(ctx, WG_APP_SSSP);
+    if (rc)
+        return rc;
+    if (root >= ctx->V)
+        return WG_E_RANGE;
+    if (ctx->wkind == WG_WEIGHT_F64) {
+        WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
+        if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, PolSF::INF)))
+            return rc;
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], root, 0ull);
+    } else {
+        WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint32_t)));
+        if ((rc = wg_fill_u32(ctx, (uint32_t*)ctx->prop[0], ctx->V, PolSU::INF)))
+            return rc;
+        WG_HIP(ctx, hipMemsetAsync((uint32_t*)ctx->prop[0] + root, 0, sizeof(uint32_t), ctx->stream));
+    }
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
+static int check_block_log2(int b)
+{
+    return b >= WG_MIN_BLOCK_LOG2 && b <= WG_MAX_BLOCK_LOG2;
+}
+
+// A queued frontier must be consumed with the granularity it was built at.
+static int prepare_frontier_build(wg_ctx* ctx, int blog2)
+{
+    if (ctx->wedge_log2 >= 0 && ctx->wedge_log2 != blog2) {
+        int rc = clear_wedge_state(ctx);
+        if (rc)
+            return rc;
+    }
+    ctx->wedge_log2 = blog2;
+    return WG_OK;
+}
+
+extern "C" int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int block_log2)
+{
+    if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
+        return WG_E_STATE;
+    if (vertex >= ctx->V)
+        return WG_E_RANGE;
+    if (!check_block_log2(block_log2))
+        return WG_E_INVAL;
+    int rc = prepare_frontier_build(ctx, block_log2);
+    if (rc)
+        return rc;
+    int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
+    hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, (uint32_t)vertex, ctx->csri_off, ctx->csri_pos,
+                       block_log2, ctx->wedge, ctx->queue, ctx->ctr);
+    rc = wg_check_launch(ctx);
+    wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
+    return rc;
+}
+
+extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
+{
+    if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
+        return WG_E_STATE;
+    if (!check_block_log2(block_log2))
+        return WG_E_INVAL;
+    int rc = prepare_frontier_build(ctx, block_log2);
+    if (rc || (rc = reset_counters(ctx, RST_HEAVY)))
+        return rc;
+    int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
+    hipLaunchKernelGGL(k_wedge_build, dim3(wg_grid(ctx, ctx->bm_words, 256, 8)), dim3(WG_BLOCK), 0, ctx->stream,
+                       ctx->bm_next, ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue,
+                       ctx->heavy_list, ctx->ctr);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
+                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue, ctx->ctr);
+    rc = wg_check_launch(ctx);
+    wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
+    return rc;
+}
+
+static int finish_pull(wg_ctx* ctx, uint64_t* edges_activated)
+{
+    ctx->iter += 1;
+    if (!edges_activated)
+        return WG_OK;
+    int rc = read_counters(ctx);
+    if (rc)
+        return rc;
+    *edges_activated = ctx->hctr->activity;
+    if (ctx->app == WG_APP_SSSP && ctx->hctr->overflow)
+        return WG_E_OVERFLOW;
+    return WG_OK;
+}
+
+extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices)
+{
+    if (!ctx)
+        return WG_E_INVAL;
+    if (edges)
+        *edges = ctx->hctr->activity;
+    if (vertices)
+        *vertices = ctx->hctr->activated;
+    return WG_OK;
+}
+
+// A dense pull covers every edge: any queued blocks are dropped.
+static int begin_pull(wg_ctx* ctx, int mode)
+{
+    int rc = reset_counters(ctx, RST_ACTIVITY);
+    if (rc)
+        return rc;
+    if (mode == WG_MODE_DENSE && ctx->wedge_log2 >= 0)
+        rc = clear_wedge_state(ctx);
+    return rc;
+}
+
+static int end_frontier_pull(wg_ctx* ctx)
+{
+    // the queue has been consumed (its wedge words were zeroed by the pull)
+    ctx->wedge_log2 = -1;
+    return reset_counters(ctx, RST_QUEUE);
+}
+
+extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated)
+{
+    if (!ctx || ctx->app != WG_APP_BFS)
+        return WG_E_STATE;
+    if (mode != WG_MODE_DENSE && mode != WG_MODE_FRONTIER)
+        return WG_E_INVAL;
+    if (mode == WG_MODE_FRONTIER && ctx->wedge_log2 < 0)
+        return WG_E_STATE;
+    int rc = begin_pull(ctx, mode);
+    if (rc)
+        return rc;
+    const uint32_t nd = (uint32_t)ctx->level + 1u;
+    uint64_t* packed = (uint64_t*)ctx->prop[0];
+    if (mode == WG_MODE_DENSE) {
+        int slot = wg_prof_begin(ctx, WG_K_BFS_DENSE);
+        hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+                           ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
+                           (uint32_t)ctx->V, nd, ctx->ctr);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+        wg_prof_end(ctx, WG_K_BFS_DENSE, slot);
+        if (ctx->nbfs_heavy) {
+            slot = wg_prof_begin(ctx, WG_K_BFS_HEAVY);
+            hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+                               ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
+                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, nd, ctx->ctr);
+            if ((rc = wg_check_launch(ctx)))
+                return rc;
+            wg_prof_end(ctx, WG_K_BFS_HEAVY, slot);
+        }
+    } else {
+        int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
+        hipLaunchKernelGGL(k_bfs_blocks, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
+                           ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
+                           ctx->bm_next, packed, ctx->wedge, nd);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+        wg_prof_end(ctx, WG_K_BFS_BLOCKS, slot);
+        if ((rc = end_frontier_pull(ctx)))
+            return rc;
+    }
+    return finish_pull(ctx, edges_activated);
+}
+
+template <class P>
+static int min_pull(wg_ctx* ctx, int mode, void* prop)
+{
+    int rc;
+    if (mode == WG_MODE_DENSE) {
+        int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
+        hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
+                           (typename P::T*)prop, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
+                           ctx->ctr, &ctx->ctr->overflow);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+        wg_prof_end(ctx, WG_K_MIN_TILE, slot);
+    } else {
+        int slot = wg_prof_begin(ctx, WG_K_MIN_BLOCKS);
+        hipLaunchKernelGGL(k_min_blocks<P>, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
+                           ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (typename P::T*)prop, ctx->od,
+                           ctx->bm_next, ctx->wedge, &ctx->ctr->overflow);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+        wg_prof_end(ctx, WG_K_MIN_BLOCKS, slot);
+        if ((rc = end_frontier_pull(ctx)))
+            return rc;
+    }
+    return WG_OK;
+}
+
+extern "C" int wg_edge_pull_cc(wg_ctx* ctx, int mode, uint64_t* edges_activated)
+{
+    if (!ctx || ctx->app != WG_APP_CC)
+        return WG_E_STATE;
+    if (mode != WG_MODE_DENSE && mode != WG_MODE_FRONTIER)
+        return WG_E_INVAL;
+    if (mode == WG_MODE_FRONTIER && ctx->wedge_log2 < 0)
+        return WG_E_STATE;
+    int rc = begin_pull(ctx, mode);
+    if (rc || (rc = min_pull<PolCC>(ctx, mode, ctx->prop[0])))
+        return rc;
+    return finish_pull(ctx, edges_activated);
+}
+
+extern "C" int wg_edge_pull_sssp(wg_ctx* ctx, int mode, uint64_t* edges_activated)
+{
+    if (!ctx || ctx->app != WG_APP_SSSP)
+        return WG_E_STATE;
+    if (mode != WG_MODE_DENSE && mode != WG_MODE_FRONTIER)
+        return WG_E_INVAL;
+    if (mode == WG_MODE_FRONTIER && ctx->wedge_log2 < 0)
+        return WG_E_STATE;
+    int rc = begin_pull(ctx, mode);
+    if (rc)
+        return rc;
+    rc = (ctx->wkind == WG_WEIGHT_F64) ? min_pull<PolSF>(ctx, mode, ctx->prop[0]) : min_pull<PolSU>(ctx, mode, ctx->prop[0]);
+    if (rc)
+        return rc;
+    return finish_pull(ctx, edges_activated);
+}
+
+extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
+{
+    if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
+        return WG_E_STATE;
+    int rc = reset_counters(ctx, RST_FRONTIER);
+    if (rc)
+        return rc;
+    int slot = wg_prof_begin(ctx, WG_K_ADVANCE);
+    const uint32_t grid = wg_grid(ctx, ctx->bm_words, 1024, 8);
+    if (ctx->app == WG_APP_BFS) {
+        hipLaunchKernelGGL(k_bfs_advance, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
+                           ctx->bm_cur, ctx->bm_words, &ctx->ctr->frontier_verts);
+        uint64_t* t = ctx->bm_cur;
+        ctx->bm_cur = ctx->bm_next;
+        ctx->bm_next = t;
+        ctx->level += 1;
+    } else {
+        hipLaunchKernelGGL(k_count_clear, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_next, ctx->bm_words,
+                           &ctx->ctr->frontier_verts);
+    }
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    wg_prof_end(ctx, WG_K_ADVANCE, slot);
+    if (frontier_vertices) {
+        if ((rc = read_counters(ctx)))
+            return rc;
+        *frontier_vertices = ctx->hctr->frontier_verts;
+    }
+    return WG_OK;
+}
+
+static int read_prop(wg_ctx* ctx, int app, void* host, size_t esz)
+{
+    if (!ctx || ctx->app != app || !host)
+        return WG_E_STATE;
+    WG_HIP(ctx, hipMemcpyAsync(host, ctx->prop[0], ctx->V * esz, hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
+extern "C" int wg_read_bfs(wg_ctx* ctx, uint64_t* host_packed)
+{
+    return read_prop(ctx, WG_APP_BFS, host_packed, sizeof(uint64_t));
+}
+
+extern "C" int wg_read_cc(wg_ctx* ctx, uint32_t* host_labels)
+{
+    return read_prop(ctx, WG_APP_CC, host_labels, sizeof(uint32_t));
+}
+
+extern "C" int wg_read_sssp(wg_ctx* ctx, void* host_dist)
+{
+    if (!ctx)
+        return WG_E_INVAL;
+    return read_prop(ctx, WG_APP_SSSP, host_dist, ctx->wkind == WG_WEIGHT_F64 ? sizeof(uint64_t) : sizeof(uint32_t));
+}
+
+extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
+{
+    if (!ctx || (ctx->app != WG_APP_BFS && ctx->app != WG_APP_SSSP && ctx->app != WG_APP_CC))
+        return WG_E_STATE;
+    WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->reached_verts, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    const uint32_t grid = wg_grid(ctx, ctx->V, 1024, 8);
+    if (ctx->app == WG_APP_BFS)
+        hipLaunchKernelGGL(k_reached_bfs, dim3(grid), dim3(256), 0, ctx->stream, (const uint64_t*)ctx->prop[0], ctx->od,
+                           ctx->V, ctx->ctr);
+    else if (ctx->app == WG_APP_CC)
+        hipLaunchKernelGGL(k_reached_min<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->prop[0],
+                           0xffffffffu, ctx->od, ctx->V, ctx->ctr);
+    else if (ctx->wkind == WG_WEIGHT_F64)
+        hipLaunchKernelGGL(k_reached_min<unsigned long long>, dim3(grid), dim3(256), 0, ctx->stream,
+                           (const unsigned long long*)ctx->prop[0], PolSF::INF, ctx->od, ctx->V, ctx->ctr);
+    else
+        hipLaunchKernelGGL(k_reached_min<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->prop[0],
+                           PolSU::INF, ctx->od, ctx->V, ctx->ctr);
+    int rc = wg_check_launch(ctx);
+    if (rc || (rc = read_counters(ctx)))
+        return rc;
+    if (vertices)
+        *vertices = ctx->hctr->reached_verts;
+    if (edges)
+        *edges = ctx->hctr->reached_edges;
+    return WG_OK;
+}

@@ -1,0 +1,29 @@
+/*
+ * wg_sched.c -- schedule choice for the converging applications.
+ */
+#include "wg_sched.h"
+
+#include "wg_mlp.h"
+
+void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_log2, const wg_sched_features* f,
+                     int* mode, int* block_log2)
+{
+    *block_log2 = plain_block_log2;
+    switch (o->schedule) {
+    case WG_SCHED_DENSE:
+        *mode = WG_MODE_DENSE;
+        return;
+    case WG_SCHED_FRONTIER:
+        *mode = WG_MODE_FRONTIER;
+        return;
+    case WG_SCHED_LEARNED:
+        if (wg_mlp_decide(f, mode, block_log2) == WG_OK)
+            return;
+        /* no model loaded: the plain heuristic is the fallback */
+        /* fall through */
+    default:
+        /* source/execution_bfs.c:141 */
+        *mode = (f->frontier_edges > threshold) ? WG_MODE_DENSE : WG_MODE_FRONTIER;
+        return;
+    }
+}
