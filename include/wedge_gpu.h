@@ -84,6 +84,12 @@ int wg_last_hip_error(const wg_ctx* ctx);
 /* Waits for all work queued on the context's stream. */
 int wg_sync(wg_ctx* ctx);
 
+/* Context options (set before wg_load_csc). */
+#define WG_OPT_RELABEL 1  /* 1 (default): store vertices in descending total-degree
+                             order on the device; every id and value crossing this
+                             API stays in the caller's original numbering */
+int wg_set_option(wg_ctx* ctx, int option, int64_t value);
+
 /* Uploads a destination-grouped CSC and builds every derived device structure:
  * out-degrees, per-edge destination column, the source-to-edge index (the
  * reference's CSRI), the PageRank edge tiles.  offsets has V+1 entries with

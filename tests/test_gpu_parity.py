@@ -19,9 +19,10 @@ PR_TOL = 1e-10  # L1, FP64 contributions
 PR_TOL_FP32 = 1e-5  # L1, FP32-stored contributions (expected ~1e-8..1e-7)
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = W.Context(0)
+@pytest.fixture(scope="module", params=[True, False], ids=["relabel", "native-order"])
+def ctx(request):
+    """Every test runs with the device-side degree relabelling on (default) and off."""
+    c = W.Context(0, relabel=request.param)
     yield c
     c.close()
 

@@ -50,6 +50,8 @@ KERNELS = {
     "advance": 9, "pr_acc": 10,
 }
 
+OPT_RELABEL = 1
+
 UNREACHED_U64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 UNREACHED_U32 = np.uint32(0xFFFFFFFF)
 
@@ -98,6 +100,7 @@ GPU_SYMBOLS = [
     ("wg_error_string", C.c_char_p, [C.c_int]),
     ("wg_last_hip_error", C.c_int, [_p]),
     ("wg_sync", C.c_int, [_p]),
+    ("wg_set_option", C.c_int, [_p, C.c_int, C.c_int64]),
     ("wg_load_csc", C.c_int, [_p, C.c_uint64, C.c_uint64, _u32p, _u32p, _p, C.c_int]),
     ("wg_graph_info", C.c_int, [_p, _u64p, _u64p]),
     ("wg_read_outdegrees", C.c_int, [_p, _u32p]),
@@ -365,13 +368,18 @@ def run_opts(**kw) -> RunOpts:
 class Context:
     """One HIP device context holding one graph (include/wedge_gpu.h)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, relabel: bool = True):
         self._lib = _lib_gpu()
         _lib_host()
         h = _p()
         _check(self._lib.wg_open(C.byref(h), device), "wg_open")
         self._h = h
         self.graph: Graph | None = None
+        self.set_option(OPT_RELABEL, int(relabel))
+
+    def set_option(self, option: int, value: int):
+        """Context option (takes effect at the next load)."""
+        self._chk(self._lib.wg_set_option(self._h, option, value), "wg_set_option")
 
     def close(self):
         if getattr(self, "_h", None):

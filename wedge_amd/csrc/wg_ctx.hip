@@ -30,19 +30,16 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t value)
 }
 
 // Destination column: a block walks one tile; light tiles binary-search the
-// tile's vertex window, heavy tiles belong to one vertex.  Also checks that
-// sources are in range and ascending within each destination.
+// tile's vertex window, heavy tiles belong to one vertex.
 __global__ __launch_bounds__(WG_BLOCK) void k_build_dst(const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ tile_v,
                                                          const uint32_t* __restrict__ tile_e, uint32_t ntiles,
-                                                         uint32_t* __restrict__ dst, uint32_t V,
-                                                         unsigned int* __restrict__ bad)
+                                                         uint32_t* __restrict__ dst)
 {
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t v0 = tile_v[t], v1 = tile_v[t + 1];
+        const bool heavy = (tile_v[t] & WG_HEAVY_BIT) != 0u;
+        const uint32_t v0 = tile_v[t] & ~WG_HEAVY_BIT, v1 = tile_v[t + 1] & ~WG_HEAVY_BIT;
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
-        const bool heavy = (off[v0 + 1] - off[v0]) > WG_TILE_EDGES;
         for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
             uint32_t d = v0;
             if (!heavy) {
@@ -57,13 +54,30 @@ __global__ __launch_bounds__(WG_BLOCK) void k_build_dst(const uint32_t* __restri
                 d = lo;
             }
             dst[e] = d;
-            const uint32_t s = src[e];
-            bool ok = s < V;
-            if (e + 1 < off[d + 1])
-                ok = ok && (s <= src[e + 1]);
-            if (!ok)
-                atomicOr(bad, 1u);
         }
+    }
+}
+
+// Input check on the caller's CSC: sources in range and ascending within each
+// destination.
+__global__ void k_validate(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t V, uint64_t E,
+                           unsigned int* __restrict__ bad)
+{
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < E; p += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = V - 1;
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= p)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t s = src[p];
+        bool ok = s < V;
+        if (p + 1 < off[lo + 1])
+            ok = ok && s <= src[p + 1];
+        if (!ok)
+            atomicOr(bad, 1u);
     }
 }
 
@@ -80,6 +94,61 @@ __global__ void k_csri_scatter(const uint32_t* __restrict__ src, uint64_t E, uin
         const uint32_t at = atomicAdd(&cursor[src[i]], 1u);
         pos[at] = (uint32_t)i;
     }
+}
+
+// ---- degree relabelling ------------------------------------------------------------
+
+__global__ void k_degree_key(const uint32_t* __restrict__ off, const uint32_t* __restrict__ od, uint64_t V,
+                             uint32_t* __restrict__ key, uint32_t* __restrict__ ids)
+{
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = (uint64_t)(off[v + 1] - off[v]) + od[v];
+        key[v] = d > 0xffffffffull ? 0xffffffffu : (uint32_t)d;
+        ids[v] = (uint32_t)v;
+    }
+}
+
+// inv[perm[i]] = i; nin[i] = in-degree of the vertex now at position i
+__global__ void k_invert(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ off, uint64_t V,
+                         uint32_t* __restrict__ inv, uint32_t* __restrict__ nin)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < V; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = perm[i];
+        inv[o] = (uint32_t)i;
+        nin[i] = off[o + 1] - off[o];
+    }
+}
+
+// New CSC: position p of the relabelled layout takes the edge at the same rank
+// of the same (original) destination; sources are renamed, their original
+// order inside each destination is kept.
+template <typename W>
+__global__ void k_relabel_edges(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ perm,
+                                const uint32_t* __restrict__ ooff, const uint32_t* __restrict__ osrc,
+                                const W* __restrict__ ow, const uint32_t* __restrict__ inv, uint64_t V, uint64_t E,
+                                uint32_t* __restrict__ nsrc, W* __restrict__ nw)
+{
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < E; p += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = V - 1; // largest i with noff[i] <= p
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) >> 1;
+            if (noff[mid] <= p)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint64_t op = ooff[perm[lo]] + (p - noff[lo]);
+        nsrc[p] = inv[osrc[op]];
+        if (nw)
+            nw[p] = ow[op];
+    }
+}
+
+template <typename T>
+__global__ void k_to_original(const T* __restrict__ in, const uint32_t* __restrict__ inv, uint64_t V, T* __restrict__ out)
+{
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = in[inv[v]];
 }
 
 // ---- helpers shared with other translation units -------------------------------
@@ -197,6 +266,62 @@ extern "C" int wg_profile_reset(wg_ctx* ctx)
     return WG_OK;
 }
 
+
+extern "C" int wg_set_option(wg_ctx* ctx, int option, int64_t value)
+{
+    if (!ctx)
+        return WG_E_INVAL;
+    switch (option) {
+    case WG_OPT_RELABEL:
+        ctx->relabel = value ? 1 : 0;
+        return WG_OK;
+    default:
+        return WG_E_INVAL;
+    }
+}
+
+int wg_to_device_id(wg_ctx* ctx, uint64_t orig, uint32_t* out)
+{
+    if (orig >= ctx->V)
+        return WG_E_RANGE;
+    if (!ctx->inv) {
+        *out = (uint32_t)orig;
+        return WG_OK;
+    }
+    uint32_t v = 0;
+    WG_HIP(ctx, hipMemcpyAsync(&v, ctx->inv + orig, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = v;
+    return WG_OK;
+}
+
+int wg_read_vertex_array(wg_ctx* ctx, const void* dev, void* host, size_t esz)
+{
+    if (!ctx->inv) {
+        WG_HIP(ctx, hipMemcpyAsync(host, dev, ctx->V * esz, hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return WG_OK;
+    }
+    void* tmp = nullptr;
+    WG_HIP(ctx, hipMalloc(&tmp, ctx->V * esz));
+    const uint32_t grid = wg_grid(ctx, ctx->V, 1024, 8);
+    if (esz == 8)
+        hipLaunchKernelGGL(k_to_original<uint64_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint64_t*)dev, ctx->inv,
+                           ctx->V, (uint64_t*)tmp);
+    else
+        hipLaunchKernelGGL(k_to_original<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)dev, ctx->inv,
+                           ctx->V, (uint32_t*)tmp);
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipMemcpyAsync(host, tmp, ctx->V * esz, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e3 = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tmp);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        ctx->last_hip_error = (int)(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3);
+        return WG_E_HIP;
+    }
+    return WG_OK;
+}
+
 // ---- context ----------------------------------------------------------------------
 
 extern "C" const char* wg_error_string(int code)
@@ -235,6 +360,7 @@ extern "C" int wg_open(wg_ctx** out, int device)
     if (!ctx)
         return WG_E_NOMEM;
     ctx->device = device;
+    ctx->relabel = 1;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         free(ctx);
         return WG_E_HIP;
@@ -299,6 +425,8 @@ static void free_graph(wg_ctx* ctx)
     free_dev((void**)&ctx->wedge);
     free_dev((void**)&ctx->queue);
     free_dev((void**)&ctx->heavy_list);
+    free_dev((void**)&ctx->perm);
+    free_dev((void**)&ctx->inv);
     ctx->V = ctx->E = 0;
     ctx->ntiles = ctx->nheavy = ctx->nbfs_heavy = 0;
 }
@@ -342,7 +470,7 @@ static void build_tiles(const uint32_t* off, uint64_t V, std::vector<uint32_t>& 
             hv.push_back((uint32_t)v);
             ht.push_back((uint32_t)tv.size());
             for (uint64_t e = off[v]; e < off[v + 1]; e += WG_TILE_EDGES) {
-                tv.push_back((uint32_t)v);
+                tv.push_back((uint32_t)v | WG_HEAVY_BIT);
                 te.push_back((uint32_t)e);
             }
             ++v;
@@ -381,12 +509,110 @@ static int upload(wg_ctx* ctx, T** dptr, const T* host, uint64_t n)
     return WG_OK;
 }
 
+// Replaces the device CSC (off/src/w/od, original numbering) by the same graph
+// with vertices in descending total-degree order (ties by original id), and
+// returns the new offsets on the host for tiling.
+static int relabel_graph(wg_ctx* ctx, std::vector<uint32_t>& hoff)
+{
+    const uint64_t V = ctx->V, E = ctx->E;
+    const size_t wsz = ctx->wkind == WG_WEIGHT_F64 ? 8 : ctx->wkind == WG_WEIGHT_U32 ? 4 : 0;
+    uint32_t *key = nullptr, *ids = nullptr, *key2 = nullptr, *perm = nullptr, *inv = nullptr, *nin = nullptr;
+    uint32_t *noff = nullptr, *nsrc = nullptr, *nod = nullptr;
+    void *nw = nullptr, *tmp = nullptr;
+    int rc = WG_OK;
+    const uint32_t gv = wg_grid(ctx, V, 1024, 8), ge = wg_grid(ctx, E, 1024, 8);
+    size_t b1 = 0, b2 = 0;
+    const uint32_t e32 = (uint32_t)E; // read by the stream before the synchronise below
+    hipError_t e = hipSuccess;
+#define RL(call)                          \
+    do {                                  \
+        if ((e = (call)) != hipSuccess) { \
+            rc = WG_E_HIP;                \
+            goto done;                    \
+        }                                 \
+    } while (0)
+    RL(hipMalloc((void**)&key, V * 4));
+    RL(hipMalloc((void**)&ids, V * 4));
+    RL(hipMalloc((void**)&key2, V * 4));
+    RL(hipMalloc((void**)&perm, V * 4));
+    RL(hipMalloc((void**)&inv, V * 4));
+    RL(hipMalloc((void**)&nin, V * 4));
+    RL(hipMalloc((void**)&noff, (V + 1) * 4));
+    RL(hipMalloc((void**)&nsrc, (E ? E : 1) * 4));
+    RL(hipMalloc((void**)&nod, V * 4));
+    if (wsz)
+        RL(hipMalloc(&nw, (E ? E : 1) * wsz));
+    hipLaunchKernelGGL(k_degree_key, dim3(gv), dim3(256), 0, ctx->stream, ctx->off, ctx->od, V, key, ids);
+    RL(hipGetLastError());
+    RL(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, b1, key, key2, ids, perm, (int)V, 0, 32, ctx->stream));
+    RL(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, nin, noff, (int)V, ctx->stream));
+    RL(hipMalloc(&tmp, (b1 > b2 ? b1 : b2) + 16));
+    RL(hipcub::DeviceRadixSort::SortPairsDescending(tmp, b1, key, key2, ids, perm, (int)V, 0, 32, ctx->stream));
+    hipLaunchKernelGGL(k_invert, dim3(gv), dim3(256), 0, ctx->stream, perm, ctx->off, V, inv, nin);
+    RL(hipGetLastError());
+    RL(hipcub::DeviceScan::ExclusiveSum(tmp, b2, nin, noff, (int)V, ctx->stream));
+    RL(hipMemcpyAsync(noff + V, &e32, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    if (E) {
+        if (wsz == 8)
+            hipLaunchKernelGGL(k_relabel_edges<uint64_t>, dim3(ge), dim3(256), 0, ctx->stream, noff, perm, ctx->off,
+                               ctx->src, (const uint64_t*)ctx->w, inv, V, E, nsrc, (uint64_t*)nw);
+        else if (wsz == 4)
+            hipLaunchKernelGGL(k_relabel_edges<uint32_t>, dim3(ge), dim3(256), 0, ctx->stream, noff, perm, ctx->off,
+                               ctx->src, (const uint32_t*)ctx->w, inv, V, E, nsrc, (uint32_t*)nw);
+        else
+            hipLaunchKernelGGL(k_relabel_edges<uint32_t>, dim3(ge), dim3(256), 0, ctx->stream, noff, perm, ctx->off,
+                               ctx->src, (const uint32_t*)nullptr, inv, V, E, nsrc, (uint32_t*)nullptr);
+        RL(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_to_original<uint32_t>, dim3(gv), dim3(256), 0, ctx->stream, ctx->od, perm, V, nod);
+    RL(hipGetLastError());
+    hoff.resize(V + 1);
+    RL(hipMemcpyAsync(hoff.data(), noff, (V + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    RL(hipStreamSynchronize(ctx->stream));
+    // swap in the relabelled arrays
+    (void)hipFree(ctx->off);
+    (void)hipFree(ctx->src);
+    (void)hipFree(ctx->od);
+    if (ctx->w)
+        (void)hipFree(ctx->w);
+    ctx->off = noff;
+    ctx->src = nsrc;
+    ctx->od = nod;
+    ctx->w = nw;
+    ctx->perm = perm;
+    ctx->inv = inv;
+    noff = nsrc = nod = perm = inv = nullptr;
+    nw = nullptr;
+done:
+#undef RL
+    if (rc)
+        ctx->last_hip_error = (int)e;
+    (void)hipFree(key);
+    (void)hipFree(ids);
+    (void)hipFree(key2);
+    (void)hipFree(nin);
+    (void)hipFree(tmp);
+    if (noff)
+        (void)hipFree(noff);
+    if (nsrc)
+        (void)hipFree(nsrc);
+    if (nod)
+        (void)hipFree(nod);
+    if (nw)
+        (void)hipFree(nw);
+    if (perm)
+        (void)hipFree(perm);
+    if (inv)
+        (void)hipFree(inv);
+    return rc;
+}
+
 extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* off, const uint32_t* src,
                            const void* weights, int weight_kind)
 {
     if (!ctx || !off || (E && !src) || V == 0)
         return WG_E_INVAL;
-    if (V >= 0xffffffffull || E >= 0xffffffffull)
+    if (V >= (uint64_t)WG_HEAVY_BIT || E >= 0xffffffffull)
         return WG_E_RANGE;
     if (weight_kind != WG_WEIGHT_NONE && weight_kind != WG_WEIGHT_F64 && weight_kind != WG_WEIGHT_U32)
         return WG_E_INVAL;
@@ -401,19 +627,15 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     free_graph(ctx);
 
-    std::vector<uint32_t> tv, te, hv, ht, bv, be;
-    tv.reserve(E / WG_TILE_EDGES + V / WG_TILE_VERTS + 16);
-    te.reserve(tv.capacity());
-    build_tiles(off, V, tv, te, hv, ht, bv, be);
-
     int rc;
+    std::vector<uint32_t> tv, te, hv, ht, bv, be;
+    std::vector<uint32_t> hoff;
+    const uint32_t* toff = off;
     ctx->V = V;
     ctx->E = E;
     ctx->wkind = weight_kind;
     if ((rc = upload(ctx, &ctx->off, off, V + 1)) || (rc = upload(ctx, &ctx->src, src, E)) ||
-        (rc = upload<uint32_t>(ctx, &ctx->dst, nullptr, E)) || (rc = upload<uint32_t>(ctx, &ctx->od, nullptr, V)) ||
-        (rc = upload<uint32_t>(ctx, &ctx->csri_off, nullptr, V + 1)) ||
-        (rc = upload<uint32_t>(ctx, &ctx->csri_pos, nullptr, E)))
+        (rc = upload<uint32_t>(ctx, &ctx->od, nullptr, V)))
         goto fail;
     if (weight_kind == WG_WEIGHT_F64) {
         if ((rc = upload(ctx, (double**)&ctx->w, (const double*)weights, E)))
@@ -422,6 +644,42 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
         if ((rc = upload(ctx, (uint32_t**)&ctx->w, (const uint32_t*)weights, E)))
             goto fail;
     }
+    {
+        // validate the caller's CSC, then out-degrees in the original numbering
+        unsigned int* bad = &ctx->ctr->overflow;
+        WG_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
+        if (E) {
+            hipLaunchKernelGGL(k_validate, dim3(wg_grid(ctx, E, 1024, 8)), dim3(256), 0, ctx->stream, ctx->off, ctx->src,
+                               V, E, bad);
+            if ((rc = wg_check_launch(ctx)))
+                goto fail;
+        }
+        unsigned int hbad = 0;
+        WG_HIP(ctx, hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipMemsetAsync(ctx->od, 0, V * sizeof(uint32_t), ctx->stream));
+        if (E) {
+            hipLaunchKernelGGL(k_outdeg, dim3(wg_grid(ctx, E, 1024, 8)), dim3(256), 0, ctx->stream, ctx->src, E, ctx->od);
+            if ((rc = wg_check_launch(ctx)))
+                goto fail;
+        }
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (hbad) {
+            rc = WG_E_FORMAT;
+            goto fail;
+        }
+    }
+    if (ctx->relabel && V > 1) {
+        if ((rc = relabel_graph(ctx, hoff)))
+            goto fail;
+        toff = hoff.data();
+    }
+    tv.reserve(E / WG_TILE_EDGES + V / WG_TILE_VERTS + 16);
+    te.reserve(tv.capacity());
+    build_tiles(toff, V, tv, te, hv, ht, bv, be);
+    if ((rc = upload<uint32_t>(ctx, &ctx->dst, nullptr, E)) ||
+        (rc = upload<uint32_t>(ctx, &ctx->csri_off, nullptr, V + 1)) ||
+        (rc = upload<uint32_t>(ctx, &ctx->csri_pos, nullptr, E)))
+        goto fail;
     ctx->ntiles = (uint32_t)(tv.size() - 1);
     ctx->nheavy = (uint32_t)hv.size();
     ctx->nbfs_heavy = (uint32_t)bv.size();
@@ -432,29 +690,13 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
         goto fail;
 
     {
-        // destination column + validation
-        unsigned int* bad = &ctx->ctr->overflow;
-        WG_HIP(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
         if (E) {
             hipLaunchKernelGGL(k_build_dst, dim3(wg_grid(ctx, ctx->ntiles, 1, 16)), dim3(WG_BLOCK), 0, ctx->stream,
-                               ctx->off, ctx->src, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->dst, (uint32_t)V, bad);
+                               ctx->off, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->dst);
             if ((rc = wg_check_launch(ctx)))
                 goto fail;
         }
-        unsigned int hbad = 0;
-        WG_HIP(ctx, hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream));
-        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (hbad) {
-            rc = WG_E_FORMAT;
-            goto fail;
-        }
-        // out-degrees and the source -> edge-position index
-        WG_HIP(ctx, hipMemsetAsync(ctx->od, 0, V * sizeof(uint32_t), ctx->stream));
-        if (E) {
-            hipLaunchKernelGGL(k_outdeg, dim3(wg_grid(ctx, E, 1024, 8)), dim3(256), 0, ctx->stream, ctx->src, E, ctx->od);
-            if ((rc = wg_check_launch(ctx)))
-                goto fail;
-        }
+        // the source -> edge-position index (the reference's CSRI)
         size_t tmp_bytes = 0;
         WG_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, ctx->od, ctx->csri_off, (int)V, ctx->stream));
         void* tmp = nullptr;
@@ -495,7 +737,7 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
     ctx->bm_words = (V + 63) / 64;
     ctx->queue_cap = (E >> WG_MIN_BLOCK_LOG2) + 2;
     ctx->wedge_words = (ctx->queue_cap + 31) / 32 + 1;
-    ctx->pr_grid = wg_grid(ctx, ctx->ntiles, 1, 8);
+    ctx->pr_grid = wg_grid(ctx, ctx->ntiles, WG_WPB, 8);
     ctx->span_grid = wg_grid(ctx, ctx->nheavy, 4, 4);
     if ((rc = upload<uint64_t>(ctx, &ctx->bm_cur, nullptr, ctx->bm_words)) ||
         (rc = upload<uint64_t>(ctx, &ctx->bm_next, nullptr, ctx->bm_words)) ||
@@ -532,19 +774,19 @@ extern "C" int wg_read_outdegrees(wg_ctx* ctx, uint32_t* host_out)
 {
     if (!ctx || !host_out || !ctx->od)
         return WG_E_INVAL;
-    WG_HIP(ctx, hipMemcpyAsync(host_out, ctx->od, ctx->V * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return WG_OK;
+    return wg_read_vertex_array(ctx, ctx->od, host_out, sizeof(uint32_t));
 }
 
 extern "C" int wg_vertex_outdegree(wg_ctx* ctx, uint64_t vertex, uint32_t* out)
 {
     if (!ctx || !out || !ctx->od)
         return WG_E_INVAL;
-    if (vertex >= ctx->V)
-        return WG_E_RANGE;
+    uint32_t dv = 0;
+    int rc = wg_to_device_id(ctx, vertex, &dv);
+    if (rc)
+        return rc;
     uint32_t v = 0;
-    WG_HIP(ctx, hipMemcpyAsync(&v, ctx->od + vertex, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(ctx, hipMemcpyAsync(&v, ctx->od + dv, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = v;
     return WG_OK;

@@ -10,11 +10,12 @@
 //   csri_off[V+1],  source -> positions of its out-edges in src[]
 //   csri_pos[E]     (the reference's CSRI, source/graphdata.c:493-585, but holding
 //                   edge positions so any block granularity can be derived)
-//   tile_v[T+1],    edge tiles for the dense pulls: a light tile owns whole
-//   tile_e[T+1]     destinations [tile_v[t], tile_v[t+1]) with <= TILE_EDGES
-//                   edges and <= TILE_VERTS vertices; a destination with more
-//                   than TILE_EDGES in-edges is cut into heavy tiles of
-//                   TILE_EDGES edges that all carry tile_v[t] = that vertex
+//   tile_v[T+1],    edge tiles for the dense pulls, one wavefront per tile: a
+//   tile_e[T+1]     light tile owns whole destinations [tile_v[t], tile_v[t+1])
+//                   with <= TILE_EDGES edges and <= TILE_VERTS vertices; a
+//                   destination with more than TILE_EDGES in-edges is cut into
+//                   heavy tiles of TILE_EDGES edges whose tile_v word is that
+//                   vertex with WG_HEAVY_BIT set
 //   heavy_v[H], heavy_t[H]  heavy destinations and their first tile
 #pragma once
 
@@ -26,9 +27,11 @@
 
 #define WG_WAVE 64
 #define WG_BLOCK 256
-#define WG_IPT 8                              // edges per thread in a tile
-#define WG_TILE_EDGES (WG_BLOCK * WG_IPT)     // 2048
-#define WG_TILE_VERTS 2048                    // destinations per light tile
+#define WG_WPB (WG_BLOCK / WG_WAVE)           // wavefronts per block
+#define WG_IPT 8                              // edges per lane in a tile
+#define WG_TILE_EDGES (WG_WAVE * WG_IPT)      // 512 edges per wavefront tile
+#define WG_TILE_VERTS 256                     // destinations per light tile
+#define WG_HEAVY_BIT 0x80000000u              // tile_v flag: heavy tile
 #define WG_BFS_SEQ 16                         // edges a lane scans alone in dense BFS
 #define WG_BFS_HEAVY 2048                     // in-degree above which dense BFS splits a vertex
 
@@ -84,6 +87,9 @@ struct wg_ctx {
     uint32_t* bfs_heavy_v;
     uint32_t* bfs_heavy_e;    // chunk start edge (chunk end = min(start + WG_BFS_HEAVY, off[v+1]))
     uint32_t n_out_pos;       // vertices with out-degree > 0
+    int relabel;              // option: store vertices in descending-degree order
+    uint32_t* perm;           // device id -> original id (NULL when not relabelled)
+    uint32_t* inv;            // original id -> device id (NULL when not relabelled)
 
     // application state
     int app;
@@ -176,6 +182,25 @@ __device__ __forceinline__ T wg_wave_sum(T v)
     return v;
 }
 
+template <typename T>
+__device__ __forceinline__ T wg_wave_min(T v)
+{
+    for (int o = WG_WAVE / 2; o > 0; o >>= 1) {
+        const T x = __shfl_xor(v, o, WG_WAVE);
+        v = x < v ? x : v;
+    }
+    return v;
+}
+
+// Orders this wavefront's LDS writes before its following LDS reads by other
+// lanes (no block-wide barrier: wavefront tiles are independent).
+__device__ __forceinline__ void wg_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // internal entry points shared across translation units
 int wg_pr_reset(wg_ctx* ctx);
 int wg_free_app_state(wg_ctx* ctx);
@@ -183,3 +208,7 @@ int wg_alloc_bitmasks(wg_ctx* ctx);
 int wg_clear_u64(wg_ctx* ctx, uint64_t* p, uint64_t words);
 int wg_fill_u32(wg_ctx* ctx, uint32_t* p, uint64_t n, uint32_t value);
 int wg_fill_u64(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t value);
+// original vertex id -> device vertex id (identity without relabelling)
+int wg_to_device_id(wg_ctx* ctx, uint64_t orig, uint32_t* out);
+// copies V values stored in device order to host memory in original order
+int wg_read_vertex_array(wg_ctx* ctx, const void* dev, void* host, size_t elem_size);

@@ -7,17 +7,17 @@
 //     contrib'[v] = ((acc[v] + (1 - S) / N) * d + (1 - d) / N) / od'[v],
 //     od'[v] = N if od[v] == 0, S = sum of all accumulators.
 //
-// MI355X design.  One workgroup (4 waves) processes one edge tile at a time
-// (grid-stride over tiles).  A light tile holds whole destinations (<= 2048
-// edges, <= 2048 vertices): each thread sums 8 consecutive edges, a
-// destination that lies inside one thread is written to LDS by that thread,
-// a destination that spans threads is summed in thread order from per-thread
-// first/last pieces, and the block then runs the vertex phase for the tile's
-// vertices with coalesced reads of od[] and writes of contrib[].  All sums
-// are taken in a fixed order, so results are bit-reproducible run to run.
-// Destinations with more than 2048 in-edges are cut into heavy tiles whose
-// partial sums are added in tile order by a second small kernel, which also
-// reduces the per-block partial sums of S.
+// MI355X design.  One wavefront processes one edge tile at a time (grid-stride
+// over tiles).  A light tile holds whole destinations (<= 512 edges, <= 256
+// vertices): each lane sums 8 consecutive edges, a destination that lies
+// inside one lane is written to LDS by that lane, a destination that spans
+// lanes is summed in lane order from per-lane first/last pieces, and the
+// wavefront then runs the vertex phase for the tile's vertices with coalesced
+// reads of od[] and writes of contrib[].  All sums are taken in a fixed order,
+// so results are bit-reproducible run to run.  Destinations with more than 512
+// in-edges are cut into heavy tiles whose partial sums are added in tile
+// order by a second small kernel, which also reduces the per-block partial
+// sums of S.
 //
 // The fused iteration needs S = sum of accumulators before any accumulator of
 // the iteration exists.  It uses the identity
@@ -38,7 +38,7 @@ __device__ __forceinline__ double block_sum_det(double v, double* s_red)
         s_red[wave] = v;
     __syncthreads();
     double r = 0.0;
-    for (int i = 0; i < WG_BLOCK / WG_WAVE; ++i)
+    for (int i = 0; i < WG_WPB; ++i)
         r += s_red[i];
     return r;
 }
@@ -51,6 +51,12 @@ __device__ __forceinline__ double load_contrib(const T* __restrict__ p, uint32_t
 
 // MODE 0: fused iteration (contrib_out + partial S of ranks)
 // MODE 1: edge phase only (acc_out + partial sum of accumulators)
+//
+// One wavefront per tile, tiles taken grid-stride.  All global loads of a
+// tile (its sources, offset window and out-degrees) are issued before the
+// first dependent use, the source stream is read non-temporally (it is read
+// once per iteration and should not displace contributions from L2), and
+// wavefronts never wait for each other.
 template <typename T, int MODE>
 __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                        const T* __restrict__ cin, T* __restrict__ cout,
@@ -60,13 +66,20 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
                                                        const double* __restrict__ S_in, double damping, double N,
                                                        double* __restrict__ tile_part, double* __restrict__ blk_part)
 {
-    __shared__ uint32_t s_off[WG_TILE_VERTS + 1];
-    __shared__ double s_acc[WG_TILE_VERTS];
-    __shared__ double s_fp[WG_BLOCK];
-    __shared__ double s_lp[WG_BLOCK];
-    __shared__ double s_red[WG_BLOCK / WG_WAVE];
+    constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
+    constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
+    __shared__ uint32_t s_off_all[WG_WPB][WG_TILE_VERTS + 1];
+    __shared__ double s_acc_all[WG_WPB][WG_TILE_VERTS];
+    __shared__ double s_fp_all[WG_WPB][WG_WAVE];
+    __shared__ double s_lp_all[WG_WPB][WG_WAVE];
+    __shared__ double s_red[WG_WPB];
 
-    const int tid = threadIdx.x;
+    const int wave = threadIdx.x / WG_WAVE;
+    const int lane = wg_lane();
+    uint32_t* s_off = s_off_all[wave];
+    double* s_acc = s_acc_all[wave];
+    double* s_fp = s_fp_all[wave];
+    double* s_lp = s_lp_all[wave];
     double corr = 0.0, omd_n = 0.0;
     if (MODE == 0) {
         corr = (1.0 - *S_in) / N;
@@ -74,43 +87,57 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
     }
     double s_part = 0.0;
 
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t v0 = tile_v[t];
+    const uint32_t stride = gridDim.x * WG_WPB;
+    for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
+        const uint32_t tv = tile_v[t];
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
-        const uint32_t a = e0 + (uint32_t)tid * WG_IPT;
-        const uint32_t deg0 = off[v0 + 1] - off[v0];
-        if (deg0 > WG_TILE_EDGES) {
-            // heavy tile: one destination, partial sum in fixed order
+        const uint32_t v0 = tv & ~WG_HEAVY_BIT;
+        const uint32_t a = e0 + (uint32_t)lane * WG_IPT;
+        uint32_t s_idx[WG_IPT];
+#pragma unroll
+        for (int j = 0; j < WG_IPT; ++j)
+            s_idx[j] = (a + j < e1) ? __builtin_nontemporal_load(&src[a + j]) : 0u;
+        if (tv & WG_HEAVY_BIT) {
+            // heavy tile: one destination, partial sum in a fixed order
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < WG_IPT; ++j) {
-                const uint32_t e = a + j;
-                if (e < e1)
-                    s += load_contrib(cin, src[e]);
-            }
-            s = block_sum_det(s, s_red);
-            if (tid == 0)
+            for (int j = 0; j < WG_IPT; ++j)
+                if (a + j < e1)
+                    s += load_contrib(cin, s_idx[j]);
+            s = wg_wave_sum(s);
+            if (lane == 0)
                 tile_part[t] = s;
-            __syncthreads();
             continue;
         }
-        const uint32_t nv = tile_v[t + 1] - v0;
-        for (uint32_t i = tid; i <= nv; i += WG_BLOCK)
-            s_off[i] = off[v0 + i];
-        for (uint32_t i = tid; i < nv; i += WG_BLOCK)
-            s_acc[i] = 0.0;
-        __syncthreads();
+        const uint32_t nv = (tile_v[t + 1] & ~WG_HEAVY_BIT) - v0;
+        uint32_t offw[NOFF];
+#pragma unroll
+        for (int k = 0; k < NOFF; ++k) {
+            const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            offw[k] = (i <= nv) ? off[v0 + i] : 0u;
+        }
+        uint32_t odw[NVTX];
+        if (MODE == 0) {
+#pragma unroll
+            for (int k = 0; k < NVTX; ++k) {
+                const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+                odw[k] = (i < nv) ? od[v0 + i] : 0u;
+            }
+        }
+        double g[WG_IPT];
+#pragma unroll
+        for (int j = 0; j < WG_IPT; ++j)
+            g[j] = (a + j < e1) ? load_contrib(cin, s_idx[j]) : 0.0;
+#pragma unroll
+        for (int k = 0; k < NOFF; ++k) {
+            const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            if (i <= nv)
+                s_off[i] = offw[k];
+        }
+        wg_wave_sync();
 
         double fp = 0.0, lp = 0.0;
         if (a < e1) {
-            uint32_t s_idx[WG_IPT];
-            double g[WG_IPT];
-#pragma unroll
-            for (int j = 0; j < WG_IPT; ++j)
-                s_idx[j] = (a + j < e1) ? src[a + j] : 0u;
-#pragma unroll
-            for (int j = 0; j < WG_IPT; ++j)
-                g[j] = (a + j < e1) ? load_contrib(cin, s_idx[j]) : 0.0;
             // owner of edge a: the largest window index with s_off[i] <= a
             uint32_t lo = 0, hi = nv - 1;
             while (lo < hi) {
@@ -122,7 +149,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
             }
             uint32_t d = lo;
             uint32_t dend = s_off[d + 1];
-            bool cont = s_off[d] < a; // this piece continues a destination from an earlier thread
+            bool cont = s_off[d] < a; // this piece continues a destination from an earlier lane
             bool first = true;
             double run = 0.0;
 #pragma unroll
@@ -154,41 +181,45 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
                     lp = run;
             }
         }
-        s_fp[tid] = fp;
-        s_lp[tid] = lp;
-        __syncthreads();
+        s_fp[lane] = fp;
+        s_lp[lane] = lp;
+        wg_wave_sync();
 
-        for (uint32_t i = tid; i < nv; i += WG_BLOCK) {
-            const uint32_t vs = s_off[i], ve = s_off[i + 1];
-            double accv = 0.0;
-            if (vs != ve) {
-                const uint32_t k0 = (vs - e0) / WG_IPT, k1 = (ve - 1 - e0) / WG_IPT;
-                if (k0 == k1) {
-                    accv = s_acc[i];
+#pragma unroll
+        for (int k = 0; k < NVTX; ++k) {
+            const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            if (i < nv) {
+                const uint32_t vs = s_off[i], ve = s_off[i + 1];
+                double accv = 0.0;
+                if (vs != ve) {
+                    const uint32_t k0 = (vs - e0) / WG_IPT, k1 = (ve - 1 - e0) / WG_IPT;
+                    if (k0 == k1) {
+                        accv = s_acc[i];
+                    } else {
+                        accv = s_lp[k0];
+                        for (uint32_t q = k0 + 1; q <= k1; ++q)
+                            accv += s_fp[q];
+                    }
+                }
+                const uint32_t v = v0 + i;
+                if (MODE == 0) {
+                    const uint32_t odv = odw[k];
+                    const double t1 = accv + corr;
+                    const double t2 = t1 * damping;
+                    const double rank = t2 + omd_n;
+                    cout[v] = (T)(rank / (odv ? (double)odv : N));
+                    if (odv)
+                        s_part += rank;
                 } else {
-                    accv = s_lp[k0];
-                    for (uint32_t k = k0 + 1; k <= k1; ++k)
-                        accv += s_fp[k];
+                    acc_out[v] = accv;
+                    s_part += accv;
                 }
             }
-            const uint32_t v = v0 + i;
-            if (MODE == 0) {
-                const uint32_t odv = od[v];
-                const double t1 = accv + corr;
-                const double t2 = t1 * damping;
-                const double rank = t2 + omd_n;
-                cout[v] = (T)(rank / (odv ? (double)odv : N));
-                if (odv)
-                    s_part += rank;
-            } else {
-                acc_out[v] = accv;
-                s_part += accv;
-            }
         }
-        __syncthreads();
+        wg_wave_sync(); // the next tile reuses this wavefront's LDS
     }
     s_part = block_sum_det(s_part, s_red);
-    if (tid == 0)
+    if (threadIdx.x == 0)
         blk_part[blockIdx.x] = s_part;
 }
 
@@ -213,7 +244,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_span(const uint32_t* __restrict
                                                        double* __restrict__ blk_part, uint32_t main_blocks,
                                                        double* __restrict__ S_out, unsigned int* __restrict__ ticket)
 {
-    __shared__ double s_red[WG_BLOCK / WG_WAVE];
+    __shared__ double s_red[WG_WPB];
     __shared__ bool s_last;
     const int lane = wg_lane();
     double corr = 0.0, omd_n = 0.0;
@@ -272,7 +303,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_vertex(const double* __restrict
                                                          double damping, double N, double* __restrict__ blk_part,
                                                          double* __restrict__ S_out, unsigned int* __restrict__ ticket)
 {
-    __shared__ double s_red[WG_BLOCK / WG_WAVE];
+    __shared__ double s_red[WG_WPB];
     __shared__ bool s_last;
     const double corr = (1.0 - *total) / N;
     const double omd_n = (1.0 - damping) / N;
@@ -473,13 +504,9 @@ extern "C" int wg_read_ranks(wg_ctx* ctx, double* host_ranks)
     else
         hipLaunchKernelGGL(k_pr_ranks<double>, dim3(grid), dim3(256), 0, ctx->stream, ctx->od,
                            (const double*)ctx->prop[ctx->pr_cur], tmp, ctx->V, N);
-    hipError_t e1 = hipGetLastError();
-    hipError_t e2 = hipMemcpyAsync(host_ranks, tmp, ctx->V * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
-    hipError_t e3 = hipStreamSynchronize(ctx->stream);
+    int rc = wg_check_launch(ctx);
+    if (rc == WG_OK)
+        rc = wg_read_vertex_array(ctx, tmp, host_ranks, sizeof(double));
     (void)hipFree(tmp);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-        ctx->last_hip_error = (int)(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3);
-        return WG_E_HIP;
-    }
-    return WG_OK;
+    return rc;
 }

@@ -106,7 +106,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
                                                          uint64_t* __restrict__ packed, uint32_t V, uint32_t newdepth,
-                                                         wg_dev_counters* __restrict__ ctr)
+                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr)
 {
     const int lane = wg_lane();
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
@@ -165,7 +165,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
         const bool disc = parent >= 0;
         const unsigned long long dm = wg_ballot(disc);
         if (disc) {
-            packed[v] = ((uint64_t)newdepth << 32) | (uint64_t)(uint32_t)parent;
+            const uint32_t pid = perm ? perm[(uint32_t)parent] : (uint32_t)parent;
+            packed[v] = ((uint64_t)newdepth << 32) | (uint64_t)pid;
             act.edges += od[v];
             act.verts += 1ull;
         }
@@ -181,7 +182,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
                                                          const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
                                                          uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
-                                                         uint32_t newdepth, wg_dev_counters* __restrict__ ctr)
+                                                         uint32_t newdepth, const uint32_t* __restrict__ perm,
+                                                         wg_dev_counters* __restrict__ ctr)
 {
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
@@ -194,8 +196,10 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
         const uint32_t vend = off[v + 1];
         const uint32_t ce = (cs + WG_BFS_HEAVY < vend) ? cs + WG_BFS_HEAVY : vend;
         // a lower chunk already found a parent smaller than anything here
+        // (parents are original ids; in-lists ascend in original id)
         const uint64_t pk = packed[v];
-        if ((uint32_t)(pk >> 32) == newdepth && (uint32_t)pk < src[cs])
+        const uint32_t first_id = perm ? perm[src[cs]] : src[cs];
+        if ((uint32_t)(pk >> 32) == newdepth && (uint32_t)pk < first_id)
             continue;
         int64_t found = -1;
         for (uint32_t p = cs; p < ce && found < 0; p += WG_WAVE) {
@@ -213,7 +217,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
             }
         }
         if (found >= 0 && lane == 0) {
-            atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)(uint32_t)found);
+            const uint32_t pid = perm ? perm[(uint32_t)found] : (uint32_t)found;
+            atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)pid);
             activate(next, od, v, act);
         }
     }
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restr
                                                           const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                           const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
                                                           uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
-                                                          uint32_t newdepth)
+                                                          uint32_t newdepth, const uint32_t* __restrict__ perm)
 {
     const int lane = wg_lane();
     const uint32_t qlen = ctr->queue_len;
@@ -277,7 +282,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restr
             const bool prev_same = ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) != 0u &&
                                    ((cm >> (lane - 1)) & 1ull) && pv == v;
             if (cand && !prev_same) {
-                atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)u);
+                const uint32_t pid = perm ? perm[u] : u;
+                atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)pid);
                 activate(next, od, v, act);
             }
         }
@@ -388,8 +394,11 @@ __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint
         activate(next, od, v, t);
 }
 
-// Dense pull over the edge tiles: per-destination minimum in LDS, then one
-// global update per changed destination.
+// Dense pull over the edge tiles, one wavefront per tile: per-destination
+// minimum in LDS, then one global update per changed destination.  Minimum is
+// order-independent, so pieces of a destination spread over lanes or tiles
+// meet through atomics (the reference's idempotent merge,
+// source/phases.c:54-65).
 template <class P>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                         const void* __restrict__ w, typename P::T* __restrict__ prop,
@@ -399,28 +408,48 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
                                                         unsigned int* __restrict__ ovf)
 {
     typedef typename P::T T;
-    __shared__ uint32_t s_off[WG_TILE_VERTS + 1];
-    __shared__ T s_min[WG_TILE_VERTS];
-    const int tid = threadIdx.x;
+    constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
+    constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
+    __shared__ uint32_t s_off_all[WG_WPB][WG_TILE_VERTS + 1];
+    __shared__ T s_min_all[WG_WPB][WG_TILE_VERTS];
+    const int wave = threadIdx.x / WG_WAVE;
+    const int lane = wg_lane();
+    uint32_t* s_off = s_off_all[wave];
+    T* s_min = s_min_all[wave];
     act_tally act = { 0ull, 0ull };
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const uint32_t v0 = tile_v[t];
+    const uint32_t stride = gridDim.x * WG_WPB;
+    for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
+        const uint32_t tv = tile_v[t];
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
-        const uint32_t a = e0 + (uint32_t)tid * WG_IPT;
-        const bool heavy = (off[v0 + 1] - off[v0]) > WG_TILE_EDGES;
-        const uint32_t nv = heavy ? 1u : tile_v[t + 1] - v0;
-        for (uint32_t i = tid; i <= nv; i += WG_BLOCK)
-            s_off[i] = heavy ? (i == 0 ? e0 : e1) : off[v0 + i];
-        for (uint32_t i = tid; i < nv; i += WG_BLOCK)
-            s_min[i] = P::INF;
-        __syncthreads();
-        if (a < e1) {
-            T g[WG_IPT];
+        const uint32_t v0 = tv & ~WG_HEAVY_BIT;
+        const uint32_t a = e0 + (uint32_t)lane * WG_IPT;
+        T g[WG_IPT];
 #pragma unroll
-            for (int j = 0; j < WG_IPT; ++j) {
-                const uint32_t e = a + j;
-                g[j] = (e < e1) ? P::val(prop, w, src[e], e, ovf) : P::INF;
-            }
+        for (int j = 0; j < WG_IPT; ++j) {
+            const uint32_t e = a + j;
+            g[j] = (e < e1) ? P::val(prop, w, __builtin_nontemporal_load(&src[e]), e, ovf) : P::INF;
+        }
+        if (tv & WG_HEAVY_BIT) {
+            T m = P::INF;
+#pragma unroll
+            for (int j = 0; j < WG_IPT; ++j)
+                m = g[j] < m ? g[j] : m;
+            m = wg_wave_min(m);
+            if (lane == 0 && m != P::INF)
+                min_apply<P>(prop, next, od, v0, m, act);
+            continue;
+        }
+        const uint32_t nv = (tile_v[t + 1] & ~WG_HEAVY_BIT) - v0;
+#pragma unroll
+        for (int k = 0; k < NOFF; ++k) {
+            const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            if (i <= nv)
+                s_off[i] = off[v0 + i];
+            if (i < nv)
+                s_min[i] = P::INF;
+        }
+        wg_wave_sync();
+        if (a < e1) {
             uint32_t lo = 0, hi = nv - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi + 1) >> 1;
@@ -451,13 +480,17 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
             if (run != P::INF && d < nv)
                 atomic_min_t<T>(&s_min[d], run);
         }
-        __syncthreads();
-        for (uint32_t i = tid; i < nv; i += WG_BLOCK) {
-            const T m = s_min[i];
-            if (m != P::INF)
-                min_apply<P>(prop, next, od, v0 + i, m, act);
+        wg_wave_sync();
+#pragma unroll
+        for (int k = 0; k < NVTX; ++k) {
+            const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            if (i < nv) {
+                const T m = s_min[i];
+                if (m != P::INF)
+                    min_apply<P>(prop, next, od, v0 + i, m, act);
+            }
         }
-        __syncthreads();
+        wg_wave_sync();
     }
     add_tally(act, ctr);
 }
@@ -602,10 +635,11 @@ __global__ void k_or_bit(uint64_t* bm, uint32_t v)
     bm[v >> 6] |= 1ull << (v & 63);
 }
 
-__global__ void k_iota_u32(uint32_t* p, uint64_t n)
+// label[v] = original id of v
+__global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* __restrict__ perm)
 {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = (uint32_t)i;
+        p[i] = perm ? perm[i] : (uint32_t)i;
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -650,14 +684,15 @@ extern "C" int wg_init_bfs(wg_ctx* ctx, uint64_t root)
     int rc = init_common(ctx, WG_APP_BFS);
     if (rc)
         return rc;
-    if (root >= ctx->V)
-        return WG_E_RANGE;
+    uint32_t r = 0;
+    if ((rc = wg_to_device_id(ctx, root, &r)))
+        return rc;
     WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
     if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, ~0ull)))
         return rc;
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], root, root);
-    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_cur, (uint32_t)root);
-    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_visited, (uint32_t)root);
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], (uint64_t)r, root);
+    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_cur, r);
+    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_visited, r);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -671,7 +706,7 @@ extern "C" int wg_init_cc(wg_ctx* ctx)
         return rc;
     WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint32_t)));
     hipLaunchKernelGGL(k_iota_u32, dim3(wg_grid(ctx, ctx->V, 1024, 8)), dim3(256), 0, ctx->stream, (uint32_t*)ctx->prop[0],
-                       ctx->V);
+                       ctx->V, ctx->perm);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -685,18 +720,19 @@ extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
     int rc = init_common(ctx, WG_APP_SSSP);
     if (rc)
         return rc;
-    if (root >= ctx->V)
-        return WG_E_RANGE;
+    uint32_t r = 0;
+    if ((rc = wg_to_device_id(ctx, root, &r)))
+        return rc;
     if (ctx->wkind == WG_WEIGHT_F64) {
         WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
         if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, PolSF::INF)))
             return rc;
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], root, 0ull);
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], (uint64_t)r, 0ull);
     } else {
         WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint32_t)));
         if ((rc = wg_fill_u32(ctx, (uint32_t*)ctx->prop[0], ctx->V, PolSU::INF)))
             return rc;
-        WG_HIP(ctx, hipMemsetAsync((uint32_t*)ctx->prop[0] + root, 0, sizeof(uint32_t), ctx->stream));
+        WG_HIP(ctx, hipMemsetAsync((uint32_t*)ctx->prop[0] + r, 0, sizeof(uint32_t), ctx->stream));
     }
     if ((rc = wg_check_launch(ctx)))
         return rc;
@@ -725,15 +761,14 @@ extern "C" int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int
 {
     if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
         return WG_E_STATE;
-    if (vertex >= ctx->V)
-        return WG_E_RANGE;
     if (!check_block_log2(block_log2))
         return WG_E_INVAL;
-    int rc = prepare_frontier_build(ctx, block_log2);
-    if (rc)
+    uint32_t dv = 0;
+    int rc = wg_to_device_id(ctx, vertex, &dv);
+    if (rc || (rc = prepare_frontier_build(ctx, block_log2)))
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
-    hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, (uint32_t)vertex, ctx->csri_off, ctx->csri_pos,
+    hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, dv, ctx->csri_off, ctx->csri_pos,
                        block_log2, ctx->wedge, ctx->queue, ctx->ctr);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
@@ -822,7 +857,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         int slot = wg_prof_begin(ctx, WG_K_BFS_DENSE);
         hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                            ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           (uint32_t)ctx->V, nd, ctx->ctr);
+                           (uint32_t)ctx->V, nd, ctx->perm, ctx->ctr);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_BFS_DENSE, slot);
@@ -830,7 +865,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
             slot = wg_prof_begin(ctx, WG_K_BFS_HEAVY);
             hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                                ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
-                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, nd, ctx->ctr);
+                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, nd, ctx->perm, ctx->ctr);
             if ((rc = wg_check_launch(ctx)))
                 return rc;
             wg_prof_end(ctx, WG_K_BFS_HEAVY, slot);
@@ -839,7 +874,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
         hipLaunchKernelGGL(k_bfs_blocks, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
-                           ctx->bm_next, packed, ctx->wedge, nd);
+                           ctx->bm_next, packed, ctx->wedge, nd, ctx->perm);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_BFS_BLOCKS, slot);
@@ -941,9 +976,7 @@ static int read_prop(wg_ctx* ctx, int app, void* host, size_t esz)
 {
     if (!ctx || ctx->app != app || !host)
         return WG_E_STATE;
-    WG_HIP(ctx, hipMemcpyAsync(host, ctx->prop[0], ctx->V * esz, hipMemcpyDeviceToHost, ctx->stream));
-    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return WG_OK;
+    return wg_read_vertex_array(ctx, ctx->prop[0], host, esz);
 }
 
 extern "C" int wg_read_bfs(wg_ctx* ctx, uint64_t* host_packed)
