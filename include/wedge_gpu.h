@@ -71,7 +71,8 @@ typedef struct wg_ctx wg_ctx;
 #define WG_K_WEDGE_BUILD    8
 #define WG_K_ADVANCE        9
 #define WG_K_PR_ACC        10  /* stand-alone PageRank edge pull (accumulators) */
-#define WG_K_COUNT         11
+#define WG_K_LOOP          11  /* whole device-resident iteration loop (wg_converge) */
+#define WG_K_COUNT         12
 
 /* ---- context -------------------------------------------------------------- */
 
@@ -172,6 +173,23 @@ int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int block_log2
  * source/execution_cc.c:243-277).  frontier_vertices (optional) = number of
  * vertices in the new frontier. */
 int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices);
+
+/* Runs the initialised BFS / CC / SSSP to convergence with the per-iteration
+ * schedule decided on the device (the reference's execution_impl_* loop,
+ * source/execution_bfs.c:122-365 and its cc/sssp twins): a one-thread decide
+ * kernel reads the activity count of each pull, stops at 0, and picks the
+ * next pull's mode (and Wedge granularity); every pull, frontier-build and
+ * advance kernel reads that choice and exits at once when not selected.  The
+ * host enqueues iterations in growing batches and never waits per iteration.
+ * schedule: 0 plain threshold rule, 1 learned (MLP on the device, plain rule
+ * if no model is set), 2 always dense, 3 always frontier. */
+typedef struct wg_loop_params {
+    int schedule;
+    uint32_t threshold_pct;  /* plain rule: dense iff activity > E * pct / 100 */
+    int block_log2;          /* Wedge granularity of the plain rule */
+    uint32_t max_iterations; /* 0 = until convergence */
+} wg_loop_params;
+int wg_converge(wg_ctx* ctx, const wg_loop_params* params, uint64_t* iterations, uint64_t* frontier_iterations);
 
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
  * the root holds (0 << 32) | root. */

@@ -118,9 +118,12 @@ def test_pr_deterministic(ctx):
 
 # ---- BFS -------------------------------------------------------------------------
 
+# device-resident loop (default) and the host-driven loop of the phase API
+# (record_trace=1), every pull mode and several Wedge granularities
 SCHEDULES = [dict(schedule="plain"), dict(schedule="dense"), dict(schedule="frontier"),
              dict(schedule="frontier", block_log2=3), dict(schedule="frontier", block_log2=6),
-             dict(schedule="plain", block_log2=8), dict(schedule="plain", threshold_pct=2)]
+             dict(schedule="plain", block_log2=8), dict(schedule="plain", threshold_pct=2),
+             dict(schedule="plain", record_trace=1), dict(schedule="frontier", block_log2=5, record_trace=1)]
 
 
 @pytest.mark.parametrize("kind", ["rmat", "rmat_sym", "grid"])

@@ -47,7 +47,7 @@ MAX_BLOCK_LOG2 = 10
 KERNELS = {
     "pr_tile": 0, "pr_span": 1, "pr_vertex": 2, "min_tile": 3, "min_blocks": 4,
     "bfs_dense": 5, "bfs_heavy": 6, "bfs_blocks": 7, "wedge_build": 8,
-    "advance": 9, "pr_acc": 10,
+    "advance": 9, "pr_acc": 10, "loop": 11,
 }
 
 OPT_RELABEL = 1
@@ -124,6 +124,7 @@ GPU_SYMBOLS = [
     ("wg_read_cc", C.c_int, [_p, _u32p]),
     ("wg_read_sssp", C.c_int, [_p, _p]),
     ("wg_reached", C.c_int, [_p, _u64p, _u64p]),
+    ("wg_converge", C.c_int, [_p, _p, _u64p, _u64p]),
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
@@ -148,6 +149,11 @@ class RunStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LoopParams(C.Structure):
+    _fields_ = [("schedule", C.c_int), ("threshold_pct", C.c_uint32), ("block_log2", C.c_int),
+                ("max_iterations", C.c_uint32)]
 
 
 class TraceRow(C.Structure):
@@ -486,6 +492,13 @@ class Context:
         n = C.c_uint64()
         self._chk(self._lib.wg_advance_frontier(self._h, C.byref(n)), "wg_advance_frontier")
         return int(n.value)
+
+    def converge(self, schedule=SCHED_PLAIN, threshold_pct=20, block_log2=DEFAULT_BLOCK_LOG2, max_iterations=0):
+        """Runs the initialised BFS/CC/SSSP to convergence in the device-resident loop."""
+        p = LoopParams(schedule, threshold_pct, block_log2, max_iterations)
+        it, fit = C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.wg_converge(self._h, C.byref(p), C.byref(it), C.byref(fit)), "wg_converge")
+        return int(it.value), int(fit.value)
 
     def read_bfs(self) -> np.ndarray:
         out = np.empty(self.V, np.uint64)

@@ -48,7 +48,40 @@ struct wg_dev_counters {
     double total_rank;                  // PR: sum of accumulators of the last iteration
     unsigned long long reached_verts;
     unsigned long long reached_edges;
+    // control block of the device-resident iteration loop (wg_converge)
+    unsigned int mode;                  // pull mode of the next iteration
+    int blog2;                          // its Wedge granularity
+    unsigned int done;                  // converged (or iteration cap reached)
+    unsigned int iters;                 // iterations completed
+    unsigned int fiters;                // of which frontier-mode
+    unsigned int level;                 // BFS: depth of the current frontier
+    unsigned int sched;                 // schedule kind (wg_loop_params.schedule)
+    int plain_blog2;                    // granularity the plain rule uses
+    unsigned int max_iters;             // 0 = no cap
+    unsigned int pad0;
+    unsigned long long threshold;       // plain rule: dense iff activity > threshold
+    unsigned long long last_activity;   // activity of the last completed pull
+    unsigned long long last_activated;
+    unsigned long long unvisited;       // BFS feature for the learned rule
 };
+
+// Gate of a kernel launched by the device-resident loop: it runs only when the
+// control block selects it.  WG_GATE_ALWAYS: launched by the phase API, runs.
+#define WG_GATE_ALWAYS (-1)
+#define WG_GATE_DENSE 0     // runs if the selected pull mode is dense
+#define WG_GATE_FRONTIER 1  // runs if the selected pull mode is frontier
+#define WG_GATE_LIVE 2      // runs until the loop has converged
+
+__device__ __forceinline__ bool wg_gated_off(const wg_dev_counters* c, int gate)
+{
+    if (gate < 0)
+        return false;
+    if (c->done)
+        return true;
+    if (gate == WG_GATE_LIVE)
+        return false;
+    return c->mode != (unsigned int)gate;
+}
 
 struct wg_prof {
     int enabled;
@@ -115,7 +148,7 @@ struct wg_ctx {
     uint64_t queue_cap;
     wg_dev_counters* ctr;     // device
     wg_dev_counters* hctr;    // pinned host mirror
-    int level;                // BFS level of the running pull (current frontier depth)
+    uint32_t root_dev;        // BFS / SSSP root in device numbering
     int iter;                 // iterations run since init
 
     wg_prof prof;

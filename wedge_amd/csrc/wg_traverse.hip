@@ -49,6 +49,36 @@ __device__ __forceinline__ void activate(uint64_t* __restrict__ next, const uint
     }
 }
 
+// Lanes hold the consecutive vertices base + lane; the changed ones are set in
+// 'next' with at most two 64-bit atomics per wavefront (the range straddles at
+// most two bitmask words) and counted when newly set.
+__device__ __forceinline__ void activate_contiguous(uint64_t* __restrict__ next, const uint32_t* __restrict__ od,
+                                                    uint32_t base, bool changed, act_tally& t)
+{
+    const unsigned long long cm = wg_ballot(changed);
+    if (cm == 0ull)
+        return;
+    const int lane = wg_lane();
+    const uint32_t w0 = base >> 6, sh = base & 63u;
+    const unsigned long long lo = cm << sh;                  // bits landing in word w0
+    const unsigned long long hi = sh ? (cm >> (64u - sh)) : 0ull; // bits landing in word w0 + 1
+    unsigned long long old_lo = 0ull, old_hi = 0ull;
+    if (lane == 0 && lo)
+        old_lo = atomicOr((unsigned long long*)&next[w0], lo);
+    if (lane == 1 && hi)
+        old_hi = atomicOr((unsigned long long*)&next[w0 + 1], hi);
+    old_lo = __shfl(old_lo, 0);
+    old_hi = __shfl(old_hi, 1);
+    if (changed) {
+        const uint32_t pos = sh + (uint32_t)lane;
+        const bool was = (pos < 64u) ? ((old_lo >> pos) & 1ull) : ((old_hi >> (pos - 64u)) & 1ull);
+        if (!was) {
+            t.edges += od[base + (uint32_t)lane];
+            t.verts += 1ull;
+        }
+    }
+}
+
 __device__ __forceinline__ void add_count(unsigned long long a, unsigned long long* __restrict__ dst)
 {
     a = wg_wave_sum(a);
@@ -105,9 +135,13 @@ static int reset_counters(wg_ctx* ctx, unsigned int flags)
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
-                                                         uint64_t* __restrict__ packed, uint32_t V, uint32_t newdepth,
-                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr)
+                                                         uint64_t* __restrict__ packed, uint32_t V,
+                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
+                                                         int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
     const uint64_t nwaves = (uint64_t)gridDim.x * (WG_BLOCK / WG_WAVE);
@@ -182,9 +216,12 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
                                                          const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
                                                          uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
-                                                         uint32_t newdepth, const uint32_t* __restrict__ perm,
-                                                         wg_dev_counters* __restrict__ ctr)
+                                                         const uint32_t* __restrict__ perm,
+                                                         wg_dev_counters* __restrict__ ctr, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull };
@@ -243,13 +280,17 @@ __device__ __forceinline__ block_walk make_walk(int blog2)
 
 // Frontier-mode BFS level over the queued edge blocks.
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restrict__ queue,
-                                                          wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                                          wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                           const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
                                                           uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
-                                                          uint32_t newdepth, const uint32_t* __restrict__ perm)
+                                                          const uint32_t* __restrict__ perm, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
+    const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint32_t qlen = ctr->queue_len;
     const block_walk bw = make_walk(blog2);
@@ -291,26 +332,37 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restr
     add_tally(act, ctr);
 }
 
-// End of a BFS level: visited |= next, the old frontier is cleared (it becomes
-// the next 'next'), and the new frontier is counted.
-__global__ void k_bfs_advance(uint64_t* __restrict__ visited, const uint64_t* __restrict__ next,
-                              uint64_t* __restrict__ oldcur, uint64_t nwords, unsigned long long* __restrict__ fv)
+// End of a BFS level: visited |= next, the new frontier replaces the old one
+// (copied, so kernels need no pointer swap), next is cleared, the new
+// frontier is counted and the level advances.
+__global__ void k_bfs_advance(uint64_t* __restrict__ visited, uint64_t* __restrict__ next, uint64_t* __restrict__ cur,
+                              uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t n = next[w];
         if (n) {
             visited[w] |= n;
             cnt += (unsigned long long)__popcll(n);
+            next[w] = 0ull;
         }
-        if (oldcur[w])
-            oldcur[w] = 0ull;
+        if (cur[w] != n)
+            cur[w] = n;
     }
-    add_count(cnt, fv);
+    add_count(cnt, &ctr->frontier_verts);
+    // the level counter moves once all blocks have read it: the pulls of the
+    // next level run after this kernel in stream order
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(&ctr->level, 1u);
 }
 
-__global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, unsigned long long* __restrict__ fv)
+__global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    unsigned long long* fv = &ctr->frontier_verts;
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = bm[w];
@@ -405,8 +457,10 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
                                                         const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
                                                         const uint32_t* __restrict__ tile_e, uint32_t ntiles,
                                                         uint64_t* __restrict__ next, wg_dev_counters* __restrict__ ctr,
-                                                        unsigned int* __restrict__ ovf)
+                                                        unsigned int* __restrict__ ovf, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
     typedef typename P::T T;
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
     constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
@@ -484,11 +538,16 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
 #pragma unroll
         for (int k = 0; k < NVTX; ++k) {
             const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
+            bool changed = false;
+            const uint32_t v = v0 + i;
             if (i < nv) {
                 const T m = s_min[i];
-                if (m != P::INF)
-                    min_apply<P>(prop, next, od, v0 + i, m, act);
+                if (m != P::INF && m < prop[v]) {
+                    const T old = atomic_min_t<T>(&prop[v], m);
+                    changed = old > m;
+                }
             }
+            activate_contiguous(next, od, v0 + (uint32_t)k * WG_WAVE, changed, act);
         }
         wg_wave_sync();
     }
@@ -498,12 +557,16 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
 // Frontier-mode pull over the queued edge blocks.
 template <class P>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(const uint32_t* __restrict__ queue,
-                                                          wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                                          wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const void* __restrict__ w, typename P::T* __restrict__ prop,
                                                           const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
-                                                          uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf)
+                                                          uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf,
+                                                          int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     typedef typename P::T T;
     const int lane = wg_lane();
     const uint32_t qlen = ctr->queue_len;
@@ -532,41 +595,73 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(const uint32_t* __restr
 
 // ---- Wedge frontier ---------------------------------------------------------------
 
-__device__ __forceinline__ void mark_block(uint32_t blk, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                                           wg_dev_counters* __restrict__ ctr)
+// Marks edge block blk in the Wedge frontier; a block marked for the first
+// time is appended to the queue.  Must be called by every lane of the
+// wavefront (valid = false for lanes with nothing to mark): appends are
+// aggregated into one queue reservation per wavefront.
+__device__ __forceinline__ void mark_block_wave(bool valid, uint32_t blk, uint32_t* __restrict__ wedge,
+                                                uint32_t* __restrict__ queue, wg_dev_counters* __restrict__ ctr)
 {
-    const uint32_t m = 1u << (blk & 31);
-    uint32_t* wp = &wedge[blk >> 5];
-    if (*wp & m)
-        return;
-    const uint32_t old = atomicOr(wp, m);
-    if (!(old & m)) {
-        const uint32_t at = atomicAdd(&ctr->queue_len, 1u);
-        queue[at] = blk;
+    bool fresh = false;
+    if (valid) {
+        const uint32_t m = 1u << (blk & 31);
+        uint32_t* wp = &wedge[blk >> 5];
+        if (!(*wp & m)) {
+            const uint32_t old = atomicOr(wp, m);
+            fresh = !(old & m);
+        }
     }
+    const unsigned long long fm = wg_ballot(fresh);
+    if (fm == 0ull)
+        return;
+    const int lane = wg_lane();
+    uint32_t base = 0;
+    if (lane == __ffsll((long long)fm) - 1)
+        base = atomicAdd(&ctr->queue_len, (uint32_t)__popcll(fm));
+    base = __shfl(base, __ffsll((long long)fm) - 1);
+    if (fresh)
+        queue[base + (uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = blk;
 }
 
 #define WG_WEDGE_LANE_MAX 32u
 
 // source/wedge.c:63-155: every set vertex marks the blocks of its out-edges.
+// One lane per bitmask word; the lanes of a wavefront walk their vertices'
+// out-edge lists in lock step so that queue appends can be aggregated.
 __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __restrict__ active, uint64_t nwords,
                                                            const uint32_t* __restrict__ csri_off,
-                                                           const uint32_t* __restrict__ csri_pos, int blog2,
+                                                           const uint32_t* __restrict__ csri_pos, int blog2_arg,
                                                            uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                                                           uint32_t* __restrict__ heavy, wg_dev_counters* __restrict__ ctr)
+                                                           uint32_t* __restrict__ heavy, wg_dev_counters* __restrict__ ctr,
+                                                           int gate)
 {
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t bits = active[w];
-        while (bits) {
-            const uint32_t u = (uint32_t)(w * 64 + (uint64_t)(__ffsll((long long)bits) - 1));
-            bits &= bits - 1ull;
-            const uint32_t s = csri_off[u], e = csri_off[u + 1];
-            if (e - s > WG_WEDGE_LANE_MAX) {
-                heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
-                continue;
+    if (wg_gated_off(ctr, gate))
+        return;
+    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // the loop bound is rounded up to whole wavefronts so every lane takes part
+    const uint64_t nround = (nwords + WG_WAVE - 1) / WG_WAVE * WG_WAVE;
+    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nround; w += stride) {
+        uint64_t bits = (w < nwords) ? active[w] : 0ull;
+        uint32_t p = 0, e = 0; // current out-edge range of this lane
+        for (;;) {
+            if (p == e && bits) {
+                const uint32_t u = (uint32_t)(w * 64 + (uint64_t)(__ffsll((long long)bits) - 1));
+                bits &= bits - 1ull;
+                const uint32_t s0 = csri_off[u], e0 = csri_off[u + 1];
+                if (e0 - s0 > WG_WEDGE_LANE_MAX) {
+                    heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
+                } else {
+                    p = s0;
+                    e = e0;
+                }
             }
-            for (uint32_t p = s; p < e; ++p)
-                mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+            const bool has = p < e;
+            if (wg_ballot(has || bits != 0ull) == 0ull)
+                break;
+            mark_block_wave(has, has ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
+            if (has)
+                ++p;
         }
     }
 }
@@ -574,28 +669,39 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
 // High out-degree vertices deferred by k_wedge_build: one wave per vertex.
 __global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy,
                                                            const uint32_t* __restrict__ csri_off,
-                                                           const uint32_t* __restrict__ csri_pos, int blog2,
+                                                           const uint32_t* __restrict__ csri_pos, int blog2_arg,
                                                            uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                                                           wg_dev_counters* __restrict__ ctr)
+                                                           wg_dev_counters* __restrict__ ctr, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t n = ctr->heavy_len;
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     for (uint32_t i = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
         const uint32_t u = heavy[i];
         const uint32_t s = csri_off[u], e = csri_off[u + 1];
-        for (uint32_t p = s + (uint32_t)wg_lane(); p < e; p += WG_WAVE)
-            mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+        for (uint32_t p0 = s; p0 < e; p0 += WG_WAVE) {
+            const uint32_t p = p0 + (uint32_t)wg_lane();
+            mark_block_wave(p < e, p < e ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
+        }
     }
 }
 
 // source/wedge.c:52-59
 __global__ void k_wedge_single(uint32_t u, const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
-                               int blog2, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                               wg_dev_counters* __restrict__ ctr)
+                               int blog2_arg, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
+                               wg_dev_counters* __restrict__ ctr, int gate)
 {
+    if (wg_gated_off(ctr, gate))
+        return;
+    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t s = csri_off[u], e = csri_off[u + 1];
-    for (uint32_t p = s + blockIdx.x * blockDim.x + threadIdx.x; p < e; p += gridDim.x * blockDim.x)
-        mark_block(csri_pos[p] >> blog2, wedge, queue, ctr);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t p0 = s + (blockIdx.x * blockDim.x + threadIdx.x) / WG_WAVE * WG_WAVE; p0 < e; p0 += stride) {
+        const uint32_t p = p0 + (uint32_t)wg_lane();
+        mark_block_wave(p < e, p < e ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
+    }
 }
 
 __global__ void k_reached_bfs(const uint64_t* __restrict__ packed, const uint32_t* __restrict__ od, uint64_t V,
@@ -642,6 +748,81 @@ __global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* __restrict__
         p[i] = perm ? perm[i] : (uint32_t)i;
 }
 
+// ---- device-resident iteration loop ------------------------------------------------
+
+#define LOOP_SCHED_PLAIN 0u
+#define LOOP_SCHED_LEARNED 1u
+#define LOOP_SCHED_DENSE 2u
+#define LOOP_SCHED_FRONTIER 3u
+
+// Schedule of the next pull from the activity of the last one.
+__device__ void loop_choose(wg_dev_counters* c, unsigned long long act)
+{
+    c->blog2 = c->plain_blog2;
+    switch (c->sched) {
+    case LOOP_SCHED_DENSE:
+        c->mode = WG_MODE_DENSE;
+        break;
+    case LOOP_SCHED_FRONTIER:
+        c->mode = WG_MODE_FRONTIER;
+        break;
+    default:
+        // source/execution_bfs.c:141
+        c->mode = act > c->threshold ? WG_MODE_DENSE : WG_MODE_FRONTIER;
+        break;
+    }
+}
+
+__global__ void k_loop_init(wg_dev_counters* c, unsigned int sched, unsigned long long threshold, int plain_blog2,
+                            unsigned int max_iters, unsigned long long initial_activity, int dense_first,
+                            unsigned long long unvisited)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    c->sched = sched;
+    c->threshold = threshold;
+    c->plain_blog2 = plain_blog2;
+    c->max_iters = max_iters;
+    c->done = 0u;
+    c->iters = 0u;
+    c->fiters = 0u;
+    c->activity = 0ull;
+    c->activated = 0ull;
+    c->queue_len = 0u;
+    c->heavy_len = 0u;
+    c->last_activity = initial_activity;
+    c->last_activated = 0ull;
+    c->unvisited = unvisited;
+    loop_choose(c, initial_activity);
+    if (dense_first)
+        c->mode = WG_MODE_DENSE; // CC: source/execution_cc.c:103
+}
+
+// End of a pull: account the iteration, stop at zero activity, choose the
+// next schedule.  The pull consumed the queued blocks, so the queue restarts.
+__global__ void k_decide(wg_dev_counters* c)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done)
+        return;
+    const unsigned long long act = c->activity, nact = c->activated;
+    c->iters += 1u;
+    if (c->mode == WG_MODE_FRONTIER)
+        c->fiters += 1u;
+    c->activity = 0ull;
+    c->activated = 0ull;
+    c->queue_len = 0u;
+    c->heavy_len = 0u;
+    c->frontier_verts = 0ull;
+    c->last_activity = act;
+    c->last_activated = nact;
+    c->unvisited = c->unvisited >= nact ? c->unvisited - nact : 0ull;
+    if (act == 0ull || (c->max_iters && c->iters >= c->max_iters)) {
+        c->done = 1u;
+        return;
+    }
+    loop_choose(c, act);
+}
+
 // ---- host side ----------------------------------------------------------------------
 
 static int read_counters(wg_ctx* ctx)
@@ -674,8 +855,9 @@ static int init_common(wg_ctx* ctx, int app)
         (rc = reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW)))
         return rc;
     ctx->app = app;
-    ctx->level = 0;
     ctx->iter = 0;
+    WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->level, 0, sizeof(unsigned int), ctx->stream));
+    WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->done, 0, sizeof(unsigned int), ctx->stream));
     return WG_OK;
 }
 
@@ -690,6 +872,7 @@ extern "C" int wg_init_bfs(wg_ctx* ctx, uint64_t root)
     WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
     if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, ~0ull)))
         return rc;
+    ctx->root_dev = r;
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], (uint64_t)r, root);
     hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_cur, r);
     hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_visited, r);
@@ -723,6 +906,7 @@ extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
     uint32_t r = 0;
     if ((rc = wg_to_device_id(ctx, root, &r)))
         return rc;
+    ctx->root_dev = r;
     if (ctx->wkind == WG_WEIGHT_F64) {
         WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
         if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, PolSF::INF)))
@@ -769,7 +953,7 @@ extern "C" int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
     hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, dv, ctx->csri_off, ctx->csri_pos,
-                       block_log2, ctx->wedge, ctx->queue, ctx->ctr);
+                       block_log2, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_ALWAYS);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
     return rc;
@@ -787,11 +971,11 @@ extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
     hipLaunchKernelGGL(k_wedge_build, dim3(wg_grid(ctx, ctx->bm_words, 256, 8)), dim3(WG_BLOCK), 0, ctx->stream,
                        ctx->bm_next, ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue,
-                       ctx->heavy_list, ctx->ctr);
+                       ctx->heavy_list, ctx->ctr, WG_GATE_ALWAYS);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue, ctx->ctr);
+                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_ALWAYS);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
     return rc;
@@ -851,13 +1035,12 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
     int rc = begin_pull(ctx, mode);
     if (rc)
         return rc;
-    const uint32_t nd = (uint32_t)ctx->level + 1u;
     uint64_t* packed = (uint64_t*)ctx->prop[0];
     if (mode == WG_MODE_DENSE) {
         int slot = wg_prof_begin(ctx, WG_K_BFS_DENSE);
         hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                            ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           (uint32_t)ctx->V, nd, ctx->perm, ctx->ctr);
+                           (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_BFS_DENSE, slot);
@@ -865,7 +1048,8 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
             slot = wg_prof_begin(ctx, WG_K_BFS_HEAVY);
             hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                                ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
-                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, nd, ctx->perm, ctx->ctr);
+                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
+                               WG_GATE_ALWAYS);
             if ((rc = wg_check_launch(ctx)))
                 return rc;
             wg_prof_end(ctx, WG_K_BFS_HEAVY, slot);
@@ -874,7 +1058,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
         hipLaunchKernelGGL(k_bfs_blocks, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
-                           ctx->bm_next, packed, ctx->wedge, nd, ctx->perm);
+                           ctx->bm_next, packed, ctx->wedge, ctx->perm, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_BFS_BLOCKS, slot);
@@ -892,7 +1076,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
         hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
                            (typename P::T*)prop, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
-                           ctx->ctr, &ctx->ctr->overflow);
+                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
@@ -900,7 +1084,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         int slot = wg_prof_begin(ctx, WG_K_MIN_BLOCKS);
         hipLaunchKernelGGL(k_min_blocks<P>, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (typename P::T*)prop, ctx->od,
-                           ctx->bm_next, ctx->wedge, &ctx->ctr->overflow);
+                           ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_MIN_BLOCKS, slot);
@@ -950,17 +1134,12 @@ extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_ADVANCE);
     const uint32_t grid = wg_grid(ctx, ctx->bm_words, 1024, 8);
-    if (ctx->app == WG_APP_BFS) {
+    if (ctx->app == WG_APP_BFS)
         hipLaunchKernelGGL(k_bfs_advance, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
-                           ctx->bm_cur, ctx->bm_words, &ctx->ctr->frontier_verts);
-        uint64_t* t = ctx->bm_cur;
-        ctx->bm_cur = ctx->bm_next;
-        ctx->bm_next = t;
-        ctx->level += 1;
-    } else {
-        hipLaunchKernelGGL(k_count_clear, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_next, ctx->bm_words,
-                           &ctx->ctr->frontier_verts);
-    }
+                           ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_ALWAYS);
+    else
+        hipLaunchKernelGGL(k_count_clear, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_next, ctx->bm_words, ctx->ctr,
+                           WG_GATE_ALWAYS);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     wg_prof_end(ctx, WG_K_ADVANCE, slot);
@@ -1021,5 +1200,124 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
         *vertices = ctx->hctr->reached_verts;
     if (edges)
         *edges = ctx->hctr->reached_edges;
+    return WG_OK;
+}
+
+// One iteration of the device-resident loop: every kernel is enqueued, the
+// gates pick the ones that run.
+static int enqueue_iteration(wg_ctx* ctx)
+{
+    uint64_t* packed = (uint64_t*)ctx->prop[0];
+    const uint32_t qgrid = ctx->num_cu * 8;
+    if (ctx->app == WG_APP_BFS) {
+        hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+                           ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
+                           (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_DENSE);
+        if (ctx->nbfs_heavy)
+            hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+                               ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
+                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
+                               WG_GATE_DENSE);
+        hipLaunchKernelGGL(k_bfs_blocks, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
+                           ctx->wedge, ctx->perm, WG_GATE_FRONTIER);
+    } else if (ctx->app == WG_APP_CC) {
+        hipLaunchKernelGGL(k_min_tile<PolCC>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
+                           (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
+                           &ctx->ctr->overflow, WG_GATE_DENSE);
+        hipLaunchKernelGGL(k_min_blocks<PolCC>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
+                           ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
+    } else if (ctx->wkind == WG_WEIGHT_F64) {
+        hipLaunchKernelGGL(k_min_tile<PolSF>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
+                           (unsigned long long*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
+                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_DENSE);
+        hipLaunchKernelGGL(k_min_blocks<PolSF>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (unsigned long long*)ctx->prop[0], ctx->od,
+                           ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
+    } else {
+        hipLaunchKernelGGL(k_min_tile<PolSU>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
+                           (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
+                           &ctx->ctr->overflow, WG_GATE_DENSE);
+        hipLaunchKernelGGL(k_min_blocks<PolSU>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
+                           ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
+    }
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr);
+    hipLaunchKernelGGL(k_wedge_build, dim3(wg_grid(ctx, ctx->bm_words, 256, 8)), dim3(WG_BLOCK), 0, ctx->stream,
+                       ctx->bm_next, ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->heavy_list,
+                       ctx->ctr, WG_GATE_FRONTIER);
+    hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
+                       ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_FRONTIER);
+    const uint32_t agrid = wg_grid(ctx, ctx->bm_words, 1024, 8);
+    if (ctx->app == WG_APP_BFS)
+        hipLaunchKernelGGL(k_bfs_advance, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
+                           ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_LIVE);
+    else
+        hipLaunchKernelGGL(k_count_clear, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_next, ctx->bm_words, ctx->ctr,
+                           WG_GATE_LIVE);
+    return wg_check_launch(ctx);
+}
+
+extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* iterations, uint64_t* frontier_iterations)
+{
+    if (!ctx || !p)
+        return WG_E_INVAL;
+    if (ctx->app != WG_APP_BFS && ctx->app != WG_APP_CC && ctx->app != WG_APP_SSSP)
+        return WG_E_STATE;
+    if (p->schedule < 0 || p->schedule > 3)
+        return WG_E_INVAL;
+    const int blog2 = p->block_log2 ? p->block_log2 : WG_DEFAULT_BLOCK_LOG2;
+    if (!check_block_log2(blog2))
+        return WG_E_INVAL;
+    int rc;
+    if (ctx->wedge_log2 >= 0 && (rc = clear_wedge_state(ctx)))
+        return rc;
+    const unsigned long long threshold = (ctx->E * (unsigned long long)p->threshold_pct) / 100ull;
+    unsigned long long act0 = ctx->E;
+    if (ctx->app != WG_APP_CC) {
+        uint32_t od = 0;
+        WG_HIP(ctx, hipMemcpyAsync(&od, ctx->od + ctx->root_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        act0 = od;
+    }
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, ctx->stream, ctx->ctr, (unsigned int)p->schedule, threshold, blog2,
+                       p->max_iterations, act0, ctx->app == WG_APP_CC ? 1 : 0,
+                       ctx->app == WG_APP_BFS ? (unsigned long long)(ctx->V - 1) : 0ull);
+    if (ctx->app != WG_APP_CC) // the root's blocks, if the first pull is frontier-filtered
+        hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, ctx->root_dev, ctx->csri_off,
+                           ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_FRONTIER);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    int slot = wg_prof_begin(ctx, WG_K_LOOP);
+    // Every iteration either activates a vertex or ends the loop, and a
+    // vertex can only improve finitely often, but the host loop does not rely
+    // on that: it stops with an error if the device stops counting iterations
+    // or the count passes 2V + 64.
+    const uint64_t cap = 2 * ctx->V + 64;
+    uint32_t batch = 2;
+    uint32_t seen = 0;
+    for (;;) {
+        for (uint32_t i = 0; i < batch; ++i)
+            if ((rc = enqueue_iteration(ctx)))
+                return rc;
+        WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, sizeof(wg_dev_counters), hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->hctr->done)
+            break;
+        if (ctx->hctr->iters <= seen || ctx->hctr->iters > cap)
+            return WG_E_STATE;
+        seen = ctx->hctr->iters;
+        batch = batch < 64 ? batch * 2 : 64;
+    }
+    wg_prof_end(ctx, WG_K_LOOP, slot);
+    ctx->wedge_log2 = -1;
+    ctx->iter += (int)ctx->hctr->iters;
+    if (iterations)
+        *iterations = ctx->hctr->iters;
+    if (frontier_iterations)
+        *frontier_iterations = ctx->hctr->fiters;
+    if (ctx->app == WG_APP_SSSP && ctx->hctr->overflow)
+        return WG_E_OVERFLOW;
     return WG_OK;
 }

@@ -144,8 +144,41 @@ int wg_run_pr(wg_ctx* ctx, const wg_run_opts* o, double* ranks, wg_run_stats* st
 
 typedef int (*pull_fn)(wg_ctx*, int, uint64_t*);
 
+/* Device-resident loop: the schedule is decided on the GPU, no per-iteration
+ * host round trip (used unless per-iteration trace rows are requested). */
+static int run_device_loop(wg_ctx* ctx, const wg_run_opts* o, wg_run_stats* st)
+{
+    wg_loop_params p;
+    p.schedule = o->schedule;
+    p.threshold_pct = o->threshold_pct;
+    p.block_log2 = block_log2_of(o);
+    p.max_iterations = o->max_iterations;
+    wg_run_stats s;
+    memset(&s, 0, sizeof(s));
+    uint64_t it = 0, fit = 0;
+    const double t0 = now_ms();
+    int rc = wg_converge(ctx, &p, &it, &fit);
+    if (rc)
+        return rc;
+    s.running_ms = now_ms() - t0;
+    s.pull_ms = s.running_ms;
+    s.iterations = it;
+    s.frontier_iterations = fit;
+    uint64_t rv = 0, re = 0;
+    rc = wg_reached(ctx, &rv, &re);
+    if (rc)
+        return rc;
+    s.vertices_reached = rv;
+    s.edges_reached = re;
+    if (st)
+        *st = s;
+    return WG_OK;
+}
+
 static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pull, wg_run_stats* st)
 {
+    if (!o->record_trace && (o->schedule != WG_SCHED_LEARNED || wg_mlp_loaded()))
+        return run_device_loop(ctx, o, st);
     uint64_t V = 0, E = 0;
     wg_graph_info(ctx, &V, &E);
     const uint64_t threshold = (E * (uint64_t)o->threshold_pct) / 100ull;
@@ -215,6 +248,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
         row.frontier_edges = active_edges;
         row.unvisited = unvisited;
         row.pull_ms = dt_pull;
+        if (it > 2 * V + 64)
+            return WG_E_STATE; /* cannot happen for a correct pull: each iteration activates a vertex */
         if (act == 0 || (o->max_iterations && it >= o->max_iterations)) {
             if (o->record_trace)
                 trace_push(&row);
@@ -242,6 +277,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
         rc = wg_advance_frontier(ctx, NULL);
         if (rc)
             return rc;
+        if (o->record_trace && (rc = wg_sync(ctx)))
+            return rc; /* charge the frontier work to this row, not the next pull */
         const double t3 = now_ms();
         s.wedge_ms += t3 - t2;
         row.wedge_ms = t3 - t2;
