@@ -86,9 +86,15 @@ int wg_last_hip_error(const wg_ctx* ctx);
 int wg_sync(wg_ctx* ctx);
 
 /* Context options (set before wg_load_csc). */
-#define WG_OPT_RELABEL 1  /* 1 (default): store vertices in descending total-degree
-                             order on the device; every id and value crossing this
-                             API stays in the caller's original numbering */
+#define WG_OPT_RELABEL 1  /* 1: store vertices in descending total-degree order on
+                             the device; 0: keep the caller's order; 2 (default):
+                             relabel only skewed graphs (maximum in-degree above
+                             64 and above 16x the mean degree).  Every id and
+                             value crossing this API stays in the caller's
+                             original numbering either way */
+#define WG_RELABEL_OFF 0
+#define WG_RELABEL_ON 1
+#define WG_RELABEL_AUTO 2
 int wg_set_option(wg_ctx* ctx, int option, int64_t value);
 
 /* Uploads a destination-grouped CSC and builds every derived device structure:

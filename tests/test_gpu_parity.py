@@ -21,7 +21,7 @@ PR_TOL_FP32 = 1e-5  # L1, FP32-stored contributions (expected ~1e-8..1e-7)
 
 @pytest.fixture(scope="module", params=[True, False], ids=["relabel", "native-order"])
 def ctx(request):
-    """Every test runs with the device-side degree relabelling on (default) and off."""
+    """Every test runs with the device-side degree relabelling forced on and off."""
     c = W.Context(0, relabel=request.param)
     yield c
     c.close()
@@ -223,3 +223,26 @@ def test_bad_arguments_are_errors(ctx):
         ctx.wedge_generate_frontier(2)  # granularity below the minimum
     with pytest.raises(W.WedgeError):
         ctx.init_sssp(0)  # unweighted graph
+    with pytest.raises(W.WedgeError):
+        ctx.set_option(W.OPT_RELABEL, 3)  # not a relabelling mode
+
+
+@pytest.mark.gpu
+def test_default_relabel_choice_keeps_results():
+    """The default context relabels a skewed graph and keeps a near-regular one
+    in place; results are identical to the oracle either way."""
+    c = W.Context(0)
+    try:
+        for g in (W.rmat(12, 16, seed=3), W.grid(48, 64, seed=5)):
+            c.load(g)
+            root = g.max_outdegree_vertex()
+            packed, _ = c.run_bfs(root)
+            _, depth, _ = oracle.Oracle.of(g, 4).bfs(root)
+            assert np.array_equal(W.bfs_depth(packed), depth)
+            if g.weights is not None:
+                dist, _ = c.run_sssp(root)
+                want, _ = oracle.Oracle.of(g, 4).sssp(root)
+                got = np.where(dist == W.UNREACHED_U32, np.uint64(2**64 - 1), dist.astype(np.uint64))
+                assert np.array_equal(got, want)
+    finally:
+        c.close()

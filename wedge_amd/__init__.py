@@ -51,6 +51,7 @@ KERNELS = {
 }
 
 OPT_RELABEL = 1
+RELABEL_OFF, RELABEL_ON, RELABEL_AUTO = 0, 1, 2
 
 UNREACHED_U64 = np.uint64(0xFFFFFFFFFFFFFFFF)
 UNREACHED_U32 = np.uint32(0xFFFFFFFF)
@@ -374,14 +375,14 @@ def run_opts(**kw) -> RunOpts:
 class Context:
     """One HIP device context holding one graph (include/wedge_gpu.h)."""
 
-    def __init__(self, device: int = 0, relabel: bool = True):
+    def __init__(self, device: int = 0, relabel="auto"):
         self._lib = _lib_gpu()
         _lib_host()
         h = _p()
         _check(self._lib.wg_open(C.byref(h), device), "wg_open")
         self._h = h
         self.graph: Graph | None = None
-        self.set_option(OPT_RELABEL, int(relabel))
+        self.set_option(OPT_RELABEL, RELABEL_AUTO if relabel == "auto" else int(bool(relabel)))
 
     def set_option(self, option: int, value: int):
         """Context option (takes effect at the next load)."""

@@ -273,7 +273,9 @@ extern "C" int wg_set_option(wg_ctx* ctx, int option, int64_t value)
         return WG_E_INVAL;
     switch (option) {
     case WG_OPT_RELABEL:
-        ctx->relabel = value ? 1 : 0;
+        if (value < WG_RELABEL_OFF || value > WG_RELABEL_AUTO)
+            return WG_E_INVAL;
+        ctx->relabel = (int)value;
         return WG_OK;
     default:
         return WG_E_INVAL;
@@ -360,7 +362,7 @@ extern "C" int wg_open(wg_ctx** out, int device)
     if (!ctx)
         return WG_E_NOMEM;
     ctx->device = device;
-    ctx->relabel = 1;
+    ctx->relabel = WG_RELABEL_AUTO;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         free(ctx);
         return WG_E_HIP;
@@ -423,7 +425,6 @@ static void free_graph(wg_ctx* ctx)
     free_dev((void**)&ctx->bm_next);
     free_dev((void**)&ctx->bm_visited);
     free_dev((void**)&ctx->wedge);
-    free_dev((void**)&ctx->queue);
     free_dev((void**)&ctx->heavy_list);
     free_dev((void**)&ctx->perm);
     free_dev((void**)&ctx->inv);
@@ -507,6 +508,20 @@ static int upload(wg_ctx* ctx, T** dptr, const T* host, uint64_t n)
     if (n && host)
         WG_HIP(ctx, hipMemcpy(*dptr, host, n * sizeof(T), hipMemcpyHostToDevice));
     return WG_OK;
+}
+
+// Relabelling pays off when a few vertices hold many edges (their values are
+// gathered over and over and end up in one place); on near-regular graphs such
+// as road networks the caller's order usually keeps neighbours closer.
+static bool skewed_degrees(const uint32_t* off, uint64_t V, uint64_t E)
+{
+    uint32_t maxin = 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        const uint32_t d = off[v + 1] - off[v];
+        maxin = d > maxin ? d : maxin;
+    }
+    const double mean = (double)E / (double)V;
+    return maxin > 64u && (double)maxin > 16.0 * mean;
 }
 
 // Replaces the device CSC (off/src/w/od, original numbering) by the same graph
@@ -668,7 +683,7 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
             goto fail;
         }
     }
-    if (ctx->relabel && V > 1) {
+    if (V > 1 && (ctx->relabel == WG_RELABEL_ON || (ctx->relabel == WG_RELABEL_AUTO && skewed_degrees(off, V, E)))) {
         if ((rc = relabel_graph(ctx, hoff)))
             goto fail;
         toff = hoff.data();
@@ -735,15 +750,13 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
     }
     // scratch shared by the apps
     ctx->bm_words = (V + 63) / 64;
-    ctx->queue_cap = (E >> WG_MIN_BLOCK_LOG2) + 2;
-    ctx->wedge_words = (ctx->queue_cap + 31) / 32 + 1;
+    ctx->wedge_words = (((E + (1ull << WG_MIN_BLOCK_LOG2) - 1) >> WG_MIN_BLOCK_LOG2) + 31) / 32 + 1;
     ctx->pr_grid = wg_grid(ctx, ctx->ntiles, WG_WPB, 8);
     ctx->span_grid = wg_grid(ctx, ctx->nheavy, 4, 4);
     if ((rc = upload<uint64_t>(ctx, &ctx->bm_cur, nullptr, ctx->bm_words)) ||
         (rc = upload<uint64_t>(ctx, &ctx->bm_next, nullptr, ctx->bm_words)) ||
         (rc = upload<uint64_t>(ctx, &ctx->bm_visited, nullptr, ctx->bm_words)) ||
         (rc = upload<uint32_t>(ctx, &ctx->wedge, nullptr, ctx->wedge_words)) ||
-        (rc = upload<uint32_t>(ctx, &ctx->queue, nullptr, ctx->queue_cap)) ||
         (rc = upload<uint32_t>(ctx, &ctx->heavy_list, nullptr, V)) ||
         (rc = upload<double>(ctx, &ctx->tile_part, nullptr, ctx->ntiles + 1)) ||
         (rc = upload<double>(ctx, &ctx->blk_part, nullptr, (uint64_t)ctx->num_cu * 16 + 16)))

@@ -4,7 +4,7 @@
 //   off[V+1]        CSC offsets by destination
 //   src[E]          source of each edge, ascending within a destination
 //   dst[E]          destination of each edge (read only by frontier-mode pulls,
-//                   which touch only the queued edge blocks)
+//                   which touch only the marked edge blocks)
 //   w[E]            weights: double (SSSP FP64) or uint32 (SSSP integer)
 //   od[V]           out-degree
 //   csri_off[V+1],  source -> positions of its out-edges in src[]
@@ -35,12 +35,25 @@
 #define WG_BFS_SEQ 16                         // edges a lane scans alone in dense BFS
 #define WG_BFS_HEAVY 2048                     // in-degree above which dense BFS splits a vertex
 
-// Device-side counters, one struct per context.
+// Counts that many workgroups add to are spread over WG_CSLOTS slots, one
+// 256-byte line each: atomics from many workgroups on one address serialise
+// in one L2 channel (about 13 ns each on gfx950), while different lines are
+// served in parallel.  A fold (one wavefront) adds the slots into the scalar
+// fields below and zeroes them.
+#define WG_CSLOTS 32
+enum { WG_CNT_ACTIVITY = 0, WG_CNT_ACTIVATED, WG_CNT_FRONTIER, WG_CNT_REACHED_V, WG_CNT_REACHED_E, WG_CNT_N };
+struct __attribute__((aligned(256))) wg_count_slot {
+    unsigned long long v[WG_CNT_N];
+    unsigned long long pad[32 - WG_CNT_N];
+};
+
+// Device-side counters, one struct per context.  The host mirror copies the
+// fields before 'slots' only.
 struct wg_dev_counters {
     unsigned long long activity;        // sum of out-degrees of vertices activated this pull
     unsigned long long activated;       // number of vertices activated this pull
     unsigned long long frontier_verts;  // popcount of the frontier made current by advance
-    unsigned int queue_len;             // edge blocks queued for the next frontier pull
+    unsigned int pad1;
     unsigned int heavy_len;             // high out-degree vertices deferred by the frontier build
     unsigned int overflow;              // integer SSSP distance reached 2^32-1
     unsigned int ticket;                // last-block-done ticket (PR span kernel)
@@ -63,7 +76,9 @@ struct wg_dev_counters {
     unsigned long long last_activity;   // activity of the last completed pull
     unsigned long long last_activated;
     unsigned long long unvisited;       // BFS feature for the learned rule
+    wg_count_slot slots[WG_CSLOTS];
 };
+#define WG_CTR_HOST_BYTES offsetof(wg_dev_counters, slots)
 
 // Gate of a kernel launched by the device-resident loop: it runs only when the
 // control block selects it.  WG_GATE_ALWAYS: launched by the phase API, runs.
@@ -142,10 +157,8 @@ struct wg_ctx {
     uint64_t* bm_visited;     // BFS: visited before this level
     uint32_t* wedge;          // Wedge frontier, one bit per edge block
     uint64_t wedge_words;     // 32-bit words allocated (finest granularity)
-    int wedge_log2;           // granularity of the queued blocks
-    uint32_t* queue;          // queued edge blocks
+    int wedge_log2;           // granularity of the marked blocks, -1 if none are marked
     uint32_t* heavy_list;     // frontier build: deferred high out-degree vertices
-    uint64_t queue_cap;
     wg_dev_counters* ctr;     // device
     wg_dev_counters* hctr;    // pinned host mirror
     uint32_t root_dev;        // BFS / SSSP root in device numbering

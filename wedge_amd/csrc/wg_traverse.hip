@@ -20,9 +20,11 @@
 // Bitmasks (one bit per vertex, 64-bit words): cur (BFS frontier), next
 // (activated by the running pull: the reference's active bitmask with its
 // cond_set_bit semantics), visited (BFS, visited before the running level).
-// The Wedge frontier is one bit per edge block of 2^b consecutive CSC edges;
-// marking a block also appends it to a queue, and the frontier pull walks the
-// queue, so its cost follows the active blocks instead of the bitmask size.
+// The Wedge frontier is one bit per edge block of 2^b consecutive CSC edges
+// (32-bit words).  A frontier pull scans the words, consumes (zeroes) the
+// non-zero ones and spreads their marked blocks over the lanes of a wavefront,
+// so the edges it reads follow the marked blocks; the word scan itself is
+// E / 2^b / 8 bytes.
 #include "wg_internal.h"
 
 __device__ __forceinline__ bool test_bit64(const uint64_t* __restrict__ bm, uint32_t i)
@@ -79,37 +81,81 @@ __device__ __forceinline__ void activate_contiguous(uint64_t* __restrict__ next,
     }
 }
 
-__device__ __forceinline__ void add_count(unsigned long long a, unsigned long long* __restrict__ dst)
+// Adds a per-lane count to field f of this workgroup's counter slot.
+__device__ __forceinline__ void add_count(unsigned long long a, wg_dev_counters* __restrict__ c, int f)
 {
     a = wg_wave_sum(a);
     if (wg_lane() == 0 && a)
-        atomicAdd(dst, a);
+        atomicAdd(&c->slots[blockIdx.x & (WG_CSLOTS - 1)].v[f], a);
 }
 
 __device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* __restrict__ c)
 {
-    add_count(t.edges, &c->activity);
-    add_count(t.verts, &c->activated);
+    add_count(t.edges, c, WG_CNT_ACTIVITY);
+    add_count(t.verts, c, WG_CNT_ACTIVATED);
+}
+
+__device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, int f)
+{
+    switch (f) {
+    case WG_CNT_ACTIVITY:
+        return &c->activity;
+    case WG_CNT_ACTIVATED:
+        return &c->activated;
+    case WG_CNT_FRONTIER:
+        return &c->frontier_verts;
+    case WG_CNT_REACHED_V:
+        return &c->reached_verts;
+    default:
+        return &c->reached_edges;
+    }
+}
+
+// Adds the slots into the scalar counts and zeroes them.  Called by the
+// first wavefront of a launch only; nothing else may be adding to the slots.
+__device__ void fold_counts(wg_dev_counters* c)
+{
+    const int lane = wg_lane();
+    for (int f = 0; f < WG_CNT_N; ++f) {
+        unsigned long long x = 0ull;
+        if (lane < WG_CSLOTS) {
+            x = c->slots[lane].v[f];
+            c->slots[lane].v[f] = 0ull;
+        }
+        x = wg_wave_sum(x);
+        if (lane == 0)
+            *count_field(c, f) += x;
+    }
+}
+
+__global__ void k_fold(wg_dev_counters* c)
+{
+    if (threadIdx.x < WG_WAVE)
+        fold_counts(c);
 }
 
 // ---- counters -------------------------------------------------------------------
 
 #define RST_ACTIVITY 1u
-#define RST_QUEUE 2u
 #define RST_FRONTIER 4u
 #define RST_OVERFLOW 8u
 #define RST_HEAVY 16u
 
 __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
 {
+    const unsigned int t = threadIdx.x;
+    if (blockIdx.x == 0 && t < WG_CSLOTS) {
+        if (flags & RST_ACTIVITY) {
+            c->slots[t].v[WG_CNT_ACTIVITY] = 0ull;
+            c->slots[t].v[WG_CNT_ACTIVATED] = 0ull;
+        }
+        if (flags & RST_FRONTIER)
+            c->slots[t].v[WG_CNT_FRONTIER] = 0ull;
+    }
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (flags & RST_ACTIVITY) {
             c->activity = 0ull;
             c->activated = 0ull;
-        }
-        if (flags & RST_QUEUE) {
-            c->queue_len = 0u;
-            c->heavy_len = 0u;
         }
         if (flags & RST_FRONTIER)
             c->frontier_verts = 0ull;
@@ -262,7 +308,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
     add_tally(act, ctr);
 }
 
-// Lane -> (queued block, edge) mapping shared by the frontier pulls.
+// Lane -> (marked block, edge) mapping shared by the frontier pulls.
 struct block_walk {
     uint32_t per_wave;   // blocks per wave step
     uint32_t lane_bits;  // log2 of lanes per block
@@ -278,9 +324,62 @@ __device__ __forceinline__ block_walk make_walk(int blog2)
     return b;
 }
 
-// Frontier-mode BFS level over the queued edge blocks.
-__global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restrict__ queue,
-                                                          wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
+// Words of the Wedge frontier at granularity 2^blog2 for E edges.
+__device__ __forceinline__ uint32_t wedge_nwords(uint32_t E, int blog2)
+{
+    const uint64_t nblk = ((uint64_t)E + (1ull << blog2) - 1ull) >> blog2;
+    return (uint32_t)((nblk + 31ull) >> 5);
+}
+
+// Walks the marked edge blocks (source/phase_edge_pull_bfs.c:245-268): each
+// wavefront takes 64 consecutive frontier words, one per lane, zeroes the
+// non-zero ones, and for each of them hands the marked blocks to lane groups
+// of 2^min(b,6) lanes, 64 >> min(b,6) blocks per step.  f(p, ok, first) is
+// called by every lane of the wavefront for each 64-edge step: p is the edge
+// position, ok whether the lane holds an edge, first whether the lane is the
+// first of its block's group.
+template <class F>
+__device__ __forceinline__ void walk_marked_blocks(uint32_t* __restrict__ wedge, int blog2, uint32_t E, F&& f)
+{
+    const block_walk bw = make_walk(blog2);
+    const uint32_t nwords = wedge_nwords(E, blog2);
+    const int lane = wg_lane();
+    const uint32_t sub = (uint32_t)lane & ((1u << bw.lane_bits) - 1u);
+    const uint32_t slot = (uint32_t)lane >> bw.lane_bits;
+    const uint32_t nwaves = gridDim.x * WG_WPB;
+    for (uint32_t w0 = (blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE) * WG_WAVE; w0 < nwords; w0 += nwaves * WG_WAVE) {
+        const uint32_t wi = w0 + (uint32_t)lane;
+        uint32_t word = 0u;
+        if (wi < nwords) {
+            word = wedge[wi];
+            if (word)
+                wedge[wi] = 0u; // consumed, source/phase_edge_pull_bfs.c:252-256
+        }
+        unsigned long long nz = wg_ballot(word != 0u);
+        while (nz) {
+            const int l = __ffsll((long long)nz) - 1;
+            nz &= nz - 1ull;
+            uint32_t bits = (uint32_t)__shfl((int)word, l); // wave-uniform
+            const uint64_t base = (uint64_t)(w0 + (uint32_t)l) * 32u;
+            while (bits) {
+                uint32_t mine = bits; // this lane group's block: set bit number 'slot'
+                for (uint32_t i = 0; i < slot && mine; ++i)
+                    mine &= mine - 1u;
+                const bool have = mine != 0u;
+                const uint64_t blk = base + (have ? (uint64_t)(__ffs((int)mine) - 1) : 0ull);
+                for (uint32_t st = 0; st < bw.steps; ++st) {
+                    const uint64_t p64 = (blk << blog2) + (uint64_t)st * 64u + sub;
+                    f((uint32_t)p64, have && p64 < E, sub == 0u);
+                }
+                for (uint32_t i = 0; i < bw.per_wave && bits; ++i)
+                    bits &= bits - 1u;
+            }
+        }
+    }
+}
+
+// Frontier-mode BFS level over the marked edge blocks.
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                           const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
@@ -292,43 +391,28 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(const uint32_t* __restr
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
-    const uint32_t qlen = ctr->queue_len;
-    const block_walk bw = make_walk(blog2);
-    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull };
-    for (uint32_t q0 = (blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE) * bw.per_wave; q0 < qlen;
-         q0 += nwaves * bw.per_wave) {
-        const uint32_t q = q0 + ((uint32_t)lane >> bw.lane_bits);
-        const bool qok = q < qlen;
-        const uint32_t blk = qok ? queue[q] : 0u;
-        if (qok && ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) == 0u)
-            wedge[blk >> 5] = 0u; // consume the word; the queue drives the work
-        for (uint32_t st = 0; st < bw.steps; ++st) {
-            const uint64_t p64 = ((uint64_t)blk << blog2) + (uint64_t)st * 64u + ((uint32_t)lane & ((1u << bw.lane_bits) - 1u));
-            const bool ok = qok && p64 < E;
-            const uint32_t p = (uint32_t)p64;
-            bool cand = false;
-            uint32_t u = 0, v = 0;
-            if (ok) {
-                u = src[p];
-                v = dst[p];
-                cand = test_bit64(cur, u) && !test_bit64(visited, v);
-            }
-            // only the first candidate of a run of equal destinations within
-            // one block needs the atomic: sources ascend within a destination
-            // (lanes of different blocks are not compared: queued blocks come
-            // in no particular order)
-            const uint32_t pv = __shfl_up(v, 1);
-            const unsigned long long cm = wg_ballot(cand);
-            const bool prev_same = ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) != 0u &&
-                                   ((cm >> (lane - 1)) & 1ull) && pv == v;
-            if (cand && !prev_same) {
-                const uint32_t pid = perm ? perm[u] : u;
-                atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)pid);
-                activate(next, od, v, act);
-            }
+    walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool first) {
+        bool cand = false;
+        uint32_t u = 0, v = 0;
+        if (ok) {
+            u = src[p];
+            v = dst[p];
+            cand = test_bit64(cur, u) && !test_bit64(visited, v);
         }
-    }
+        // only the first candidate of a run of equal destinations within one
+        // block needs the atomic: sources ascend within a destination (lanes
+        // of different blocks are not compared: their blocks need not be
+        // adjacent)
+        const uint32_t pv = __shfl_up(v, 1);
+        const unsigned long long cm = wg_ballot(cand);
+        const bool prev_same = !first && ((cm >> (lane - 1)) & 1ull) && pv == v;
+        if (cand && !prev_same) {
+            const uint32_t pid = perm ? perm[u] : u;
+            atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)pid);
+            activate(next, od, v, act);
+        }
+    });
     add_tally(act, ctr);
 }
 
@@ -343,15 +427,16 @@ __global__ void k_bfs_advance(uint64_t* __restrict__ visited, uint64_t* __restri
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t n = next[w];
+        const uint64_t c = cur[w];
         if (n) {
             visited[w] |= n;
             cnt += (unsigned long long)__popcll(n);
             next[w] = 0ull;
         }
-        if (cur[w] != n)
+        if (c != n)
             cur[w] = n;
     }
-    add_count(cnt, &ctr->frontier_verts);
+    add_count(cnt, ctr, WG_CNT_FRONTIER);
     // the level counter moves once all blocks have read it: the pulls of the
     // next level run after this kernel in stream order
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -362,7 +447,6 @@ __global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev
 {
     if (wg_gated_off(ctr, gate))
         return;
-    unsigned long long* fv = &ctr->frontier_verts;
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = bm[w];
@@ -371,7 +455,7 @@ __global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev
             bm[w] = 0ull;
         }
     }
-    add_count(cnt, fv);
+    add_count(cnt, ctr, WG_CNT_FRONTIER);
 }
 
 // ---- CC / SSSP (min propagation) -------------------------------------------------
@@ -554,10 +638,9 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
     add_tally(act, ctr);
 }
 
-// Frontier-mode pull over the queued edge blocks.
+// Frontier-mode pull over the marked edge blocks.
 template <class P>
-__global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(const uint32_t* __restrict__ queue,
-                                                          wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
+__global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const void* __restrict__ w, typename P::T* __restrict__ prop,
                                                           const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
@@ -568,100 +651,66 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(const uint32_t* __restr
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     typedef typename P::T T;
-    const int lane = wg_lane();
-    const uint32_t qlen = ctr->queue_len;
-    const block_walk bw = make_walk(blog2);
-    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull };
-    for (uint32_t q0 = (blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE) * bw.per_wave; q0 < qlen;
-         q0 += nwaves * bw.per_wave) {
-        const uint32_t q = q0 + ((uint32_t)lane >> bw.lane_bits);
-        const bool qok = q < qlen;
-        const uint32_t blk = qok ? queue[q] : 0u;
-        if (qok && ((uint32_t)lane & ((1u << bw.lane_bits) - 1u)) == 0u)
-            wedge[blk >> 5] = 0u;
-        for (uint32_t st = 0; st < bw.steps; ++st) {
-            const uint64_t p64 = ((uint64_t)blk << blog2) + (uint64_t)st * 64u + ((uint32_t)lane & ((1u << bw.lane_bits) - 1u));
-            if (qok && p64 < E) {
-                const uint32_t p = (uint32_t)p64;
-                const T m = P::val(prop, w, src[p], p, ovf);
-                if (m != P::INF)
-                    min_apply<P>(prop, next, od, dst[p], m, act);
-            }
+    walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool) {
+        if (ok) {
+            const T m = P::val(prop, w, src[p], p, ovf);
+            if (m != P::INF)
+                min_apply<P>(prop, next, od, dst[p], m, act);
         }
-    }
+    });
     add_tally(act, ctr);
 }
 
 // ---- Wedge frontier ---------------------------------------------------------------
 
-// Marks edge block blk in the Wedge frontier; a block marked for the first
-// time is appended to the queue.  Must be called by every lane of the
-// wavefront (valid = false for lanes with nothing to mark): appends are
-// aggregated into one queue reservation per wavefront.
-__device__ __forceinline__ void mark_block_wave(bool valid, uint32_t blk, uint32_t* __restrict__ wedge,
-                                                uint32_t* __restrict__ queue, wg_dev_counters* __restrict__ ctr)
+// Marks edge block blk (source/wedge.c:22-31).  The result of the atomic is
+// not needed, so the lane does not wait for it.
+__device__ __forceinline__ void mark_block(uint32_t blk, uint32_t* __restrict__ wedge)
 {
-    bool fresh = false;
-    if (valid) {
-        const uint32_t m = 1u << (blk & 31);
-        uint32_t* wp = &wedge[blk >> 5];
-        if (!(*wp & m)) {
-            const uint32_t old = atomicOr(wp, m);
-            fresh = !(old & m);
-        }
-    }
-    const unsigned long long fm = wg_ballot(fresh);
-    if (fm == 0ull)
-        return;
-    const int lane = wg_lane();
-    uint32_t base = 0;
-    if (lane == __ffsll((long long)fm) - 1)
-        base = atomicAdd(&ctr->queue_len, (uint32_t)__popcll(fm));
-    base = __shfl(base, __ffsll((long long)fm) - 1);
-    if (fresh)
-        queue[base + (uint32_t)__popcll(fm & ((1ull << lane) - 1ull))] = blk;
+    const uint32_t m = 1u << (blk & 31);
+    uint32_t* wp = &wedge[blk >> 5];
+    if (!(*wp & m))
+        atomicOr(wp, m);
 }
 
 #define WG_WEDGE_LANE_MAX 32u
 
 // source/wedge.c:63-155: every set vertex marks the blocks of its out-edges.
-// One lane per bitmask word; the lanes of a wavefront walk their vertices'
-// out-edge lists in lock step so that queue appends can be aggregated.
+// Each lane loads one bitmask word; the wavefront then takes the non-zero
+// words one at a time with one lane per vertex of the word, so a dense word
+// costs one step and an empty one nothing.  Vertices with more than
+// WG_WEDGE_LANE_MAX out-edges are deferred to k_wedge_heavy.
 __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __restrict__ active, uint64_t nwords,
                                                            const uint32_t* __restrict__ csri_off,
                                                            const uint32_t* __restrict__ csri_pos, int blog2_arg,
-                                                           uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                                                           uint32_t* __restrict__ heavy, wg_dev_counters* __restrict__ ctr,
-                                                           int gate)
+                                                           uint32_t* __restrict__ wedge, uint32_t* __restrict__ heavy,
+                                                           wg_dev_counters* __restrict__ ctr, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // the loop bound is rounded up to whole wavefronts so every lane takes part
-    const uint64_t nround = (nwords + WG_WAVE - 1) / WG_WAVE * WG_WAVE;
-    for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nround; w += stride) {
-        uint64_t bits = (w < nwords) ? active[w] : 0ull;
-        uint32_t p = 0, e = 0; // current out-edge range of this lane
-        for (;;) {
-            if (p == e && bits) {
-                const uint32_t u = (uint32_t)(w * 64 + (uint64_t)(__ffsll((long long)bits) - 1));
-                bits &= bits - 1ull;
-                const uint32_t s0 = csri_off[u], e0 = csri_off[u + 1];
-                if (e0 - s0 > WG_WEDGE_LANE_MAX) {
-                    heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
-                } else {
-                    p = s0;
-                    e = e0;
-                }
+    const int lane = wg_lane();
+    const uint64_t nwaves = (uint64_t)gridDim.x * WG_WPB;
+    for (uint64_t w0 = ((uint64_t)blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE) * WG_WAVE; w0 < nwords;
+         w0 += nwaves * WG_WAVE) {
+        const uint64_t wi = w0 + (uint64_t)lane;
+        const uint64_t word = (wi < nwords) ? active[wi] : 0ull;
+        unsigned long long nz = wg_ballot(word != 0ull);
+        while (nz) {
+            const int l = __ffsll((long long)nz) - 1;
+            nz &= nz - 1ull;
+            const uint64_t bits = __shfl(word, l);
+            if (!((bits >> lane) & 1ull))
+                continue;
+            const uint32_t u = (uint32_t)((w0 + (uint64_t)l) * 64u + (uint64_t)lane);
+            const uint32_t s0 = csri_off[u], e0 = csri_off[u + 1];
+            if (e0 - s0 > WG_WEDGE_LANE_MAX) {
+                heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
+                continue;
             }
-            const bool has = p < e;
-            if (wg_ballot(has || bits != 0ull) == 0ull)
-                break;
-            mark_block_wave(has, has ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
-            if (has)
-                ++p;
+            for (uint32_t p = s0; p < e0; ++p)
+                mark_block(csri_pos[p] >> blog2, wedge);
         }
     }
 }
@@ -670,38 +719,32 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
 __global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy,
                                                            const uint32_t* __restrict__ csri_off,
                                                            const uint32_t* __restrict__ csri_pos, int blog2_arg,
-                                                           uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                                                           wg_dev_counters* __restrict__ ctr, int gate)
+                                                           uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr,
+                                                           int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t n = ctr->heavy_len;
-    const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
-    for (uint32_t i = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
+    const uint32_t nwaves = gridDim.x * WG_WPB;
+    for (uint32_t i = blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
         const uint32_t u = heavy[i];
         const uint32_t s = csri_off[u], e = csri_off[u + 1];
-        for (uint32_t p0 = s; p0 < e; p0 += WG_WAVE) {
-            const uint32_t p = p0 + (uint32_t)wg_lane();
-            mark_block_wave(p < e, p < e ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
-        }
+        for (uint32_t p = s + (uint32_t)wg_lane(); p < e; p += WG_WAVE)
+            mark_block(csri_pos[p] >> blog2, wedge);
     }
 }
 
 // source/wedge.c:52-59
 __global__ void k_wedge_single(uint32_t u, const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
-                               int blog2_arg, uint32_t* __restrict__ wedge, uint32_t* __restrict__ queue,
-                               wg_dev_counters* __restrict__ ctr, int gate)
+                               int blog2_arg, uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t s = csri_off[u], e = csri_off[u + 1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t p0 = s + (blockIdx.x * blockDim.x + threadIdx.x) / WG_WAVE * WG_WAVE; p0 < e; p0 += stride) {
-        const uint32_t p = p0 + (uint32_t)wg_lane();
-        mark_block_wave(p < e, p < e ? csri_pos[p] >> blog2 : 0u, wedge, queue, ctr);
-    }
+    for (uint32_t p = s + blockIdx.x * blockDim.x + threadIdx.x; p < e; p += gridDim.x * blockDim.x)
+        mark_block(csri_pos[p] >> blog2, wedge);
 }
 
 __global__ void k_reached_bfs(const uint64_t* __restrict__ packed, const uint32_t* __restrict__ od, uint64_t V,
@@ -713,8 +756,8 @@ __global__ void k_reached_bfs(const uint64_t* __restrict__ packed, const uint32_
             nv += 1;
             ne += od[v];
         }
-    add_count(nv, &c->reached_verts);
-    add_count(ne, &c->reached_edges);
+    add_count(nv, c, WG_CNT_REACHED_V);
+    add_count(ne, c, WG_CNT_REACHED_E);
 }
 
 template <typename T>
@@ -727,8 +770,8 @@ __global__ void k_reached_min(const T* __restrict__ prop, T inf, const uint32_t*
             nv += 1;
             ne += od[v];
         }
-    add_count(nv, &c->reached_verts);
-    add_count(ne, &c->reached_edges);
+    add_count(nv, c, WG_CNT_REACHED_V);
+    add_count(ne, c, WG_CNT_REACHED_E);
 }
 
 __global__ void k_set_u64(uint64_t* p, uint64_t i, uint64_t v)
@@ -788,7 +831,6 @@ __global__ void k_loop_init(wg_dev_counters* c, unsigned int sched, unsigned lon
     c->fiters = 0u;
     c->activity = 0ull;
     c->activated = 0ull;
-    c->queue_len = 0u;
     c->heavy_len = 0u;
     c->last_activity = initial_activity;
     c->last_activated = 0ull;
@@ -799,10 +841,13 @@ __global__ void k_loop_init(wg_dev_counters* c, unsigned int sched, unsigned lon
 }
 
 // End of a pull: account the iteration, stop at zero activity, choose the
-// next schedule.  The pull consumed the queued blocks, so the queue restarts.
+// next schedule.
 __global__ void k_decide(wg_dev_counters* c)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || c->done)
+    if (blockIdx.x != 0 || threadIdx.x >= WG_WAVE || c->done)
+        return;
+    fold_counts(c);
+    if (threadIdx.x != 0)
         return;
     const unsigned long long act = c->activity, nact = c->activated;
     c->iters += 1u;
@@ -810,7 +855,6 @@ __global__ void k_decide(wg_dev_counters* c)
         c->fiters += 1u;
     c->activity = 0ull;
     c->activated = 0ull;
-    c->queue_len = 0u;
     c->heavy_len = 0u;
     c->frontier_verts = 0ull;
     c->last_activity = act;
@@ -825,9 +869,12 @@ __global__ void k_decide(wg_dev_counters* c)
 
 // ---- host side ----------------------------------------------------------------------
 
+// Folds the counter slots and copies the counters to the host mirror.
 static int read_counters(wg_ctx* ctx)
 {
-    WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, sizeof(wg_dev_counters), hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, WG_CTR_HOST_BYTES, hipMemcpyDeviceToHost, ctx->stream));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return WG_OK;
 }
@@ -837,7 +884,7 @@ static int clear_wedge_state(wg_ctx* ctx)
     int rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u);
     if (rc)
         return rc;
-    rc = reset_counters(ctx, RST_QUEUE);
+    rc = reset_counters(ctx, RST_HEAVY);
     ctx->wedge_log2 = -1;
     return rc;
 }
@@ -924,12 +971,30 @@ extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
     return WG_OK;
 }
 
+// Launch shapes.  Frontier pulls: one wavefront per 64 frontier words at the
+// finest granularity.  Frontier build: one wavefront per 64 bitmask words.
+// End-of-iteration bitmask kernels: one thread per bitmask word.
+static uint32_t frontier_grid(const wg_ctx* ctx)
+{
+    return wg_grid(ctx, ctx->wedge_words, WG_WPB * WG_WAVE, 8);
+}
+
+static uint32_t wedge_build_grid(const wg_ctx* ctx)
+{
+    return wg_grid(ctx, ctx->bm_words, WG_WPB * WG_WAVE, 8);
+}
+
+static uint32_t advance_grid(const wg_ctx* ctx)
+{
+    return wg_grid(ctx, ctx->bm_words, 256, 32);
+}
+
 static int check_block_log2(int b)
 {
     return b >= WG_MIN_BLOCK_LOG2 && b <= WG_MAX_BLOCK_LOG2;
 }
 
-// A queued frontier must be consumed with the granularity it was built at.
+// A marked frontier must be consumed with the granularity it was built at.
 static int prepare_frontier_build(wg_ctx* ctx, int blog2)
 {
     if (ctx->wedge_log2 >= 0 && ctx->wedge_log2 != blog2) {
@@ -953,7 +1018,7 @@ extern "C" int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
     hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, dv, ctx->csri_off, ctx->csri_pos,
-                       block_log2, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_ALWAYS);
+                       block_log2, ctx->wedge, ctx->ctr, WG_GATE_ALWAYS);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
     return rc;
@@ -969,13 +1034,13 @@ extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
     if (rc || (rc = reset_counters(ctx, RST_HEAVY)))
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
-    hipLaunchKernelGGL(k_wedge_build, dim3(wg_grid(ctx, ctx->bm_words, 256, 8)), dim3(WG_BLOCK), 0, ctx->stream,
-                       ctx->bm_next, ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue,
-                       ctx->heavy_list, ctx->ctr, WG_GATE_ALWAYS);
+    hipLaunchKernelGGL(k_wedge_build, dim3(wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
+                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->heavy_list, ctx->ctr,
+                       WG_GATE_ALWAYS);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_ALWAYS);
+                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->ctr, WG_GATE_ALWAYS);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
     return rc;
@@ -1006,7 +1071,7 @@ extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* verti
     return WG_OK;
 }
 
-// A dense pull covers every edge: any queued blocks are dropped.
+// A dense pull covers every edge: any marked blocks are dropped.
 static int begin_pull(wg_ctx* ctx, int mode)
 {
     int rc = reset_counters(ctx, RST_ACTIVITY);
@@ -1019,9 +1084,9 @@ static int begin_pull(wg_ctx* ctx, int mode)
 
 static int end_frontier_pull(wg_ctx* ctx)
 {
-    // the queue has been consumed (its wedge words were zeroed by the pull)
+    // the pull zeroed every frontier word it consumed
     ctx->wedge_log2 = -1;
-    return reset_counters(ctx, RST_QUEUE);
+    return reset_counters(ctx, RST_HEAVY);
 }
 
 extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated)
@@ -1056,7 +1121,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         }
     } else {
         int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
-        hipLaunchKernelGGL(k_bfs_blocks, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
+        hipLaunchKernelGGL(k_bfs_blocks, dim3(frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
                            ctx->bm_next, packed, ctx->wedge, ctx->perm, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -1082,7 +1147,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
     } else {
         int slot = wg_prof_begin(ctx, WG_K_MIN_BLOCKS);
-        hipLaunchKernelGGL(k_min_blocks<P>, dim3(ctx->num_cu * 8), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr,
+        hipLaunchKernelGGL(k_min_blocks<P>, dim3(frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (typename P::T*)prop, ctx->od,
                            ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -1133,7 +1198,7 @@ extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
     if (rc)
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_ADVANCE);
-    const uint32_t grid = wg_grid(ctx, ctx->bm_words, 1024, 8);
+    const uint32_t grid = advance_grid(ctx);
     if (ctx->app == WG_APP_BFS)
         hipLaunchKernelGGL(k_bfs_advance, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
                            ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_ALWAYS);
@@ -1208,7 +1273,7 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
 static int enqueue_iteration(wg_ctx* ctx)
 {
     uint64_t* packed = (uint64_t*)ctx->prop[0];
-    const uint32_t qgrid = ctx->num_cu * 8;
+    const uint32_t qgrid = frontier_grid(ctx);
     if (ctx->app == WG_APP_BFS) {
         hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                            ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
@@ -1218,38 +1283,38 @@ static int enqueue_iteration(wg_ctx* ctx)
                                ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
                                ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
                                WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_bfs_blocks, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+        hipLaunchKernelGGL(k_bfs_blocks, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
                            (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            ctx->wedge, ctx->perm, WG_GATE_FRONTIER);
     } else if (ctx->app == WG_APP_CC) {
         hipLaunchKernelGGL(k_min_tile<PolCC>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
                            (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
                            &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolCC>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+        hipLaunchKernelGGL(k_min_blocks<PolCC>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
                            (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
                            ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
     } else if (ctx->wkind == WG_WEIGHT_F64) {
         hipLaunchKernelGGL(k_min_tile<PolSF>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
                            (unsigned long long*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
                            ctx->ctr, &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolSF>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+        hipLaunchKernelGGL(k_min_blocks<PolSF>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
                            (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (unsigned long long*)ctx->prop[0], ctx->od,
                            ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
     } else {
         hipLaunchKernelGGL(k_min_tile<PolSU>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
                            (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
                            &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolSU>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->queue, ctx->ctr, 0,
+        hipLaunchKernelGGL(k_min_blocks<PolSU>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
                            (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
                            ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
     }
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr);
-    hipLaunchKernelGGL(k_wedge_build, dim3(wg_grid(ctx, ctx->bm_words, 256, 8)), dim3(WG_BLOCK), 0, ctx->stream,
-                       ctx->bm_next, ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->heavy_list,
-                       ctx->ctr, WG_GATE_FRONTIER);
+    hipLaunchKernelGGL(k_wedge_build, dim3(wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
+                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->ctr,
+                       WG_GATE_FRONTIER);
     hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_FRONTIER);
-    const uint32_t agrid = wg_grid(ctx, ctx->bm_words, 1024, 8);
+                       ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+    const uint32_t agrid = advance_grid(ctx);
     if (ctx->app == WG_APP_BFS)
         hipLaunchKernelGGL(k_bfs_advance, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
                            ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_LIVE);
@@ -1286,7 +1351,7 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
                        ctx->app == WG_APP_BFS ? (unsigned long long)(ctx->V - 1) : 0ull);
     if (ctx->app != WG_APP_CC) // the root's blocks, if the first pull is frontier-filtered
         hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, ctx->root_dev, ctx->csri_off,
-                           ctx->csri_pos, 0, ctx->wedge, ctx->queue, ctx->ctr, WG_GATE_FRONTIER);
+                           ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_LOOP);
@@ -1301,7 +1366,7 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
         for (uint32_t i = 0; i < batch; ++i)
             if ((rc = enqueue_iteration(ctx)))
                 return rc;
-        WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, sizeof(wg_dev_counters), hipMemcpyDeviceToHost, ctx->stream));
+        WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, WG_CTR_HOST_BYTES, hipMemcpyDeviceToHost, ctx->stream));
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         if (ctx->hctr->done)
             break;
