@@ -50,6 +50,20 @@ def pr_algorithmic_bytes(V: int, E: int, fp32: bool) -> int:
     return 4 * E + 4 * (V + 1) + 4 * V + 2 * c * V
 
 
+def measured_traffic(config: str, fp32: bool):
+    """HBM bytes per PageRank iteration from the committed rocprofv3 counter
+    summary of the same workload (FETCH_SIZE and WRITE_SIZE passes, scaled by
+    a copy kernel of known size), or None when no summary covers it."""
+    name = f"k_pr_tile_rmat_{config}_{'fp32_' if fp32 else ''}hbm_traffic.md"
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(g, damping, budget_s=20.0):
     """The oracle's PageRank on the same graph, all host threads, a bounded
     number of iterations (time per iteration -> GTEPS)."""
@@ -160,7 +174,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.config, args.fp32),
             "algorithmic_bytes_per_launch": alg,
             "avg_launch_ms": round(kern_ms, 5),
         },
