@@ -10,7 +10,10 @@ GTEPS.  The same run reports BFS and SSSP from the max-out-degree root
 (GTEPS = edges whose source was reached / run time), the HBM roofline of the
 PageRank kernels measured with HIP events on the engine's stream, and the CPU
 oracle (a C restatement of Wedge's pull engine) timed on a bounded sample on
-the host cores.
+the host cores.  BFS and SSSP run under the plain threshold rule and under the
+learned schedule (the packaged model) on this graph and on a 4096x6144 road
+grid (integer weights, seed 3), with learned/plain speedups and an identity
+check of the outputs.
 
 Prints one JSON line.
 """
@@ -64,6 +67,37 @@ def measured_traffic(config: str, fp32: bool):
     return None
 
 
+def run_best(ctx, app, root, reps, **kw):
+    best = None
+    for _ in range(reps):
+        res, st = ctx.run_bfs(root, **kw) if app == "bfs" else ctx.run_sssp(root, **kw)
+        if best is None or st["running_ms"] < best[1]["running_ms"]:
+            best = (res, st)
+    return best
+
+
+def schedule_pair(ctx, g, root, reps, weights):
+    """BFS and SSSP under the plain rule and the learned schedule: GTEPS
+    (edges whose source was reached / run time), iterations, and whether the
+    outputs are identical."""
+    out = {"vertices": g.num_vertices, "edges": g.num_edges, "root": root}
+    for app in ("bfs", "sssp"):
+        res = {}
+        for sched in ("plain", "learned"):
+            r, st = run_best(ctx, app, root, reps, schedule=sched)
+            res[sched] = (r, st)
+            out[f"{app}_{sched}"] = {
+                "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
+                "ms": round(st["running_ms"], 3), "iterations": st["iterations"],
+                "frontier_iterations": st["frontier_iterations"],
+            }
+        out[f"{app}_learned_vs_plain"] = round(res["plain"][1]["running_ms"] / res["learned"][1]["running_ms"], 3)
+        out[f"{app}_identical"] = bool(np.array_equal(res["plain"][0], res["learned"][0]))
+        log(f"{app} ({weights} weights for SSSP): plain {out[f'{app}_plain']}, learned {out[f'{app}_learned']}, "
+            f"x{out[f'{app}_learned_vs_plain']}, identical {out[f'{app}_identical']}")
+    return out
+
+
 def cpu_baseline(g, damping, budget_s=20.0):
     """The oracle's PageRank on the same graph, all host threads, a bounded
     number of iterations (time per iteration -> GTEPS)."""
@@ -101,6 +135,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-apps", action="store_true", help="skip BFS/SSSP")
     ap.add_argument("--bfs-reps", type=int, default=3)
+    ap.add_argument("--no-grid", action="store_true", help="skip the 4096x6144 road grid (plain vs learned)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -181,32 +216,17 @@ def main():
         "pr_rank_sum": rank_sum,
     }
 
-    # ---- BFS / SSSP on the same graph ----------------------------------------------
+    # ---- BFS / SSSP on the same graph, plain and learned schedules --------------------
     if not args.no_apps:
+        W.mlp_load()
         root = g.max_outdegree_vertex()
-        apps = {}
-        for sched in ("plain",):
-            best = None
-            for _ in range(args.bfs_reps):
-                packed, st = ctx.run_bfs(root, schedule=sched)
-                if best is None or st["running_ms"] < best["running_ms"]:
-                    best = st
-            apps[f"bfs_{sched}"] = {
-                "gteps": round(best["edges_reached"] / (best["running_ms"] * 1e-3) / 1e9, 3),
-                "ms": round(best["running_ms"], 3), "iterations": best["iterations"],
-                "frontier_iterations": best["frontier_iterations"], "edges_reached": best["edges_reached"],
-                "root": root,
-            }
-            log(f"BFS {sched}: {apps[f'bfs_{sched}']}")
-        dist, st = ctx.run_sssp(root)
-        dist, st = ctx.run_sssp(root)
-        apps["sssp_f64_plain"] = {
-            "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
-            "ms": round(st["running_ms"], 3), "iterations": st["iterations"],
-            "frontier_iterations": st["frontier_iterations"],
-        }
-        log(f"SSSP: {apps['sssp_f64_plain']}")
-        out["apps"] = apps
+        out["apps"] = {"rmat": schedule_pair(ctx, g, root, args.bfs_reps, "f64")}
+        if not args.no_grid:
+            t0 = time.time()
+            road = W.grid(4096, 6144, seed=3)
+            ctx.load(road)
+            log(f"road grid 4096x6144 seed 3: V={road.num_vertices} E={road.num_edges} ({time.time() - t0:.1f} s)")
+            out["apps"]["road_grid_4096x6144"] = schedule_pair(ctx, road, road.max_outdegree_vertex(), 2, "u32")
     ctx.close()
 
     if not args.no_cpu:

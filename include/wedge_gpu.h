@@ -164,8 +164,10 @@ int wg_edge_pull_cc(wg_ctx* ctx, int mode, uint64_t* edges_activated);
 int wg_edge_pull_sssp(wg_ctx* ctx, int mode, uint64_t* edges_activated);
 
 /* Edges and vertices activated by the last pull whose edges_activated
- * out-parameter was non-NULL (values already on the host; no device wait). */
-int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices);
+ * out-parameter was non-NULL, and how many of those vertices were reached for
+ * the first time (BFS: all; SSSP: distance was infinite; CC: 0).  Values
+ * already on the host; no device wait. */
+int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices, uint64_t* newly_reached);
 
 /* wedge_generate_frontier (source/wedge.c:63-155): marks every 2^block_log2
  * edge block that holds an out-edge of a vertex activated by the last pull,
@@ -187,8 +189,9 @@ int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices);
  * next pull's mode (and Wedge granularity); every pull, frontier-build and
  * advance kernel reads that choice and exits at once when not selected.  The
  * host enqueues iterations in growing batches and never waits per iteration.
- * schedule: 0 plain threshold rule, 1 learned (MLP on the device, plain rule
- * if no model is set), 2 always dense, 3 always frontier. */
+ * schedule: 0 plain threshold rule, 1 learned (the model of
+ * wg_set_schedule_model evaluated on the device; the plain rule if no model is
+ * set), 2 always dense, 3 always frontier. */
 typedef struct wg_loop_params {
     int schedule;
     uint32_t threshold_pct;  /* plain rule: dense iff activity > E * pct / 100 */
@@ -209,6 +212,71 @@ int wg_read_sssp(wg_ctx* ctx, void* host_dist);
 /* Activity counters of the current state: vertices reached (BFS: visited;
  * SSSP: finite distance) and the sum of their out-degrees. */
 int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges);
+
+/* ---- learned schedule --------------------------------------------------------
+ * No reference counterpart: replaces the fixed threshold rule of
+ * source/execution_bfs.c:141 (and its cc/sssp twins).  An FP32 MLP maps
+ * WG_MLP_IN features of the state before a pull to the predicted log2 time of
+ * each of WG_MLP_ACTIONS pull choices, as a learned correction to an analytic
+ * log2 work estimate of each choice; the cheapest is taken.  It only picks
+ * schedules, so results are the same whatever it predicts.
+ *
+ * Features (wg_mlp_features in wedge_amd/csrc/wg_mlp_math.h) are scale-free,
+ * with L(x) = log2(1 + x), fv / fe the vertices / out-edges activated by the
+ * last pull and u the vertices not reached yet (BFS, SSSP): L(fv) - L(V),
+ * L(fe) - L(E), L(u) - L(V), L(E) - L(V), L(fe) - L(fv), L(iteration) - L(V)/2,
+ * app is BFS, app is CC, app is SSSP, L(fe) - L(V), L(fv) - L(E), L(u) - L(E)
+ * (the u terms are 0 for CC).
+ * The plain rule's choice stands unless the model predicts the cheapest
+ * action to be faster than it by more than a factor 2^margin[app] (margins
+ * for BFS, CC, SSSP are part of the model).
+ * Actions: 0 dense; 1..5 frontier at block_log2 3, 4, 5, 6, 8.  BFS may take
+ * any action; CC and SSSP only 0 and 2 (dense, or frontier at the default
+ * granularity): their iteration count depends on the granularity, which a
+ * per-pull cost does not see, while a BFS level is the same in every mode.
+ * Parameter layout (floats): lo[IN], hi[IN] (inputs are clamped to the
+ * training range), mean[IN], inv_std[IN], W1[IN][H], b1[H], W2[H][H], b2[H],
+ * W3[H][A], b3[A], margin[3], lv_max; W[i][j] is input i to unit j.  CC and
+ * SSSP keep the plain rule on graphs with log2(1 + V) > lv_max (larger than
+ * the training graphs). */
+#define WG_MLP_IN 12
+#define WG_MLP_HIDDEN 64
+#define WG_MLP_ACTIONS 6
+#define WG_MLP_PARAMS                                                                                    \
+    (4 * WG_MLP_IN + WG_MLP_IN * WG_MLP_HIDDEN + WG_MLP_HIDDEN + WG_MLP_HIDDEN * WG_MLP_HIDDEN + \
+     WG_MLP_HIDDEN + WG_MLP_HIDDEN * WG_MLP_ACTIONS + WG_MLP_ACTIONS + 4)
+
+/* Mode and block granularity of action a (WG_E_INVAL if out of range). */
+int wg_mlp_action(int action, int* mode, int* block_log2);
+
+/* Sets (params != NULL, count == WG_MLP_PARAMS) or clears (params == NULL)
+ * the model the device loop uses for schedule 1. */
+int wg_set_schedule_model(wg_ctx* ctx, const float* params, uint64_t count);
+
+/* Batch inference with the context's model and the device decision code:
+ * features[n][WG_MLP_IN], work estimates base[n][WG_MLP_ACTIONS] (may be
+ * NULL: 0) and the plain rule's actions plain[n] (may be NULL: none) ->
+ * predicted log2 times out[n][WG_MLP_ACTIONS] (may be NULL) and the chosen
+ * actions (may be NULL). */
+int wg_mlp_infer(wg_ctx* ctx, int app, const float* features, const float* base, const int32_t* plain, uint64_t n,
+                 float* out, int32_t* actions);
+
+/* Training on the device (hand-written kernels; minibatch Adam on the squared
+ * error of the taken action's output, fixed-order gradient sums, so a run is
+ * reproducible).  samples: features[n][IN], the action taken and the target
+ * of its network output (measured log2 time minus the work estimate).
+ * params: in = initial parameters including lo/hi/mean/inv_std, the
+ * margins and lv_max (none of which is trained), out = trained.  final_loss: mean
+ * squared error over all samples after training. */
+typedef struct wg_mlp_train_opts {
+    uint32_t epochs;
+    uint32_t batch;     /* samples per step */
+    float lr;           /* Adam step size */
+    float beta1, beta2, eps;
+    uint64_t seed;      /* minibatch order */
+} wg_mlp_train_opts;
+int wg_mlp_train(wg_ctx* ctx, const float* features, const int32_t* actions, const float* targets, uint64_t n,
+                 const wg_mlp_train_opts* opts, float* params, float* final_loss);
 
 /* ---- measurement ---------------------------------------------------------- */
 

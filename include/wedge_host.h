@@ -126,9 +126,11 @@ typedef struct wg_trace_row {
     uint32_t pad;
     uint64_t frontier_vertices; /* vertices active at iteration start */
     uint64_t frontier_edges;    /* their out-edges (reference num_edges_active) */
-    uint64_t unvisited;         /* BFS: unvisited vertices; CC/SSSP: 0 */
-    double pull_ms;
-    double wedge_ms;
+    uint64_t unvisited;         /* BFS / SSSP: vertices not reached yet; CC: 0 */
+    double pull_ms;             /* host wall time of the pull call */
+    double wedge_ms;            /* host wall time of frontier build + advance */
+    double gpu_pull_ms;         /* device time of the pull kernels (HIP events) */
+    double gpu_build_ms;        /* device time of the frontier build for the next pull */
 } wg_trace_row;
 
 /* Runs an application end to end on a context that already holds the graph.
@@ -145,11 +147,32 @@ int wg_trace_copy(wg_trace_row* rows, uint64_t max_rows);
 
 /* ---- learned schedule ------------------------------------------------------ */
 
-/* Loads / saves the schedule model (FP32 MLP weights) used by WG_SCHED_LEARNED. */
+/* The schedule model (FP32 MLP, layout in wedge_gpu.h) used by
+ * WG_SCHED_LEARNED: load / save a model file, set / get the parameters
+ * (WG_MLP_PARAMS floats).  Without a model the learned schedule falls back to
+ * the plain rule.  The device loop evaluates the same model on the GPU. */
 int wg_mlp_load(const char* path);
 int wg_mlp_save(const char* path);
+int wg_mlp_set(const float* params, uint64_t count);
+int wg_mlp_get(float* params, uint64_t count);
 void wg_mlp_unload(void);
 int wg_mlp_loaded(void);
+
+/* Feature vector (WG_MLP_IN floats) of a loop state, as the schedule sees it:
+ * app, iterations completed, graph size, vertices and out-edges activated by
+ * the last pull, vertices not reached yet (BFS, SSSP; 0 for CC). */
+int wg_mlp_feature_vector(int app, uint64_t iteration, uint64_t num_vertices, uint64_t num_edges,
+                          uint64_t frontier_vertices, uint64_t frontier_edges, uint64_t unvisited, float* out);
+
+/* log2 work estimate of every action (WG_MLP_ACTIONS floats) for a loop
+ * state: the part of the prediction that is not learned. */
+int wg_mlp_baseline(uint64_t num_vertices, uint64_t num_edges, uint64_t frontier_edges, float* out);
+
+/* Host evaluation of the loaded model: predicted log2 times of every action
+ * (network output + base; base may be NULL) into out (WG_MLP_ACTIONS floats,
+ * may be NULL); returns the action chosen for app (>= 0), settled against the
+ * plain rule's action plain_action (-1: none), or an error code. */
+int wg_mlp_predict(int app, const float* features, const float* base, int plain_action, float* out);
 
 /* Loads a graph into a context (wraps wg_load_csc). */
 int wg_ctx_load_graph(wg_ctx* ctx, const wg_graph* g);

@@ -93,6 +93,18 @@ _p = C.c_void_p
 _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
 _f64p = C.POINTER(C.c_double)
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+
+# learned schedule model shape (include/wedge_gpu.h)
+MLP_IN, MLP_HIDDEN, MLP_ACTIONS = 12, 64, 6
+MLP_PARAMS = 4 * MLP_IN + MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
+    MLP_HIDDEN * MLP_ACTIONS + MLP_ACTIONS + 4  # + margins for BFS, CC, SSSP and lv_max
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
 
 # (name, restype, argtypes) of every C-ABI entry point in include/wedge_gpu.h
 GPU_SYMBOLS = [
@@ -117,7 +129,7 @@ GPU_SYMBOLS = [
     ("wg_edge_pull_bfs", C.c_int, [_p, C.c_int, _u64p]),
     ("wg_edge_pull_cc", C.c_int, [_p, C.c_int, _u64p]),
     ("wg_edge_pull_sssp", C.c_int, [_p, C.c_int, _u64p]),
-    ("wg_last_pull_counts", C.c_int, [_p, _u64p, _u64p]),
+    ("wg_last_pull_counts", C.c_int, [_p, _u64p, _u64p, _u64p]),
     ("wg_wedge_generate_frontier", C.c_int, [_p, C.c_int]),
     ("wg_wedge_activate_single_vertex", C.c_int, [_p, C.c_uint64, C.c_int]),
     ("wg_advance_frontier", C.c_int, [_p, _u64p]),
@@ -129,6 +141,10 @@ GPU_SYMBOLS = [
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
+    ("wg_mlp_action", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("wg_set_schedule_model", C.c_int, [_p, _f32p, C.c_uint64]),
+    ("wg_mlp_infer", C.c_int, [_p, C.c_int, _f32p, _f32p, _i32p, C.c_uint64, _f32p, _i32p]),
+    ("wg_mlp_train", C.c_int, [_p, _f32p, _i32p, _f32p, C.c_uint64, C.POINTER(MlpTrainOpts), _f32p, _f32p]),
 ]
 
 
@@ -161,7 +177,7 @@ class TraceRow(C.Structure):
     _fields_ = [
         ("iteration", C.c_uint32), ("mode", C.c_uint32), ("block_log2", C.c_int32), ("pad", C.c_uint32),
         ("frontier_vertices", C.c_uint64), ("frontier_edges", C.c_uint64), ("unvisited", C.c_uint64),
-        ("pull_ms", C.c_double), ("wedge_ms", C.c_double),
+        ("pull_ms", C.c_double), ("wedge_ms", C.c_double), ("gpu_pull_ms", C.c_double), ("gpu_build_ms", C.c_double),
     ]
 
 
@@ -194,8 +210,14 @@ HOST_SYMBOLS = [
     ("wg_trace_copy", C.c_int, [C.POINTER(TraceRow), C.c_uint64]),
     ("wg_mlp_load", C.c_int, [C.c_char_p]),
     ("wg_mlp_save", C.c_int, [C.c_char_p]),
+    ("wg_mlp_set", C.c_int, [_f32p, C.c_uint64]),
+    ("wg_mlp_get", C.c_int, [_f32p, C.c_uint64]),
     ("wg_mlp_unload", None, []),
     ("wg_mlp_loaded", C.c_int, []),
+    ("wg_mlp_feature_vector", C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                                        C.c_uint64, _f32p]),
+    ("wg_mlp_baseline", C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, _f32p]),
+    ("wg_mlp_predict", C.c_int, [C.c_int, _f32p, _f32p, C.c_int, _f32p]),
     ("wg_ctx_load_graph", C.c_int, [_p, C.POINTER(GraphStruct)]),
 ]
 
@@ -478,9 +500,10 @@ class Context:
         return int(a.value)
 
     def last_pull_counts(self):
-        e, v = C.c_uint64(), C.c_uint64()
-        self._chk(self._lib.wg_last_pull_counts(self._h, C.byref(e), C.byref(v)), "wg_last_pull_counts")
-        return int(e.value), int(v.value)
+        """(edges activated, vertices activated, vertices reached for the first time) of the last pull."""
+        e, v, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.wg_last_pull_counts(self._h, C.byref(e), C.byref(v), C.byref(f)), "wg_last_pull_counts")
+        return int(e.value), int(v.value), int(f.value)
 
     def wedge_generate_frontier(self, block_log2=DEFAULT_BLOCK_LOG2):
         self._chk(self._lib.wg_wedge_generate_frontier(self._h, block_log2), "wg_wedge_generate_frontier")
@@ -556,6 +579,46 @@ class Context:
         self._chk(_lib_host().wg_run_cc(self._h, C.byref(o), _ptr(out, C.c_uint32), C.byref(st)), "wg_run_cc")
         return out, st.as_dict()
 
+    def set_schedule_model(self, params: np.ndarray | None):
+        """Model for the device loop's learned schedule (None clears it)."""
+        if params is None:
+            self._chk(self._lib.wg_set_schedule_model(self._h, None, 0), "wg_set_schedule_model")
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        self._chk(self._lib.wg_set_schedule_model(self._h, _ptr(p, C.c_float), p.size), "wg_set_schedule_model")
+
+    def mlp_infer(self, X: np.ndarray, app: int = APP_BFS, base: np.ndarray | None = None,
+                  plain: np.ndarray | None = None):
+        """Device batch inference: predictions [n][A] and the actions chosen for app [n]."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+        n = X.shape[0]
+        Y = np.empty((n, MLP_ACTIONS), np.float32)
+        A = np.empty(n, np.int32)
+        bp = None
+        if base is not None:
+            base = np.ascontiguousarray(base, np.float32).reshape(n, MLP_ACTIONS)
+            bp = _ptr(base, C.c_float)
+        pp = None
+        if plain is not None:
+            plain = np.ascontiguousarray(plain, np.int32)
+            pp = _ptr(plain, C.c_int32)
+        self._chk(self._lib.wg_mlp_infer(self._h, app, _ptr(X, C.c_float), bp, pp, n, _ptr(Y, C.c_float),
+                                         _ptr(A, C.c_int32)), "wg_mlp_infer")
+        return Y, A
+
+    def mlp_train(self, X, actions, targets, params, epochs=200, batch=512, lr=1e-3, seed=1,
+                  beta1=0.9, beta2=0.999, eps=1e-8) -> tuple[np.ndarray, float]:
+        """Device training (hand-written Adam kernels); returns (params, mean squared error)."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, MLP_IN)
+        A = np.ascontiguousarray(actions, np.int32)
+        T = np.ascontiguousarray(targets, np.float32)
+        P = np.ascontiguousarray(params, np.float32).copy()
+        o = MlpTrainOpts(epochs=epochs, batch=batch, lr=lr, beta1=beta1, beta2=beta2, eps=eps, seed=seed)
+        loss = C.c_float()
+        self._chk(self._lib.wg_mlp_train(self._h, _ptr(X, C.c_float), _ptr(A, C.c_int32), _ptr(T, C.c_float),
+                                         X.shape[0], C.byref(o), _ptr(P, C.c_float), C.byref(loss)), "wg_mlp_train")
+        return P, float(loss.value)
+
     def run_sssp(self, root: int, **kw):
         o = run_opts(root=root, **kw)
         f64 = self.graph is not None and self.graph.weight_kind == WEIGHT_F64
@@ -563,6 +626,77 @@ class Context:
         st = RunStats()
         self._chk(_lib_host().wg_run_sssp(self._h, C.byref(o), out.ctypes.data, C.byref(st)), "wg_run_sssp")
         return out, st.as_dict()
+
+
+# ---- learned schedule ------------------------------------------------------------------
+
+DEFAULT_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "schedule_mlp.txt")
+
+
+def mlp_action(a: int) -> tuple[int, int]:
+    """(mode, block_log2) of schedule action a."""
+    m, b = C.c_int(), C.c_int()
+    _check(_lib_gpu().wg_mlp_action(a, C.byref(m), C.byref(b)), "wg_mlp_action")
+    return m.value, b.value
+
+
+def mlp_set(params: np.ndarray):
+    p = np.ascontiguousarray(params, np.float32)
+    _check(_lib_host().wg_mlp_set(_ptr(p, C.c_float), p.size), "wg_mlp_set")
+
+
+def mlp_get() -> np.ndarray:
+    p = np.empty(MLP_PARAMS, np.float32)
+    _check(_lib_host().wg_mlp_get(_ptr(p, C.c_float), p.size), "wg_mlp_get")
+    return p
+
+
+def mlp_load(path: str = DEFAULT_MODEL_PATH):
+    _check(_lib_host().wg_mlp_load(path.encode()), f"wg_mlp_load {path}")
+
+
+def mlp_save(path: str):
+    _check(_lib_host().wg_mlp_save(path.encode()), f"wg_mlp_save {path}")
+
+
+def mlp_unload():
+    _lib_host().wg_mlp_unload()
+
+
+def mlp_loaded() -> bool:
+    return bool(_lib_host().wg_mlp_loaded())
+
+
+def mlp_features(app: int, iteration: int, V: int, E: int, frontier_vertices: int, frontier_edges: int,
+                 unvisited: int) -> np.ndarray:
+    """The schedule's feature vector of one loop state (host library)."""
+    x = np.empty(MLP_IN, np.float32)
+    _check(_lib_host().wg_mlp_feature_vector(app, iteration, V, E, frontier_vertices, frontier_edges, unvisited,
+                                             _ptr(x, C.c_float)), "wg_mlp_feature_vector")
+    return x
+
+
+def mlp_base(V: int, E: int, frontier_edges: int) -> np.ndarray:
+    """log2 work estimate of each action for a loop state (the unlearned part)."""
+    b = np.empty(MLP_ACTIONS, np.float32)
+    _check(_lib_host().wg_mlp_baseline(V, E, frontier_edges, _ptr(b, C.c_float)), "wg_mlp_baseline")
+    return b
+
+
+def mlp_predict(x: np.ndarray, app: int = APP_BFS, base: np.ndarray | None = None,
+                plain: int = -1) -> tuple[int, np.ndarray]:
+    """Host evaluation of the loaded model: (action chosen for app, settled
+    against the plain rule's action plain (-1: none), predictions)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty(MLP_ACTIONS, np.float32)
+    bp = None
+    if base is not None:
+        base = np.ascontiguousarray(base, np.float32)
+        bp = _ptr(base, C.c_float)
+    a = _lib_host().wg_mlp_predict(app, _ptr(x, C.c_float), bp, plain, _ptr(y, C.c_float))
+    if a < 0:
+        raise WedgeError(a, "wg_mlp_predict")
+    return a, y
 
 
 def trace_rows() -> list[dict]:

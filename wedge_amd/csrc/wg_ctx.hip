@@ -445,6 +445,7 @@ extern "C" void wg_close(wg_ctx* ctx)
         free(ctx->prof.ev);
         free(ctx->prof.kid);
     }
+    (void)hipFree(ctx->mlp);
     (void)hipFree(ctx->ctr);
     (void)hipHostFree(ctx->hctr);
     (void)hipStreamDestroy(ctx->stream);

@@ -41,7 +41,7 @@
 // served in parallel.  A fold (one wavefront) adds the slots into the scalar
 // fields below and zeroes them.
 #define WG_CSLOTS 32
-enum { WG_CNT_ACTIVITY = 0, WG_CNT_ACTIVATED, WG_CNT_FRONTIER, WG_CNT_REACHED_V, WG_CNT_REACHED_E, WG_CNT_N };
+enum { WG_CNT_ACTIVITY = 0, WG_CNT_ACTIVATED, WG_CNT_FRESH, WG_CNT_FRONTIER, WG_CNT_REACHED_V, WG_CNT_REACHED_E, WG_CNT_N };
 struct __attribute__((aligned(256))) wg_count_slot {
     unsigned long long v[WG_CNT_N];
     unsigned long long pad[32 - WG_CNT_N];
@@ -52,6 +52,7 @@ struct __attribute__((aligned(256))) wg_count_slot {
 struct wg_dev_counters {
     unsigned long long activity;        // sum of out-degrees of vertices activated this pull
     unsigned long long activated;       // number of vertices activated this pull
+    unsigned long long fresh;           // of which reached for the first time (BFS, SSSP)
     unsigned long long frontier_verts;  // popcount of the frontier made current by advance
     unsigned int pad1;
     unsigned int heavy_len;             // high out-degree vertices deferred by the frontier build
@@ -75,7 +76,11 @@ struct wg_dev_counters {
     unsigned long long threshold;       // plain rule: dense iff activity > threshold
     unsigned long long last_activity;   // activity of the last completed pull
     unsigned long long last_activated;
-    unsigned long long unvisited;       // BFS feature for the learned rule
+    unsigned long long unvisited;       // vertices not reached yet (BFS, SSSP): learned-rule feature
+    unsigned int app;                   // features of the learned rule: application,
+    unsigned int pad2;
+    unsigned long long num_vertices;    // graph size
+    unsigned long long num_edges;
     wg_count_slot slots[WG_CSLOTS];
 };
 #define WG_CTR_HOST_BYTES offsetof(wg_dev_counters, slots)
@@ -159,6 +164,8 @@ struct wg_ctx {
     uint64_t wedge_words;     // 32-bit words allocated (finest granularity)
     int wedge_log2;           // granularity of the marked blocks, -1 if none are marked
     uint32_t* heavy_list;     // frontier build: deferred high out-degree vertices
+    float* mlp;               // schedule model parameters (WG_MLP_PARAMS floats)
+    int mlp_set;              // a model is set for the learned schedule
     wg_dev_counters* ctr;     // device
     wg_dev_counters* hctr;    // pinned host mirror
     uint32_t root_dev;        // BFS / SSSP root in device numbering

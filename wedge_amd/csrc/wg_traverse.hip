@@ -25,7 +25,7 @@
 // non-zero ones and spreads their marked blocks over the lanes of a wavefront,
 // so the edges it reads follow the marked blocks; the word scan itself is
 // E / 2^b / 8 bytes.
-#include "wg_internal.h"
+#include "wg_mlp_wave.h"
 
 __device__ __forceinline__ bool test_bit64(const uint64_t* __restrict__ bm, uint32_t i)
 {
@@ -36,6 +36,7 @@ __device__ __forceinline__ bool test_bit64(const uint64_t* __restrict__ bm, uint
 struct act_tally {
     unsigned long long edges;
     unsigned long long verts;
+    unsigned long long fresh; // activated vertices reached for the first time
 };
 
 // Marks vertex v activated (the reference's cond_set_bit + out-degree count,
@@ -93,6 +94,7 @@ __device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* _
 {
     add_count(t.edges, c, WG_CNT_ACTIVITY);
     add_count(t.verts, c, WG_CNT_ACTIVATED);
+    add_count(t.fresh, c, WG_CNT_FRESH);
 }
 
 __device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, int f)
@@ -102,6 +104,8 @@ __device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, i
         return &c->activity;
     case WG_CNT_ACTIVATED:
         return &c->activated;
+    case WG_CNT_FRESH:
+        return &c->fresh;
     case WG_CNT_FRONTIER:
         return &c->frontier_verts;
     case WG_CNT_REACHED_V:
@@ -111,9 +115,10 @@ __device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, i
     }
 }
 
-// Adds the slots into the scalar counts and zeroes them.  Called by the
-// first wavefront of a launch only; nothing else may be adding to the slots.
-__device__ void fold_counts(wg_dev_counters* c)
+// Adds the slots into the scalar counts and zeroes them; every lane gets the
+// slot totals in tot (if given).  Called by the first wavefront of a launch
+// only; nothing else may be adding to the slots.
+__device__ void fold_counts(wg_dev_counters* c, unsigned long long* tot = nullptr)
 {
     const int lane = wg_lane();
     for (int f = 0; f < WG_CNT_N; ++f) {
@@ -123,6 +128,8 @@ __device__ void fold_counts(wg_dev_counters* c)
             c->slots[lane].v[f] = 0ull;
         }
         x = wg_wave_sum(x);
+        if (tot)
+            tot[f] = x;
         if (lane == 0)
             *count_field(c, f) += x;
     }
@@ -148,6 +155,7 @@ __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
         if (flags & RST_ACTIVITY) {
             c->slots[t].v[WG_CNT_ACTIVITY] = 0ull;
             c->slots[t].v[WG_CNT_ACTIVATED] = 0ull;
+            c->slots[t].v[WG_CNT_FRESH] = 0ull;
         }
         if (flags & RST_FRONTIER)
             c->slots[t].v[WG_CNT_FRONTIER] = 0ull;
@@ -156,6 +164,7 @@ __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
         if (flags & RST_ACTIVITY) {
             c->activity = 0ull;
             c->activated = 0ull;
+            c->fresh = 0ull;
         }
         if (flags & RST_FRONTIER)
             c->frontier_verts = 0ull;
@@ -191,7 +200,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
     const int lane = wg_lane();
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
     const uint64_t nwaves = (uint64_t)gridDim.x * (WG_BLOCK / WG_WAVE);
-    act_tally act = { 0ull, 0ull };
+    act_tally act = { 0ull, 0ull, 0ull };
     for (uint64_t w = (uint64_t)blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
         const uint64_t vis = visited[w];
         if (vis == ~0ull)
@@ -253,6 +262,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
         if (lane == 0 && dm)
             atomicOr((unsigned long long*)&next[w], dm);
     }
+    act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
 }
 
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
-    act_tally act = { 0ull, 0ull };
+    act_tally act = { 0ull, 0ull, 0ull };
     for (uint32_t c = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; c < n; c += nwaves) {
         const uint32_t v = hv[c];
         if (test_bit64(visited, v))
@@ -305,6 +315,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
             activate(next, od, v, act);
         }
     }
+    act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
 }
 
@@ -391,7 +402,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __rest
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
-    act_tally act = { 0ull, 0ull };
+    act_tally act = { 0ull, 0ull, 0ull };
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool first) {
         bool cand = false;
         uint32_t u = 0, v = 0;
@@ -413,6 +424,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __rest
             activate(next, od, v, act);
         }
     });
+    act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
 }
 
@@ -526,8 +538,11 @@ __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint
     if (m >= prop[v])
         return;
     const typename P::T old = atomic_min_t<typename P::T>(&prop[v], m);
-    if (old > m)
+    if (old > m) {
         activate(next, od, v, t);
+        if (old == P::INF)
+            t.fresh += 1ull;
+    }
 }
 
 // Dense pull over the edge tiles, one wavefront per tile: per-destination
@@ -554,7 +569,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
     const int lane = wg_lane();
     uint32_t* s_off = s_off_all[wave];
     T* s_min = s_min_all[wave];
-    act_tally act = { 0ull, 0ull };
+    act_tally act = { 0ull, 0ull, 0ull };
     const uint32_t stride = gridDim.x * WG_WPB;
     for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
         const uint32_t tv = tile_v[t];
@@ -629,6 +644,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
                 if (m != P::INF && m < prop[v]) {
                     const T old = atomic_min_t<T>(&prop[v], m);
                     changed = old > m;
+                    if (changed && old == P::INF)
+                        act.fresh += 1ull;
                 }
             }
             activate_contiguous(next, od, v0 + (uint32_t)k * WG_WAVE, changed, act);
@@ -651,7 +668,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __rest
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     typedef typename P::T T;
-    act_tally act = { 0ull, 0ull };
+    act_tally act = { 0ull, 0ull, 0ull };
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool) {
         if (ok) {
             const T m = P::val(prop, w, src[p], p, ovf);
@@ -798,31 +815,89 @@ __global__ void k_iota_u32(uint32_t* p, uint64_t n, const uint32_t* __restrict__
 #define LOOP_SCHED_DENSE 2u
 #define LOOP_SCHED_FRONTIER 3u
 
-// Schedule of the next pull from the activity of the last one.
-__device__ void loop_choose(wg_dev_counters* c, unsigned long long act)
+// Schedule inputs of the device loop.
+struct loop_cfg {
+    unsigned int sched;
+    unsigned int app;
+    int plain_blog2;
+    unsigned long long threshold;
+    unsigned long long nv, ne;
+};
+
+__device__ __forceinline__ loop_cfg read_cfg(const wg_dev_counters* c)
 {
-    c->blog2 = c->plain_blog2;
-    switch (c->sched) {
+    loop_cfg g;
+    g.sched = c->sched;
+    g.app = c->app;
+    g.plain_blog2 = c->plain_blog2;
+    g.threshold = c->threshold;
+    g.nv = c->num_vertices;
+    g.ne = c->num_edges;
+    return g;
+}
+
+// Schedule of the next pull from the state after the last one.  Called by all
+// lanes of one wavefront (the learned rule evaluates the model across it).
+__device__ void loop_choose_wave(const loop_cfg& g, const float* __restrict__ mlp, unsigned long long act,
+                                 unsigned long long nact, unsigned int iteration, unsigned long long unvisited, int* mode,
+                                 int* blog2)
+{
+    *blog2 = g.plain_blog2;
+    switch (g.sched) {
     case LOOP_SCHED_DENSE:
-        c->mode = WG_MODE_DENSE;
-        break;
+        *mode = WG_MODE_DENSE;
+        return;
     case LOOP_SCHED_FRONTIER:
-        c->mode = WG_MODE_FRONTIER;
-        break;
+        *mode = WG_MODE_FRONTIER;
+        return;
+    case LOOP_SCHED_LEARNED:
+        if (mlp && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
+            float x[WG_MLP_IN];
+            float base[WG_MLP_ACTIONS];
+            wg_mlp_features((int)g.app, iteration, g.nv, g.ne, nact, act, unvisited, x);
+            wg_mlp_base(g.nv, g.ne, act, base);
+            const int plain = wg_mlp_plain_action(act > g.threshold, g.plain_blog2);
+            const int a = wg_mlp_choose_wave(mlp, (int)g.app, x, base, plain, nullptr);
+            *mode = a == 0 ? WG_MODE_DENSE : WG_MODE_FRONTIER;
+            if (a)
+                *blog2 = WG_MLP_ACTION_BLOG2(a);
+            return;
+        }
+        // no model: the plain rule
+        [[fallthrough]];
     default:
         // source/execution_bfs.c:141
-        c->mode = act > c->threshold ? WG_MODE_DENSE : WG_MODE_FRONTIER;
-        break;
+        *mode = act > g.threshold ? WG_MODE_DENSE : WG_MODE_FRONTIER;
+        return;
     }
 }
 
-__global__ void k_loop_init(wg_dev_counters* c, unsigned int sched, unsigned long long threshold, int plain_blog2,
-                            unsigned int max_iters, unsigned long long initial_activity, int dense_first,
-                            unsigned long long unvisited)
+// One wavefront.  initial_vertices / initial_activity describe the state
+// before the first pull (the root, or every vertex for CC).
+__global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, unsigned int sched,
+                            unsigned long long threshold, int plain_blog2, unsigned int max_iters,
+                            unsigned long long initial_activity, unsigned long long initial_vertices, int dense_first,
+                            unsigned long long unvisited, unsigned int app, unsigned long long nv, unsigned long long ne)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
+    if (blockIdx.x != 0 || threadIdx.x >= WG_WAVE)
+        return;
+    loop_cfg g;
+    g.sched = sched;
+    g.app = app;
+    g.plain_blog2 = plain_blog2;
+    g.threshold = threshold;
+    g.nv = nv;
+    g.ne = ne;
+    int mode = 0, blog2 = plain_blog2;
+    loop_choose_wave(g, mlp, initial_activity, initial_vertices, 0u, unvisited, &mode, &blog2);
+    if (dense_first)
+        mode = WG_MODE_DENSE; // CC: source/execution_cc.c:103
+    if (threadIdx.x != 0)
         return;
     c->sched = sched;
+    c->app = app;
+    c->num_vertices = nv;
+    c->num_edges = ne;
     c->threshold = threshold;
     c->plain_blog2 = plain_blog2;
     c->max_iters = max_iters;
@@ -831,40 +906,53 @@ __global__ void k_loop_init(wg_dev_counters* c, unsigned int sched, unsigned lon
     c->fiters = 0u;
     c->activity = 0ull;
     c->activated = 0ull;
+    c->fresh = 0ull;
     c->heavy_len = 0u;
     c->last_activity = initial_activity;
-    c->last_activated = 0ull;
+    c->last_activated = initial_vertices;
     c->unvisited = unvisited;
-    loop_choose(c, initial_activity);
-    if (dense_first)
-        c->mode = WG_MODE_DENSE; // CC: source/execution_cc.c:103
+    c->mode = (unsigned int)mode;
+    c->blog2 = blog2;
 }
 
-// End of a pull: account the iteration, stop at zero activity, choose the
-// next schedule.
-__global__ void k_decide(wg_dev_counters* c)
+// End of a pull (one wavefront): fold the counts, account the iteration,
+// stop at zero activity, choose the next schedule.
+__global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
 {
     if (blockIdx.x != 0 || threadIdx.x >= WG_WAVE || c->done)
         return;
-    fold_counts(c);
+    const unsigned long long act0 = c->activity, nact0 = c->activated, fresh0 = c->fresh;
+    const unsigned int iters = c->iters + 1u, fiters = c->fiters + (c->mode == WG_MODE_FRONTIER ? 1u : 0u);
+    const unsigned int max_iters = c->max_iters;
+    const unsigned long long unv0 = c->unvisited;
+    const loop_cfg g = read_cfg(c);
+    unsigned long long tot[WG_CNT_N];
+    fold_counts(c, tot);
+    const unsigned long long act = act0 + tot[WG_CNT_ACTIVITY], nact = nact0 + tot[WG_CNT_ACTIVATED];
+    const unsigned long long fresh = fresh0 + tot[WG_CNT_FRESH];
+    const unsigned long long unvisited = unv0 >= fresh ? unv0 - fresh : 0ull;
+    const bool stop = act == 0ull || (max_iters && iters >= max_iters);
+    int mode = 0, blog2 = g.plain_blog2;
+    if (!stop)
+        loop_choose_wave(g, mlp, act, nact, iters, unvisited, &mode, &blog2);
     if (threadIdx.x != 0)
         return;
-    const unsigned long long act = c->activity, nact = c->activated;
-    c->iters += 1u;
-    if (c->mode == WG_MODE_FRONTIER)
-        c->fiters += 1u;
+    c->iters = iters;
+    c->fiters = fiters;
     c->activity = 0ull;
     c->activated = 0ull;
+    c->fresh = 0ull;
     c->heavy_len = 0u;
     c->frontier_verts = 0ull;
     c->last_activity = act;
     c->last_activated = nact;
-    c->unvisited = c->unvisited >= nact ? c->unvisited - nact : 0ull;
-    if (act == 0ull || (c->max_iters && c->iters >= c->max_iters)) {
+    c->unvisited = unvisited;
+    if (stop) {
         c->done = 1u;
         return;
     }
-    loop_choose(c, act);
+    c->mode = (unsigned int)mode;
+    c->blog2 = blog2;
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -1060,7 +1148,7 @@ static int finish_pull(wg_ctx* ctx, uint64_t* edges_activated)
     return WG_OK;
 }
 
-extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices)
+extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* vertices, uint64_t* newly_reached)
 {
     if (!ctx)
         return WG_E_INVAL;
@@ -1068,6 +1156,8 @@ extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* verti
         *edges = ctx->hctr->activity;
     if (vertices)
         *vertices = ctx->hctr->activated;
+    if (newly_reached)
+        *newly_reached = ctx->hctr->fresh;
     return WG_OK;
 }
 
@@ -1308,7 +1398,7 @@ static int enqueue_iteration(wg_ctx* ctx)
                            (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
                            ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
     }
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->mlp_set ? ctx->mlp : nullptr);
     hipLaunchKernelGGL(k_wedge_build, dim3(wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
                        ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->ctr,
                        WG_GATE_FRONTIER);
@@ -1346,9 +1436,11 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
         act0 = od;
     }
-    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, ctx->stream, ctx->ctr, (unsigned int)p->schedule, threshold, blog2,
-                       p->max_iterations, act0, ctx->app == WG_APP_CC ? 1 : 0,
-                       ctx->app == WG_APP_BFS ? (unsigned long long)(ctx->V - 1) : 0ull);
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->mlp_set ? ctx->mlp : nullptr,
+                       (unsigned int)p->schedule, threshold, blog2, p->max_iterations, act0,
+                       ctx->app == WG_APP_CC ? (unsigned long long)ctx->V : 1ull, ctx->app == WG_APP_CC ? 1 : 0,
+                       ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - 1), (unsigned int)ctx->app,
+                       (unsigned long long)ctx->V, (unsigned long long)ctx->E);
     if (ctx->app != WG_APP_CC) // the root's blocks, if the first pull is frontier-filtered
         hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, ctx->root_dev, ctx->csri_off,
                            ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);

@@ -21,6 +21,26 @@
 #include "cmdline.h"
 #include "wedge_host.h"
 
+#include <unistd.h>
+
+/* The schedule model shipped with the package: <binary dir>/../data/schedule_mlp.txt. */
+static const char* default_model_path(char* buf, size_t n)
+{
+    const ssize_t len = readlink("/proc/self/exe", buf, n - 1);
+    if (len <= 0)
+        return NULL;
+    buf[len] = 0;
+    char* slash = strrchr(buf, '/');
+    if (!slash)
+        return NULL;
+    *slash = 0;
+    const char* rel = "/../data/schedule_mlp.txt";
+    if (strlen(buf) + strlen(rel) + 1 > n)
+        return NULL;
+    strcat(buf, rel);
+    return buf;
+}
+
 static double now_ms(void)
 {
     struct timespec t;
@@ -162,11 +182,15 @@ int main(int argc, char* argv[])
                (unsigned long long)o.root, CMDLINE_DEFAULT_ROOT_VERTEX);
         o.root = CMDLINE_DEFAULT_ROOT_VERTEX;
     }
-    if (cs->model[0]) {
-        rc = wg_mlp_load(cs->model);
+    if (cs->model[0] || cs->schedule == WG_SCHED_LEARNED) {
+        char path[4096];
+        const char* model = cs->model[0] ? cs->model : default_model_path(path, sizeof(path));
+        rc = model ? wg_mlp_load(model) : WG_E_IO;
         if (rc)
-            printf("Schedule:    WARNING: cannot load model \"%s\" (%s); using the plain heuristic\n", cs->model,
-                   wg_error_string(rc));
+            printf("Schedule:    WARNING: cannot load model \"%s\" (%s); using the plain heuristic\n",
+                   model ? model : "", wg_error_string(rc));
+        else
+            printf("Schedule:    model \"%s\"\n", model);
     }
 
     const uint64_t V = g.num_vertices;

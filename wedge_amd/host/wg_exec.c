@@ -21,6 +21,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "wg_mlp.h"
 #include "wg_sched.h"
 
 static double now_ms(void)
@@ -156,8 +157,11 @@ static int run_device_loop(wg_ctx* ctx, const wg_run_opts* o, wg_run_stats* st)
     wg_run_stats s;
     memset(&s, 0, sizeof(s));
     uint64_t it = 0, fit = 0;
+    int rc = wg_set_schedule_model(ctx, o->schedule == WG_SCHED_LEARNED ? wg_mlp_params() : NULL, WG_MLP_PARAMS);
+    if (rc)
+        return rc;
     const double t0 = now_ms();
-    int rc = wg_converge(ctx, &p, &it, &fit);
+    rc = wg_converge(ctx, &p, &it, &fit);
     if (rc)
         return rc;
     s.running_ms = now_ms() - t0;
@@ -175,6 +179,24 @@ static int run_device_loop(wg_ctx* ctx, const wg_run_opts* o, wg_run_stats* st)
     return WG_OK;
 }
 
+/* Device time of the kernels in ids (terminated by -1) since the last profile reset. */
+static const int pull_kernels[] = { WG_K_MIN_TILE, WG_K_MIN_BLOCKS, WG_K_BFS_DENSE, WG_K_BFS_HEAVY, WG_K_BFS_BLOCKS, -1 };
+static const int build_kernels[] = { WG_K_WEDGE_BUILD, -1 };
+
+static int gpu_ms(wg_ctx* ctx, const int* ids, double* ms)
+{
+    *ms = 0.0;
+    for (; *ids >= 0; ++ids) {
+        uint64_t n = 0;
+        double t = 0.0;
+        int rc = wg_profile_get(ctx, *ids, &n, &t);
+        if (rc)
+            return rc;
+        *ms += t;
+    }
+    return WG_OK;
+}
+
 static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pull, wg_run_stats* st)
 {
     if (!o->record_trace && (o->schedule != WG_SCHED_LEARNED || wg_mlp_loaded()))
@@ -185,8 +207,12 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
     const int plain_blog2 = block_log2_of(o);
     wg_run_stats s;
     memset(&s, 0, sizeof(s));
-    if (o->record_trace)
+    if (o->record_trace) {
         trace_reset();
+        int rc = wg_profile_enable(ctx, 1);
+        if (rc)
+            return rc;
+    }
 
     uint64_t active_edges = E;          /* CC: source/execution_cc.c:95 */
     uint64_t frontier_vertices = V;
@@ -198,7 +224,7 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
         active_edges = od_root;
         frontier_vertices = 1;
     }
-    uint64_t unvisited = (app == WG_APP_BFS) ? V - 1 : 0;
+    uint64_t unvisited = (app == WG_APP_CC) ? 0 : V - 1; /* not reached yet (BFS, SSSP) */
 
     wg_sched_features f;
     memset(&f, 0, sizeof(f));
@@ -224,6 +250,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
     }
     uint32_t it = 0;
     for (;;) {
+        if (o->record_trace && (rc = wg_profile_reset(ctx)))
+            return rc;
         const double t0 = now_ms();
         uint64_t act = 0;
         rc = pull(ctx, mode, &act);
@@ -248,6 +276,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
         row.frontier_edges = active_edges;
         row.unvisited = unvisited;
         row.pull_ms = dt_pull;
+        if (o->record_trace && (rc = gpu_ms(ctx, pull_kernels, &row.gpu_pull_ms)))
+            return rc;
         if (it > 2 * V + 64)
             return WG_E_STATE; /* cannot happen for a correct pull: each iteration activates a vertex */
         if (act == 0 || (o->max_iterations && it >= o->max_iterations)) {
@@ -256,11 +286,10 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
             break;
         }
         active_edges = act;
-        uint64_t nact = 0;
-        wg_last_pull_counts(ctx, NULL, &nact);
+        uint64_t nact = 0, fresh = 0;
+        wg_last_pull_counts(ctx, NULL, &nact, &fresh);
         frontier_vertices = nact;
-        if (app == WG_APP_BFS)
-            unvisited = (unvisited >= nact) ? unvisited - nact : 0;
+        unvisited = (unvisited >= fresh) ? unvisited - fresh : 0;
         /* next iteration's schedule */
         int next_mode, next_blog2;
         f.iteration = it;
@@ -282,6 +311,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
         const double t3 = now_ms();
         s.wedge_ms += t3 - t2;
         row.wedge_ms = t3 - t2;
+        if (o->record_trace && (rc = gpu_ms(ctx, build_kernels, &row.gpu_build_ms)))
+            return rc;
         if (o->record_trace)
             trace_push(&row);
         mode = next_mode;
@@ -289,6 +320,8 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
     }
     s.running_ms = now_ms() - t_start;
     s.pull_ms = s.dense_pull_ms + s.frontier_pull_ms;
+    if (o->record_trace && (rc = wg_profile_enable(ctx, 0)))
+        return rc;
     uint64_t rv = 0, re = 0;
     rc = wg_reached(ctx, &rv, &re);
     if (rc)

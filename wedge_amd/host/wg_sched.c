@@ -17,7 +17,7 @@ void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_l
         *mode = WG_MODE_FRONTIER;
         return;
     case WG_SCHED_LEARNED:
-        if (wg_mlp_decide(f, mode, block_log2) == WG_OK)
+        if (wg_mlp_decide(f, f->frontier_edges > threshold, mode, block_log2) == WG_OK)
             return;
         /* no model loaded: the plain heuristic is the fallback */
         /* fall through */
