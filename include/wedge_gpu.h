@@ -49,6 +49,9 @@ typedef struct wg_ctx wg_ctx;
 #define WG_APP_CC   2
 #define WG_APP_PR   3
 #define WG_APP_SSSP 4
+#define WG_APP_MSBFS 5   /* batched sources (wg_msbfs) */
+#define WG_APP_MSSSSP 6  /* batched sources (wg_mssssp) */
+#define WG_APP_PPR 7     /* batched personalised PageRank (wg_ppr) */
 
 #define WG_MODE_DENSE    0  /* reference: perform_edge_pull_phase_X */
 #define WG_MODE_FRONTIER 1  /* reference: perform_edge_pull_phase_X_with_wedge_frontier */
@@ -199,6 +202,25 @@ typedef struct wg_loop_params {
     uint32_t max_iterations; /* 0 = until convergence */
 } wg_loop_params;
 int wg_converge(wg_ctx* ctx, const wg_loop_params* params, uint64_t* iterations, uint64_t* frontier_iterations);
+
+/* ---- batched sources -------------------------------------------------------
+ * No reference counterpart (the reference's BFS and SSSP take one root,
+ * source/execution_bfs.c:84-118, source/execution_sssp.c:70-89): up to
+ * WG_MS_MAX_SOURCES sources traverse the graph together, lane s of a
+ * per-vertex 64-bit mask (BFS) or of a per-vertex distance row (SSSP)
+ * belonging to source s.  Each source's result equals that of a single-source
+ * run from it.  params: schedule 0 = plain rule per level (on the summed
+ * out-degree of the vertices any source newly reached), 2 = dense, 3 =
+ * frontier (1 behaves as 0); threshold_pct, block_log2, max_iterations as in
+ * wg_converge.  Roots may repeat. */
+#define WG_MS_MAX_SOURCES 64
+/* levels: BFS levels run; edges_reached: sum over sources of the out-degrees
+ * of the vertices each reached (both optional).  WG_E_OVERFLOW if a depth
+ * would exceed 65534. */
+int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_params* params, uint64_t* levels,
+             uint64_t* edges_reached);
+/* Depths (V entries) from root number `source`, UINT32_MAX if unreached. */
+int wg_read_msbfs(wg_ctx* ctx, uint32_t source, uint32_t* host_depths);
 
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
  * the root holds (0 << 32) | root. */

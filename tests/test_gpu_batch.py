@@ -1,0 +1,80 @@
+"""Batched sources on the MI355X: every source of a batched traversal must
+equal an independent single-source run of the oracle (bit-exact depths)."""
+import numpy as np
+import pytest
+
+import oracle
+import wedge_amd as W
+
+pytestmark = pytest.mark.gpu
+
+MS_SCHEDULES = [dict(schedule="plain"), dict(schedule="dense"), dict(schedule="frontier", block_log2=3),
+                dict(schedule="frontier", block_log2=6)]
+
+
+@pytest.fixture(scope="module", params=[True, False], ids=["relabel", "native-order"])
+def ctx(request):
+    c = W.Context(0, relabel=request.param)
+    yield c
+    c.close()
+
+
+def _graphs():
+    return [("rmat", W.rmat(12, 16, seed=2)), ("rmat_sym", W.rmat(11, 16, seed=3, symmetrize=True)),
+            ("grid", W.grid(40, 70, seed=4))]
+
+
+def _roots(g, n, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.integers(0, g.num_vertices, n).astype(np.uint64)
+    r[0] = g.max_outdegree_vertex()
+    if n > 3:
+        r[3] = r[1]  # a repeated root
+    return r
+
+
+@pytest.mark.parametrize("nroots", [1, 5, 64])
+def test_msbfs_matches_single_source_oracle(ctx, nroots):
+    for name, g in _graphs():
+        ctx.load(g)
+        roots = _roots(g, nroots, seed=nroots)
+        orc = oracle.Oracle.of(g, 4)
+        want = [orc.bfs(int(r))[1] for r in roots]
+        od = g.outdegrees().astype(np.uint64)
+        for opts in MS_SCHEDULES:
+            levels, reached = ctx.msbfs(roots, **opts)
+            for i in range(nroots):
+                assert np.array_equal(ctx.read_msbfs(i), want[i]), f"{name} {opts} root #{i}"
+            assert levels == max(int(d[d != np.uint32(2**32 - 1)].max()) for d in want) + 1
+            assert reached == sum(int(od[d != np.uint32(2**32 - 1)].sum()) for d in want)
+
+
+def test_msbfs_heavy_destinations(ctx):
+    rng = np.random.default_rng(7)
+    V = 6000
+    src = np.concatenate([rng.integers(1, V, 5000), rng.integers(0, V, 20000)])
+    dst = np.concatenate([np.zeros(5000, np.int64), rng.integers(0, V, 20000)])
+    g = W.from_edges(V, src, dst, dedup=True)
+    ctx.load(g)
+    roots = _roots(g, 64, seed=9)
+    orc = oracle.Oracle.of(g, 4)
+    ctx.msbfs(roots)
+    for i in (0, 17, 63):
+        assert np.array_equal(ctx.read_msbfs(i), orc.bfs(int(roots[i]))[1])
+
+
+def test_msbfs_errors(ctx):
+    g = W.rmat(8, 4, seed=1)
+    ctx.load(g)
+    with pytest.raises(W.WedgeError):
+        ctx.msbfs(np.zeros(0, np.uint64))
+    with pytest.raises(W.WedgeError):
+        ctx.msbfs(np.zeros(65, np.uint64))
+    with pytest.raises(W.WedgeError):
+        ctx.msbfs([g.num_vertices])  # root out of range
+    ctx.msbfs([0, 1])
+    with pytest.raises(W.WedgeError):
+        ctx.read_msbfs(2)  # only two roots
+    ctx.init_bfs(0)
+    with pytest.raises(W.WedgeError):
+        ctx.read_msbfs(0)  # the batched state was replaced

@@ -38,6 +38,8 @@ WG_E_NODEV = -9
 
 WEIGHT_NONE, WEIGHT_F64, WEIGHT_U32 = 0, 1, 2
 APP_BFS, APP_CC, APP_PR, APP_SSSP = 1, 2, 3, 4
+APP_MSBFS, APP_MSSSSP, APP_PPR = 5, 6, 7
+MS_MAX_SOURCES = 64
 MODE_DENSE, MODE_FRONTIER = 0, 1
 SCHED_PLAIN, SCHED_LEARNED, SCHED_DENSE, SCHED_FRONTIER = 0, 1, 2, 3
 DEFAULT_BLOCK_LOG2 = 4
@@ -141,6 +143,8 @@ GPU_SYMBOLS = [
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
+    ("wg_msbfs", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p, _u64p]),
+    ("wg_read_msbfs", C.c_int, [_p, C.c_uint32, _u32p]),
     ("wg_mlp_action", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("wg_set_schedule_model", C.c_int, [_p, _f32p, C.c_uint64]),
     ("wg_mlp_infer", C.c_int, [_p, C.c_int, _f32p, _f32p, _i32p, C.c_uint64, _f32p, _i32p]),
@@ -378,6 +382,9 @@ def host_threads() -> int:
 # ---- device context ---------------------------------------------------------------
 
 
+SCHEDULES = {"plain": 0, "learned": 1, "dense": 2, "frontier": 3}
+
+
 def run_opts(**kw) -> RunOpts:
     o = RunOpts()
     _lib_host().wg_run_opts_default(C.byref(o))
@@ -578,6 +585,25 @@ class Context:
         st = RunStats()
         self._chk(_lib_host().wg_run_cc(self._h, C.byref(o), _ptr(out, C.c_uint32), C.byref(st)), "wg_run_cc")
         return out, st.as_dict()
+
+    def msbfs(self, roots, schedule="plain", threshold_pct=5, block_log2=0, max_iterations=0):
+        """BFS from up to 64 roots in one traversal; returns (levels, edges reached over all roots).
+        Depths per root: read_msbfs(i).  The default threshold is lower than the single-source
+        rule's 20 %: a batched dense level gathers a 64-bit mask per edge, so the frontier pull
+        stays cheaper up to a smaller active share."""
+        r = np.ascontiguousarray(roots, np.uint64)
+        p = LoopParams(schedule=SCHEDULES[schedule] if isinstance(schedule, str) else schedule,
+                       threshold_pct=threshold_pct, block_log2=block_log2,
+                       max_iterations=max_iterations)
+        lv, er = C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.wg_msbfs(self._h, _ptr(r, C.c_uint64), r.size, C.byref(p), C.byref(lv), C.byref(er)),
+                  "wg_msbfs")
+        return int(lv.value), int(er.value)
+
+    def read_msbfs(self, source: int) -> np.ndarray:
+        out = np.empty(self.V, np.uint32)
+        self._chk(self._lib.wg_read_msbfs(self._h, source, _ptr(out, C.c_uint32)), "wg_read_msbfs")
+        return out
 
     def set_schedule_model(self, params: np.ndarray | None):
         """Model for the device loop's learned schedule (None clears it)."""

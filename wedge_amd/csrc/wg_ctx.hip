@@ -398,6 +398,10 @@ int wg_free_app_state(wg_ctx* ctx)
 {
     free_dev(&ctx->prop[0]);
     free_dev(&ctx->prop[1]);
+    free_dev(&ctx->ms[0]);
+    free_dev(&ctx->ms[1]);
+    free_dev(&ctx->ms[2]);
+    ctx->ms_n = 0;
     free_dev((void**)&ctx->acc);
     ctx->app = WG_APP_NONE;
     return WG_OK;
@@ -753,12 +757,13 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
     ctx->bm_words = (V + 63) / 64;
     ctx->wedge_words = (((E + (1ull << WG_MIN_BLOCK_LOG2) - 1) >> WG_MIN_BLOCK_LOG2) + 31) / 32 + 1;
     ctx->pr_grid = wg_grid(ctx, ctx->ntiles, WG_WPB, 8);
+    ctx->heavy_cap = (uint32_t)(E / 32 + E / 4096 + 16); // bound on deferred out-edge chunks per build
     ctx->span_grid = wg_grid(ctx, ctx->nheavy, 4, 4);
     if ((rc = upload<uint64_t>(ctx, &ctx->bm_cur, nullptr, ctx->bm_words)) ||
         (rc = upload<uint64_t>(ctx, &ctx->bm_next, nullptr, ctx->bm_words)) ||
         (rc = upload<uint64_t>(ctx, &ctx->bm_visited, nullptr, ctx->bm_words)) ||
         (rc = upload<uint32_t>(ctx, &ctx->wedge, nullptr, ctx->wedge_words)) ||
-        (rc = upload<uint32_t>(ctx, &ctx->heavy_list, nullptr, V)) ||
+        (rc = upload<uint32_t>(ctx, &ctx->heavy_list, nullptr, 2 * (uint64_t)ctx->heavy_cap)) ||
         (rc = upload<double>(ctx, &ctx->tile_part, nullptr, ctx->ntiles + 1)) ||
         (rc = upload<double>(ctx, &ctx->blk_part, nullptr, (uint64_t)ctx->num_cu * 16 + 16)))
         goto fail;

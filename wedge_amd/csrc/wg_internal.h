@@ -147,6 +147,8 @@ struct wg_ctx {
     // application state
     int app;
     void* prop[2];            // PR contributions (double buffer) / BFS packed / CC labels / SSSP dist
+    void* ms[3];              // batched sources: per-vertex masks or rows (freed with the app state)
+    uint32_t ms_n;            // batched sources: number of roots / seeds
     int pr_cur;               // PR: index of the current contribution buffer
     int pr_parity;            // PR: index of S[] holding the current iteration's sum
     int pr_fp32;
@@ -163,7 +165,8 @@ struct wg_ctx {
     uint32_t* wedge;          // Wedge frontier, one bit per edge block
     uint64_t wedge_words;     // 32-bit words allocated (finest granularity)
     int wedge_log2;           // granularity of the marked blocks, -1 if none are marked
-    uint32_t* heavy_list;     // frontier build: deferred high out-degree vertices
+    uint32_t* heavy_list;     // frontier build: deferred out-edge chunks (vertex, first position)
+    uint32_t heavy_cap;       // chunks heavy_list holds
     float* mlp;               // schedule model parameters (WG_MLP_PARAMS floats)
     int mlp_set;              // a model is set for the learned schedule
     wg_dev_counters* ctr;     // device
@@ -255,6 +258,12 @@ __device__ __forceinline__ void wg_wave_sync()
 }
 
 // internal entry points shared across translation units
+int wg_reset_counters(wg_ctx* ctx, unsigned int flags); // RST_* flags (wg_frontier.h)
+int wg_read_counters(wg_ctx* ctx);                      // fold the slots, copy to ctx->hctr, wait
+int wg_launch_wedge_build(wg_ctx* ctx, const uint64_t* active, int block_log2);
+uint32_t wg_frontier_grid(const wg_ctx* ctx);
+uint32_t wg_wedge_build_grid(const wg_ctx* ctx);
+uint32_t wg_advance_grid(const wg_ctx* ctx);
 int wg_pr_reset(wg_ctx* ctx);
 int wg_free_app_state(wg_ctx* ctx);
 int wg_alloc_bitmasks(wg_ctx* ctx);

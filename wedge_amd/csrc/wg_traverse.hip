@@ -25,115 +25,10 @@
 // non-zero ones and spreads their marked blocks over the lanes of a wavefront,
 // so the edges it reads follow the marked blocks; the word scan itself is
 // E / 2^b / 8 bytes.
+#include "wg_frontier.h"
 #include "wg_mlp_wave.h"
 
-__device__ __forceinline__ bool test_bit64(const uint64_t* __restrict__ bm, uint32_t i)
-{
-    return (bm[i >> 6] >> (i & 63)) & 1ull;
-}
-
-// Per-thread activation tally: activated vertices and their out-edges.
-struct act_tally {
-    unsigned long long edges;
-    unsigned long long verts;
-    unsigned long long fresh; // activated vertices reached for the first time
-};
-
-// Marks vertex v activated (the reference's cond_set_bit + out-degree count,
-// source/phase_edge_pull_cc.c:119-123); counts it if this call set the bit.
-__device__ __forceinline__ void activate(uint64_t* __restrict__ next, const uint32_t* __restrict__ od, uint32_t v,
-                                         act_tally& t)
-{
-    const unsigned long long m = 1ull << (v & 63);
-    const unsigned long long old = atomicOr((unsigned long long*)&next[v >> 6], m);
-    if (!(old & m)) {
-        t.edges += od[v];
-        t.verts += 1ull;
-    }
-}
-
-// Lanes hold the consecutive vertices base + lane; the changed ones are set in
-// 'next' with at most two 64-bit atomics per wavefront (the range straddles at
-// most two bitmask words) and counted when newly set.
-__device__ __forceinline__ void activate_contiguous(uint64_t* __restrict__ next, const uint32_t* __restrict__ od,
-                                                    uint32_t base, bool changed, act_tally& t)
-{
-    const unsigned long long cm = wg_ballot(changed);
-    if (cm == 0ull)
-        return;
-    const int lane = wg_lane();
-    const uint32_t w0 = base >> 6, sh = base & 63u;
-    const unsigned long long lo = cm << sh;                  // bits landing in word w0
-    const unsigned long long hi = sh ? (cm >> (64u - sh)) : 0ull; // bits landing in word w0 + 1
-    unsigned long long old_lo = 0ull, old_hi = 0ull;
-    if (lane == 0 && lo)
-        old_lo = atomicOr((unsigned long long*)&next[w0], lo);
-    if (lane == 1 && hi)
-        old_hi = atomicOr((unsigned long long*)&next[w0 + 1], hi);
-    old_lo = __shfl(old_lo, 0);
-    old_hi = __shfl(old_hi, 1);
-    if (changed) {
-        const uint32_t pos = sh + (uint32_t)lane;
-        const bool was = (pos < 64u) ? ((old_lo >> pos) & 1ull) : ((old_hi >> (pos - 64u)) & 1ull);
-        if (!was) {
-            t.edges += od[base + (uint32_t)lane];
-            t.verts += 1ull;
-        }
-    }
-}
-
-// Adds a per-lane count to field f of this workgroup's counter slot.
-__device__ __forceinline__ void add_count(unsigned long long a, wg_dev_counters* __restrict__ c, int f)
-{
-    a = wg_wave_sum(a);
-    if (wg_lane() == 0 && a)
-        atomicAdd(&c->slots[blockIdx.x & (WG_CSLOTS - 1)].v[f], a);
-}
-
-__device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* __restrict__ c)
-{
-    add_count(t.edges, c, WG_CNT_ACTIVITY);
-    add_count(t.verts, c, WG_CNT_ACTIVATED);
-    add_count(t.fresh, c, WG_CNT_FRESH);
-}
-
-__device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, int f)
-{
-    switch (f) {
-    case WG_CNT_ACTIVITY:
-        return &c->activity;
-    case WG_CNT_ACTIVATED:
-        return &c->activated;
-    case WG_CNT_FRESH:
-        return &c->fresh;
-    case WG_CNT_FRONTIER:
-        return &c->frontier_verts;
-    case WG_CNT_REACHED_V:
-        return &c->reached_verts;
-    default:
-        return &c->reached_edges;
-    }
-}
-
-// Adds the slots into the scalar counts and zeroes them; every lane gets the
-// slot totals in tot (if given).  Called by the first wavefront of a launch
-// only; nothing else may be adding to the slots.
-__device__ void fold_counts(wg_dev_counters* c, unsigned long long* tot = nullptr)
-{
-    const int lane = wg_lane();
-    for (int f = 0; f < WG_CNT_N; ++f) {
-        unsigned long long x = 0ull;
-        if (lane < WG_CSLOTS) {
-            x = c->slots[lane].v[f];
-            c->slots[lane].v[f] = 0ull;
-        }
-        x = wg_wave_sum(x);
-        if (tot)
-            tot[f] = x;
-        if (lane == 0)
-            *count_field(c, f) += x;
-    }
-}
+// ---- counters -------------------------------------------------------------------
 
 __global__ void k_fold(wg_dev_counters* c)
 {
@@ -141,12 +36,6 @@ __global__ void k_fold(wg_dev_counters* c)
         fold_counts(c);
 }
 
-// ---- counters -------------------------------------------------------------------
-
-#define RST_ACTIVITY 1u
-#define RST_FRONTIER 4u
-#define RST_OVERFLOW 8u
-#define RST_HEAVY 16u
 
 __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
 {
@@ -175,7 +64,7 @@ __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
     }
 }
 
-static int reset_counters(wg_ctx* ctx, unsigned int flags)
+int wg_reset_counters(wg_ctx* ctx, unsigned int flags)
 {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr, flags);
     return wg_check_launch(ctx);
@@ -319,76 +208,6 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
     add_tally(act, ctr);
 }
 
-// Lane -> (marked block, edge) mapping shared by the frontier pulls.
-struct block_walk {
-    uint32_t per_wave;   // blocks per wave step
-    uint32_t lane_bits;  // log2 of lanes per block
-    uint32_t steps;      // 64-edge steps per block
-};
-
-__device__ __forceinline__ block_walk make_walk(int blog2)
-{
-    block_walk b;
-    b.lane_bits = blog2 < 6 ? (uint32_t)blog2 : 6u;
-    b.per_wave = 64u >> b.lane_bits;
-    b.steps = blog2 > 6 ? (1u << (blog2 - 6)) : 1u;
-    return b;
-}
-
-// Words of the Wedge frontier at granularity 2^blog2 for E edges.
-__device__ __forceinline__ uint32_t wedge_nwords(uint32_t E, int blog2)
-{
-    const uint64_t nblk = ((uint64_t)E + (1ull << blog2) - 1ull) >> blog2;
-    return (uint32_t)((nblk + 31ull) >> 5);
-}
-
-// Walks the marked edge blocks (source/phase_edge_pull_bfs.c:245-268): each
-// wavefront takes 64 consecutive frontier words, one per lane, zeroes the
-// non-zero ones, and for each of them hands the marked blocks to lane groups
-// of 2^min(b,6) lanes, 64 >> min(b,6) blocks per step.  f(p, ok, first) is
-// called by every lane of the wavefront for each 64-edge step: p is the edge
-// position, ok whether the lane holds an edge, first whether the lane is the
-// first of its block's group.
-template <class F>
-__device__ __forceinline__ void walk_marked_blocks(uint32_t* __restrict__ wedge, int blog2, uint32_t E, F&& f)
-{
-    const block_walk bw = make_walk(blog2);
-    const uint32_t nwords = wedge_nwords(E, blog2);
-    const int lane = wg_lane();
-    const uint32_t sub = (uint32_t)lane & ((1u << bw.lane_bits) - 1u);
-    const uint32_t slot = (uint32_t)lane >> bw.lane_bits;
-    const uint32_t nwaves = gridDim.x * WG_WPB;
-    for (uint32_t w0 = (blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE) * WG_WAVE; w0 < nwords; w0 += nwaves * WG_WAVE) {
-        const uint32_t wi = w0 + (uint32_t)lane;
-        uint32_t word = 0u;
-        if (wi < nwords) {
-            word = wedge[wi];
-            if (word)
-                wedge[wi] = 0u; // consumed, source/phase_edge_pull_bfs.c:252-256
-        }
-        unsigned long long nz = wg_ballot(word != 0u);
-        while (nz) {
-            const int l = __ffsll((long long)nz) - 1;
-            nz &= nz - 1ull;
-            uint32_t bits = (uint32_t)__shfl((int)word, l); // wave-uniform
-            const uint64_t base = (uint64_t)(w0 + (uint32_t)l) * 32u;
-            while (bits) {
-                uint32_t mine = bits; // this lane group's block: set bit number 'slot'
-                for (uint32_t i = 0; i < slot && mine; ++i)
-                    mine &= mine - 1u;
-                const bool have = mine != 0u;
-                const uint64_t blk = base + (have ? (uint64_t)(__ffs((int)mine) - 1) : 0ull);
-                for (uint32_t st = 0; st < bw.steps; ++st) {
-                    const uint64_t p64 = (blk << blog2) + (uint64_t)st * 64u + sub;
-                    f((uint32_t)p64, have && p64 < E, sub == 0u);
-                }
-                for (uint32_t i = 0; i < bw.per_wave && bits; ++i)
-                    bits &= bits - 1u;
-            }
-        }
-    }
-}
-
 // Frontier-mode BFS level over the marked edge blocks.
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
@@ -517,19 +336,6 @@ struct PolSU { // 32-bit integer distances
         return s;
     }
 };
-
-template <typename T>
-__device__ __forceinline__ T atomic_min_t(T* p, T v);
-template <>
-__device__ __forceinline__ uint32_t atomic_min_t<uint32_t>(uint32_t* p, uint32_t v)
-{
-    return atomicMin(p, v);
-}
-template <>
-__device__ __forceinline__ unsigned long long atomic_min_t<unsigned long long>(unsigned long long* p, unsigned long long v)
-{
-    return atomicMin(p, v);
-}
 
 template <class P>
 __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint64_t* __restrict__ next,
@@ -681,28 +487,20 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __rest
 
 // ---- Wedge frontier ---------------------------------------------------------------
 
-// Marks edge block blk (source/wedge.c:22-31).  The result of the atomic is
-// not needed, so the lane does not wait for it.
-__device__ __forceinline__ void mark_block(uint32_t blk, uint32_t* __restrict__ wedge)
-{
-    const uint32_t m = 1u << (blk & 31);
-    uint32_t* wp = &wedge[blk >> 5];
-    if (!(*wp & m))
-        atomicOr(wp, m);
-}
-
-#define WG_WEDGE_LANE_MAX 32u
 
 // source/wedge.c:63-155: every set vertex marks the blocks of its out-edges.
 // Each lane loads one bitmask word; the wavefront then takes the non-zero
 // words one at a time with one lane per vertex of the word, so a dense word
 // costs one step and an empty one nothing.  Vertices with more than
-// WG_WEDGE_LANE_MAX out-edges are deferred to k_wedge_heavy.
+// WG_WEDGE_LANE_MAX out-edges are deferred to k_wedge_heavy as chunks of
+// WG_WEDGE_CHUNK out-edges (pairs vertex, first position), so that a hub's
+// list is spread over many wavefronts.  At most E/32 + E/WG_WEDGE_CHUNK + 1
+// chunks are ever listed; heavy_cap bounds the writes regardless.
 __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __restrict__ active, uint64_t nwords,
                                                            const uint32_t* __restrict__ csri_off,
                                                            const uint32_t* __restrict__ csri_pos, int blog2_arg,
                                                            uint32_t* __restrict__ wedge, uint32_t* __restrict__ heavy,
-                                                           wg_dev_counters* __restrict__ ctr, int gate)
+                                                           uint32_t heavy_cap, wg_dev_counters* __restrict__ ctr, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
@@ -723,7 +521,12 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
             const uint32_t u = (uint32_t)((w0 + (uint64_t)l) * 64u + (uint64_t)lane);
             const uint32_t s0 = csri_off[u], e0 = csri_off[u + 1];
             if (e0 - s0 > WG_WEDGE_LANE_MAX) {
-                heavy[atomicAdd(&ctr->heavy_len, 1u)] = u;
+                const uint32_t nch = (e0 - s0 + WG_WEDGE_CHUNK - 1) / WG_WEDGE_CHUNK;
+                const uint32_t base = atomicAdd(&ctr->heavy_len, nch);
+                for (uint32_t c = 0; c < nch && base + c < heavy_cap; ++c) {
+                    heavy[2 * (base + c)] = u;
+                    heavy[2 * (base + c) + 1] = s0 + c * WG_WEDGE_CHUNK;
+                }
                 continue;
             }
             for (uint32_t p = s0; p < e0; ++p)
@@ -732,8 +535,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
     }
 }
 
-// High out-degree vertices deferred by k_wedge_build: one wave per vertex.
-__global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy,
+// Out-edge chunks deferred by k_wedge_build: one wave per chunk.
+__global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy, uint32_t heavy_cap,
                                                            const uint32_t* __restrict__ csri_off,
                                                            const uint32_t* __restrict__ csri_pos, int blog2_arg,
                                                            uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr,
@@ -742,11 +545,12 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __rest
     if (wg_gated_off(ctr, gate))
         return;
     const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
-    const uint32_t n = ctr->heavy_len;
+    const uint32_t n = ctr->heavy_len < heavy_cap ? ctr->heavy_len : heavy_cap;
     const uint32_t nwaves = gridDim.x * WG_WPB;
     for (uint32_t i = blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
-        const uint32_t u = heavy[i];
-        const uint32_t s = csri_off[u], e = csri_off[u + 1];
+        const uint32_t u = heavy[2 * i], s = heavy[2 * i + 1];
+        const uint32_t end = csri_off[u + 1];
+        const uint32_t e = (s + WG_WEDGE_CHUNK < end) ? s + WG_WEDGE_CHUNK : end;
         for (uint32_t p = s + (uint32_t)wg_lane(); p < e; p += WG_WAVE)
             mark_block(csri_pos[p] >> blog2, wedge);
     }
@@ -958,7 +762,7 @@ __global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
 // ---- host side ----------------------------------------------------------------------
 
 // Folds the counter slots and copies the counters to the host mirror.
-static int read_counters(wg_ctx* ctx)
+int wg_read_counters(wg_ctx* ctx)
 {
     hipLaunchKernelGGL(k_fold, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr);
     WG_HIP(ctx, hipGetLastError());
@@ -972,7 +776,7 @@ static int clear_wedge_state(wg_ctx* ctx)
     int rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u);
     if (rc)
         return rc;
-    rc = reset_counters(ctx, RST_HEAVY);
+    rc = wg_reset_counters(ctx, RST_HEAVY);
     ctx->wedge_log2 = -1;
     return rc;
 }
@@ -987,7 +791,7 @@ static int init_common(wg_ctx* ctx, int app)
     int rc;
     if ((rc = wg_clear_u64(ctx, ctx->bm_cur, ctx->bm_words)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
         (rc = wg_clear_u64(ctx, ctx->bm_visited, ctx->bm_words)) || (rc = clear_wedge_state(ctx)) ||
-        (rc = reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW)))
+        (rc = wg_reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW)))
         return rc;
     ctx->app = app;
     ctx->iter = 0;
@@ -1062,17 +866,17 @@ extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
 // Launch shapes.  Frontier pulls: one wavefront per 64 frontier words at the
 // finest granularity.  Frontier build: one wavefront per 64 bitmask words.
 // End-of-iteration bitmask kernels: one thread per bitmask word.
-static uint32_t frontier_grid(const wg_ctx* ctx)
+uint32_t wg_frontier_grid(const wg_ctx* ctx)
 {
     return wg_grid(ctx, ctx->wedge_words, WG_WPB * WG_WAVE, 8);
 }
 
-static uint32_t wedge_build_grid(const wg_ctx* ctx)
+uint32_t wg_wedge_build_grid(const wg_ctx* ctx)
 {
     return wg_grid(ctx, ctx->bm_words, WG_WPB * WG_WAVE, 8);
 }
 
-static uint32_t advance_grid(const wg_ctx* ctx)
+uint32_t wg_advance_grid(const wg_ctx* ctx)
 {
     return wg_grid(ctx, ctx->bm_words, 256, 32);
 }
@@ -1112,6 +916,23 @@ extern "C" int wg_wedge_activate_single_vertex(wg_ctx* ctx, uint64_t vertex, int
     return rc;
 }
 
+// Marks the blocks of every out-edge of the vertices set in `active` (V bits)
+// at granularity 2^block_log2 (no granularity bookkeeping: callers track it).
+int wg_launch_wedge_build(wg_ctx* ctx, const uint64_t* active, int block_log2)
+{
+    int rc = wg_reset_counters(ctx, RST_HEAVY);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, active,
+                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->heavy_list, ctx->heavy_cap, ctx->ctr,
+                       WG_GATE_ALWAYS);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
+                       ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->ctr, WG_GATE_ALWAYS);
+    return wg_check_launch(ctx);
+}
+
 extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
 {
     if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
@@ -1119,16 +940,16 @@ extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
     if (!check_block_log2(block_log2))
         return WG_E_INVAL;
     int rc = prepare_frontier_build(ctx, block_log2);
-    if (rc || (rc = reset_counters(ctx, RST_HEAVY)))
+    if (rc || (rc = wg_reset_counters(ctx, RST_HEAVY)))
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_WEDGE_BUILD);
-    hipLaunchKernelGGL(k_wedge_build, dim3(wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
-                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->heavy_list, ctx->ctr,
+    hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
+                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->heavy_list, ctx->heavy_cap, ctx->ctr,
                        WG_GATE_ALWAYS);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->ctr, WG_GATE_ALWAYS);
+                       ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, block_log2, ctx->wedge, ctx->ctr, WG_GATE_ALWAYS);
     rc = wg_check_launch(ctx);
     wg_prof_end(ctx, WG_K_WEDGE_BUILD, slot);
     return rc;
@@ -1139,7 +960,7 @@ static int finish_pull(wg_ctx* ctx, uint64_t* edges_activated)
     ctx->iter += 1;
     if (!edges_activated)
         return WG_OK;
-    int rc = read_counters(ctx);
+    int rc = wg_read_counters(ctx);
     if (rc)
         return rc;
     *edges_activated = ctx->hctr->activity;
@@ -1164,7 +985,7 @@ extern "C" int wg_last_pull_counts(wg_ctx* ctx, uint64_t* edges, uint64_t* verti
 // A dense pull covers every edge: any marked blocks are dropped.
 static int begin_pull(wg_ctx* ctx, int mode)
 {
-    int rc = reset_counters(ctx, RST_ACTIVITY);
+    int rc = wg_reset_counters(ctx, RST_ACTIVITY);
     if (rc)
         return rc;
     if (mode == WG_MODE_DENSE && ctx->wedge_log2 >= 0)
@@ -1176,7 +997,7 @@ static int end_frontier_pull(wg_ctx* ctx)
 {
     // the pull zeroed every frontier word it consumed
     ctx->wedge_log2 = -1;
-    return reset_counters(ctx, RST_HEAVY);
+    return wg_reset_counters(ctx, RST_HEAVY);
 }
 
 extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated)
@@ -1211,7 +1032,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         }
     } else {
         int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
-        hipLaunchKernelGGL(k_bfs_blocks, dim3(frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
+        hipLaunchKernelGGL(k_bfs_blocks, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
                            ctx->bm_next, packed, ctx->wedge, ctx->perm, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -1237,7 +1058,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
     } else {
         int slot = wg_prof_begin(ctx, WG_K_MIN_BLOCKS);
-        hipLaunchKernelGGL(k_min_blocks<P>, dim3(frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
+        hipLaunchKernelGGL(k_min_blocks<P>, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (typename P::T*)prop, ctx->od,
                            ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -1284,11 +1105,11 @@ extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
 {
     if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
         return WG_E_STATE;
-    int rc = reset_counters(ctx, RST_FRONTIER);
+    int rc = wg_reset_counters(ctx, RST_FRONTIER);
     if (rc)
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_ADVANCE);
-    const uint32_t grid = advance_grid(ctx);
+    const uint32_t grid = wg_advance_grid(ctx);
     if (ctx->app == WG_APP_BFS)
         hipLaunchKernelGGL(k_bfs_advance, dim3(grid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
                            ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_ALWAYS);
@@ -1299,7 +1120,7 @@ extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
         return rc;
     wg_prof_end(ctx, WG_K_ADVANCE, slot);
     if (frontier_vertices) {
-        if ((rc = read_counters(ctx)))
+        if ((rc = wg_read_counters(ctx)))
             return rc;
         *frontier_vertices = ctx->hctr->frontier_verts;
     }
@@ -1349,7 +1170,7 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
         hipLaunchKernelGGL(k_reached_min<uint32_t>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->prop[0],
                            PolSU::INF, ctx->od, ctx->V, ctx->ctr);
     int rc = wg_check_launch(ctx);
-    if (rc || (rc = read_counters(ctx)))
+    if (rc || (rc = wg_read_counters(ctx)))
         return rc;
     if (vertices)
         *vertices = ctx->hctr->reached_verts;
@@ -1363,7 +1184,7 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
 static int enqueue_iteration(wg_ctx* ctx)
 {
     uint64_t* packed = (uint64_t*)ctx->prop[0];
-    const uint32_t qgrid = frontier_grid(ctx);
+    const uint32_t qgrid = wg_frontier_grid(ctx);
     if (ctx->app == WG_APP_BFS) {
         hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                            ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
@@ -1399,12 +1220,12 @@ static int enqueue_iteration(wg_ctx* ctx)
                            ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
     }
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->mlp_set ? ctx->mlp : nullptr);
-    hipLaunchKernelGGL(k_wedge_build, dim3(wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
-                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->ctr,
+    hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
+                       ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->heavy_cap, ctx->ctr,
                        WG_GATE_FRONTIER);
     hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
-    const uint32_t agrid = advance_grid(ctx);
+                       ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+    const uint32_t agrid = wg_advance_grid(ctx);
     if (ctx->app == WG_APP_BFS)
         hipLaunchKernelGGL(k_bfs_advance, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
                            ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_LIVE);
