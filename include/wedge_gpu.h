@@ -222,6 +222,18 @@ int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_
 /* Depths (V entries) from root number `source`, UINT32_MAX if unreached. */
 int wg_read_msbfs(wg_ctx* ctx, uint32_t source, uint32_t* host_depths);
 
+/* Personalised PageRank for up to WG_PPR_MAX_SEEDS seeds in one pass over
+ * the edges per iteration (no reference counterpart; the reference's PageRank
+ * teleports uniformly, source/phase_vertex_pr.c:26-77).  Seed s restarts at s
+ * and its dangling mass returns to s; starting from all mass on s:
+ *   r(v) = d * (sum over in-edges u->v of r(u)/od(u) + [v = s] * D) + [v = s] * (1 - d)
+ * with D the mass on vertices without out-edges.  Contributions are stored in
+ * FP32 (a 128-byte row per vertex), sums and ranks are FP64, in a fixed order. */
+#define WG_PPR_MAX_SEEDS 32
+int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping);
+/* Ranks (V entries, FP64) for seed number `seed`. */
+int wg_read_ppr(wg_ctx* ctx, uint32_t seed, double* host_ranks);
+
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
  * the root holds (0 << 32) | root. */
 int wg_read_bfs(wg_ctx* ctx, uint64_t* host_packed);

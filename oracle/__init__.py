@@ -60,6 +60,8 @@ def _lib():
         lib.wcpu_cc.argtypes = [p, C.c_uint32, f64p, C.POINTER(Stats)]
         lib.wcpu_sssp.restype = C.c_int
         lib.wcpu_sssp.argtypes = [p, C.c_uint64, C.c_uint32, p, C.POINTER(Stats)]
+        lib.wcpu_ppr.restype = C.c_int
+        lib.wcpu_ppr.argtypes = [p, u64p, C.c_uint32, C.c_uint32, C.c_double, f64p]
         _LIB = lib
     return _LIB
 
@@ -149,3 +151,13 @@ class Oracle:
         if rc:
             raise RuntimeError("wcpu_sssp failed")
         return out, st.as_dict()
+
+    def ppr(self, seeds, iterations: int, damping: float):
+        """Personalised PageRank per seed: array [len(seeds)][V] (FP64)."""
+        sd = np.ascontiguousarray(seeds, np.uint64)
+        out = np.empty((sd.size, self.V), np.float64)
+        rc = _lib().wcpu_ppr(self._h, sd.ctypes.data_as(C.POINTER(C.c_uint64)), sd.size, iterations, damping,
+                             out.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc:
+            raise RuntimeError("wcpu_ppr failed")
+        return out

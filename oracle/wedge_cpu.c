@@ -1123,3 +1123,52 @@ int wcpu_sssp(wcpu* w, uint64_t root, uint32_t threshold_pct, void* dist, wcpu_s
         *st = w->stats;
     return 0;
 }
+
+/* ---- personalised PageRank ------------------------------------------------------ */
+
+int wcpu_ppr(wcpu* w, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping, double* ranks)
+{
+    if (!w || !seeds || !ranks || nseeds == 0 || iterations == 0)
+        return -1;
+    const uint64_t V = w->V;
+    for (uint32_t i = 0; i < nseeds; ++i)
+        if (seeds[i] >= V)
+            return -1;
+    double* c = (double*)calloc(V, sizeof(double));
+    double* acc = (double*)calloc(V, sizeof(double));
+    if (!c || !acc) {
+        free(c);
+        free(acc);
+        return -1;
+    }
+    for (uint32_t i = 0; i < nseeds; ++i) {
+        const uint64_t s = seeds[i];
+        memset(c, 0, V * sizeof(double));
+        c[s] = w->outdeg[s] ? 1.0 / (double)w->outdeg[s] : 0.0;
+        double D = w->outdeg[s] ? 0.0 : 1.0;
+        for (uint32_t t = 0; t < iterations; ++t) {
+            memset(acc, 0, V * sizeof(double));
+            for (uint64_t k = 0; k < w->nvec; ++k) {
+                const uint64_t* l = vec_at(w, k);
+                const uint64_t dst = vec_dst(l);
+                for (int q = 0; q < 4; ++q)
+                    if (l[q] & LANE_VALID)
+                        acc[dst] += c[l[q] & LANE_SRC];
+            }
+            double Dn = 0.0;
+            for (uint64_t v = 0; v < V; ++v) {
+                const double a = (v == s) ? acc[v] + D : acc[v];
+                const double r = (v == s) ? a * damping + (1.0 - damping) : a * damping;
+                c[v] = w->outdeg[v] ? r / (double)w->outdeg[v] : 0.0;
+                if (!w->outdeg[v])
+                    Dn += r;
+                if (t + 1 == iterations)
+                    ranks[(uint64_t)i * V + v] = r;
+            }
+            D = Dn;
+        }
+    }
+    free(c);
+    free(acc);
+    return 0;
+}

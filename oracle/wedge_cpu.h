@@ -87,6 +87,14 @@ int wcpu_cc(wcpu* w, uint32_t threshold_pct, double* labels, wcpu_stats* st);
  *       FP64 graphs    -> double distances (+inf unreached). */
 int wcpu_sssp(wcpu* w, uint64_t root, uint32_t threshold_pct, void* dist, wcpu_stats* st);
 
+/* Personalised PageRank (no counterpart in the reference; its PageRank
+ * teleports uniformly).  For each seed s, in FP64, starting from r = e_s:
+ *   r(v) = d * (sum over in-edges u->v of c(u) + [v = s] * D) + [v = s] * (1 - d)
+ *   c(u) = r(u) / od(u) (0 when od(u) = 0), D = sum of r over vertices with od = 0
+ * ranks receives nseeds rows of V values after `iterations` iterations.  In-edge
+ * sums follow the edge-vector order (ascending source within a destination). */
+int wcpu_ppr(wcpu* w, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping, double* ranks);
+
 #ifdef __cplusplus
 }
 #endif

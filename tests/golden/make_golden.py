@@ -1,7 +1,7 @@
 """Writes tests/golden/golden_small.npz: small seeded graphs and their expected
-BFS depths, CC labels, SSSP distances and PageRank ranks, computed with scipy
-(breadth-first shortest paths, connected components, Dijkstra) and a numpy
-sparse mat-vec PageRank loop -- implementations independent of both the
+BFS depths, CC labels, SSSP distances, PageRank and personalised PageRank
+ranks, computed with scipy (breadth-first shortest paths, connected
+components, Dijkstra) and numpy sparse mat-vec PageRank loops -- implementations independent of both the
 oracle and the HIP path.  The reference repository ships no test vectors, so
 these fixtures are the committed known answers.
 
@@ -18,7 +18,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import wedge_amd as W  # noqa: E402
 from helpers import (bfs_depths_scipy, cc_min_labels_undirected, pagerank_closed_form,  # noqa: E402
-                     sssp_dijkstra)
+                     ppr_closed_form, sssp_dijkstra)
 
 CASES = {
     "rmat10_s1": lambda: W.rmat(10, 16, seed=1),
@@ -41,6 +41,9 @@ def main():
         out[f"{name}__bfs_depth"] = bfs_depths_scipy(g, root)
         out[f"{name}__pr_d085_n16"] = pagerank_closed_form(g, 16, 0.85)
         out[f"{name}__pr_d0875_n16"] = pagerank_closed_form(g, 16, 0.875)
+        seeds = np.array([root, 1, 7], np.uint64)
+        out[f"{name}__ppr_seeds"] = seeds
+        out[f"{name}__ppr_d085_n16"] = ppr_closed_form(g, seeds, 16, 0.85)
         if "sym" in name or "grid" in name:
             out[f"{name}__cc"] = cc_min_labels_undirected(g)
         if g.weights is not None:

@@ -42,6 +42,24 @@ def pagerank_closed_form(g, iterations, damping):
     return c * odp
 
 
+def ppr_closed_form(g, seeds, iterations, damping):
+    """Personalised PageRank with a sparse mat-vec, one column per seed:
+    C = R / od (0 where od = 0), D = mass on vertices without out-edges,
+    R' = d * (A^T C + e_s D) + (1 - d) e_s, from R = e_s."""
+    V = g.num_vertices
+    od = g.outdegrees().astype(np.float64)
+    At = adjacency(g).T.tocsr()
+    Es = np.zeros((V, len(seeds)))
+    Es[np.asarray(seeds, np.int64), np.arange(len(seeds))] = 1.0
+    R = Es.copy()
+    sink = od == 0
+    for _ in range(iterations):
+        C = np.where(od[:, None] > 0, R / np.where(od > 0, od, 1.0)[:, None], 0.0)
+        D = R[sink].sum(axis=0)
+        R = damping * (At @ C + Es * D) + (1.0 - damping) * Es
+    return R.T
+
+
 def cc_min_labels_undirected(g):
     n, comp = cg.connected_components(adjacency(g), directed=False)
     mins = np.full(n, np.iinfo(np.int64).max)

@@ -78,3 +78,44 @@ def test_msbfs_errors(ctx):
     ctx.init_bfs(0)
     with pytest.raises(W.WedgeError):
         ctx.read_msbfs(0)  # the batched state was replaced
+
+
+# ---- personalised PageRank ---------------------------------------------------------------
+
+PPR_TOL = 1e-5  # L1 per seed: FP32-stored contributions (expected ~1e-7), FP64 sums
+
+
+@pytest.mark.parametrize("nseeds", [1, 7, 32])
+def test_ppr_matches_oracle(ctx, nseeds):
+    for name, g in _graphs() + [("hub", W.from_edges(6000, np.concatenate([np.random.default_rng(7).integers(1, 6000, 5000),
+                                                                           np.random.default_rng(8).integers(0, 6000, 20000)]),
+                                                     np.concatenate([np.zeros(5000, np.int64),
+                                                                     np.random.default_rng(9).integers(0, 6000, 20000)]),
+                                                     dedup=True))]:
+        ctx.load(g)
+        seeds = _roots(g, nseeds, seed=nseeds + 100)
+        want = oracle.Oracle.of(g, 4).ppr(seeds, 12, 0.85)
+        ctx.ppr(seeds, 12, 0.85)
+        for i in range(nseeds):
+            got = ctx.read_ppr(i)
+            err = np.abs(got - want[i]).sum()
+            assert err <= PPR_TOL, f"{name} seed #{i}: L1 {err:.3e}"
+            assert abs(got.sum() - 1.0) < 1e-6
+        print(f"PPR {name} {nseeds} seeds: max L1 {max(np.abs(ctx.read_ppr(i) - want[i]).sum() for i in range(nseeds)):.2e}")
+
+
+def test_ppr_is_reproducible_and_checks_arguments(ctx):
+    g = W.rmat(12, 16, seed=2)
+    ctx.load(g)
+    seeds = _roots(g, 32, seed=3)
+    ctx.ppr(seeds, 8, 0.85)
+    a = [ctx.read_ppr(i) for i in (0, 31)]
+    ctx.ppr(seeds, 8, 0.85)
+    b = [ctx.read_ppr(i) for i in (0, 31)]
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))  # fixed-order sums
+    with pytest.raises(W.WedgeError):
+        ctx.ppr(np.zeros(33, np.uint64))
+    with pytest.raises(W.WedgeError):
+        ctx.ppr([0], iterations=0)
+    with pytest.raises(W.WedgeError):
+        ctx.read_ppr(32)

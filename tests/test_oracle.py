@@ -53,6 +53,9 @@ def test_oracle_matches_golden(name):
     if f"{name}__cc" in z.files:
         lab, _ = orc.cc()
         assert np.array_equal(lab.astype(np.uint32), z[f"{name}__cc"])
+    ppr = orc.ppr(z[f"{name}__ppr_seeds"], 16, 0.85)
+    assert np.abs(ppr - z[f"{name}__ppr_d085_n16"]).sum() <= 1e-13
+    assert np.allclose(ppr.sum(axis=1), 1.0, rtol=0, atol=1e-12)  # mass is conserved
     if f"{name}__sssp" in z.files:
         dist, _ = orc.sssp(root)
         assert np.array_equal(dist, z[f"{name}__sssp"])  # bit-exact, FP64 included
@@ -119,6 +122,45 @@ def test_bfs_cc_sssp_known_answers():
     gf = _tiny(4, e, weights=np.array(w, np.float64) / 8.0)
     dist, _ = oracle.Oracle.of(gf, 2).sssp(0)
     assert dist.tolist() == [0.0, 3 / 8, 1 / 8, 4 / 8]
+
+
+def test_ppr_known_answers():
+    # 2-cycle, seed 0, d = 1/2: r1 = (1/2, 1/2), r2 = (3/4, 1/4)
+    g = _tiny(2, [(0, 1), (1, 0)])
+    orc = oracle.Oracle.of(g, 1)
+    assert np.allclose(orc.ppr([0], 1, 0.5)[0], [0.5, 0.5], rtol=0, atol=1e-16)
+    assert np.allclose(orc.ppr([0], 2, 0.5)[0], [0.75, 0.25], rtol=0, atol=1e-16)
+    # 0 -> 1 with 1 a sink: its mass returns to the seed
+    d = Fraction(85, 100)
+    r1 = [1 - d, d]
+    r2 = [d * d + 1 - d, d * (1 - d)]
+    g = _tiny(2, [(0, 1)])
+    orc = oracle.Oracle.of(g, 1)
+    assert np.allclose(orc.ppr([0], 1, 0.85)[0], [float(x) for x in r1], rtol=0, atol=1e-15)
+    assert np.allclose(orc.ppr([0], 2, 0.85)[0], [float(x) for x in r2], rtol=0, atol=1e-15)
+    # no damping: all mass stays on the seed
+    g = W.rmat(8, 8, seed=3)
+    r = oracle.Oracle.of(g, 1).ppr([5, 9], 4, 0.0)
+    assert r[0, 5] == 1.0 and r[1, 9] == 1.0 and r.sum() == 2.0
+
+
+def test_planted_mistake_ppr_dangling_mass():
+    g = W.rmat(10, 8, seed=1)
+    seeds = [g.max_outdegree_vertex(), 3]
+    want = oracle.Oracle.of(g, 2).ppr(seeds, 12, 0.85)
+    V = g.num_vertices
+    od = g.outdegrees().astype(np.float64)
+    from helpers import adjacency
+    At = adjacency(g).T.tocsr()
+    # wrong: dangling mass spread uniformly instead of returned to the seed
+    R = np.zeros((V, 2))
+    R[seeds, [0, 1]] = 1.0
+    E = R.copy()
+    for _ in range(12):
+        C = np.where(od[:, None] > 0, R / np.where(od > 0, od, 1.0)[:, None], 0.0)
+        D = R[od == 0].sum(axis=0)
+        R = 0.85 * (At @ C + D / V) + 0.15 * E
+    assert np.abs(R.T - want).sum() > 1e-6
 
 
 def test_cc_directed_is_min_reaching_label():

@@ -145,6 +145,8 @@ GPU_SYMBOLS = [
     ("wg_profile_reset", C.c_int, [_p]),
     ("wg_msbfs", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p, _u64p]),
     ("wg_read_msbfs", C.c_int, [_p, C.c_uint32, _u32p]),
+    ("wg_ppr", C.c_int, [_p, _u64p, C.c_uint32, C.c_uint32, C.c_double]),
+    ("wg_read_ppr", C.c_int, [_p, C.c_uint32, _f64p]),
     ("wg_mlp_action", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("wg_set_schedule_model", C.c_int, [_p, _f32p, C.c_uint64]),
     ("wg_mlp_infer", C.c_int, [_p, C.c_int, _f32p, _f32p, _i32p, C.c_uint64, _f32p, _i32p]),
@@ -599,6 +601,16 @@ class Context:
         self._chk(self._lib.wg_msbfs(self._h, _ptr(r, C.c_uint64), r.size, C.byref(p), C.byref(lv), C.byref(er)),
                   "wg_msbfs")
         return int(lv.value), int(er.value)
+
+    def ppr(self, seeds, iterations=16, damping=0.85):
+        """Personalised PageRank for up to 32 seeds; ranks per seed: read_ppr(i)."""
+        s = np.ascontiguousarray(seeds, np.uint64)
+        self._chk(self._lib.wg_ppr(self._h, _ptr(s, C.c_uint64), s.size, iterations, damping), "wg_ppr")
+
+    def read_ppr(self, seed: int) -> np.ndarray:
+        out = np.empty(self.V, np.float64)
+        self._chk(self._lib.wg_read_ppr(self._h, seed, _ptr(out, C.c_double)), "wg_read_ppr")
+        return out
 
     def read_msbfs(self, source: int) -> np.ndarray:
         out = np.empty(self.V, np.uint32)
