@@ -221,6 +221,14 @@ int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_
              uint64_t* edges_reached);
 /* Depths (V entries) from root number `source`, UINT32_MAX if unreached. */
 int wg_read_msbfs(wg_ctx* ctx, uint32_t source, uint32_t* host_depths);
+/* SSSP from up to 64 roots (weighted graphs): per vertex a row of 64
+ * distances.  iterations: relaxation rounds run (optional).  WG_E_OVERFLOW if
+ * an integer distance reached 2^32 - 1. */
+int wg_mssssp(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_params* params,
+              uint64_t* iterations);
+/* Distances (V entries) from root number `source`: double (+inf unreached) on
+ * FP64-weighted graphs, uint32 (0xFFFFFFFF unreached) on integer ones. */
+int wg_read_mssssp(wg_ctx* ctx, uint32_t source, void* host_dist);
 
 /* Personalised PageRank for up to WG_PPR_MAX_SEEDS seeds in one pass over
  * the edges per iteration (no reference counterpart; the reference's PageRank

@@ -119,3 +119,36 @@ def test_ppr_is_reproducible_and_checks_arguments(ctx):
         ctx.ppr([0], iterations=0)
     with pytest.raises(W.WedgeError):
         ctx.read_ppr(32)
+
+
+# ---- SSSP from many sources ----------------------------------------------------------------
+
+
+@pytest.mark.parametrize("nroots", [1, 6, 64])
+def test_mssssp_matches_single_source_oracle(ctx, nroots):
+    for name, g in (("rmat_w", W.rmat(12, 16, seed=2, weights="f64")), ("grid", W.grid(40, 70, seed=4))):
+        ctx.load(g)
+        roots = _roots(g, nroots, seed=nroots + 50)
+        orc = oracle.Oracle.of(g, 4)
+        want = [orc.sssp(int(r))[0] for r in roots]
+        for opts in (dict(schedule="plain"), dict(schedule="dense"), dict(schedule="frontier")):
+            ctx.mssssp(roots, **opts)
+            for i in range(nroots):
+                got = ctx.read_mssssp(i)
+                if got.dtype == np.uint32:
+                    got = np.where(got == W.UNREACHED_U32, np.uint64(2**64 - 1), got.astype(np.uint64))
+                assert np.array_equal(got, want[i]), f"{name} {opts} root #{i}"
+
+
+def test_mssssp_errors(ctx):
+    g = W.rmat(8, 4, seed=1)
+    ctx.load(g)
+    with pytest.raises(W.WedgeError):
+        ctx.mssssp([0])  # unweighted graph
+    g = W.grid(10, 10, seed=1)
+    ctx.load(g)
+    with pytest.raises(W.WedgeError):
+        ctx.mssssp(np.zeros(65, np.uint64))
+    ctx.mssssp([0, 5])
+    with pytest.raises(W.WedgeError):
+        ctx.read_mssssp(2)

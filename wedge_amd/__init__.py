@@ -145,6 +145,8 @@ GPU_SYMBOLS = [
     ("wg_profile_reset", C.c_int, [_p]),
     ("wg_msbfs", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p, _u64p]),
     ("wg_read_msbfs", C.c_int, [_p, C.c_uint32, _u32p]),
+    ("wg_mssssp", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p]),
+    ("wg_read_mssssp", C.c_int, [_p, C.c_uint32, _p]),
     ("wg_ppr", C.c_int, [_p, _u64p, C.c_uint32, C.c_uint32, C.c_double]),
     ("wg_read_ppr", C.c_int, [_p, C.c_uint32, _f64p]),
     ("wg_mlp_action", C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -601,6 +603,22 @@ class Context:
         self._chk(self._lib.wg_msbfs(self._h, _ptr(r, C.c_uint64), r.size, C.byref(p), C.byref(lv), C.byref(er)),
                   "wg_msbfs")
         return int(lv.value), int(er.value)
+
+    def mssssp(self, roots, schedule="plain", threshold_pct=20, max_iterations=0):
+        """SSSP from up to 64 roots in one traversal; returns the iterations run.
+        Distances per root: read_mssssp(i)."""
+        r = np.ascontiguousarray(roots, np.uint64)
+        p = LoopParams(schedule=SCHEDULES[schedule] if isinstance(schedule, str) else schedule,
+                       threshold_pct=threshold_pct, block_log2=0, max_iterations=max_iterations)
+        it = C.c_uint64()
+        self._chk(self._lib.wg_mssssp(self._h, _ptr(r, C.c_uint64), r.size, C.byref(p), C.byref(it)), "wg_mssssp")
+        return int(it.value)
+
+    def read_mssssp(self, source: int) -> np.ndarray:
+        f64 = self.graph is not None and self.graph.weight_kind == WEIGHT_F64
+        out = np.empty(self.V, np.float64 if f64 else np.uint32)
+        self._chk(self._lib.wg_read_mssssp(self._h, source, out.ctypes.data), "wg_read_mssssp")
+        return out
 
     def ppr(self, seeds, iterations=16, damping=0.85):
         """Personalised PageRank for up to 32 seeds; ranks per seed: read_ppr(i)."""

@@ -420,3 +420,341 @@ extern "C" int wg_read_msbfs(wg_ctx* ctx, uint32_t source, uint32_t* host_depths
     (void)hipFree(tmp);
     return rc;
 }
+
+// ---- SSSP from up to 64 sources ---------------------------------------------------------
+//
+//   dist[v][64]  distance from source s (lane s): FP64 bit patterns (non-negative
+//                doubles order like their bits) or uint32, INF if unreached
+//   chg[v]       sources whose distance to v improved in the last iteration
+// An iteration relaxes destinations, one wavefront per destination, lane s
+// taking min over in-edges u->v of dist[u][s] + w; the row of u is read only
+// when some source improved u.  Dense iterations relax every destination;
+// sparse ones first mark the out-neighbours of the improved vertices and
+// relax only those.  Minimum is order-independent, so the result is each
+// source's single-source SSSP fixpoint (source/phase_edge_pull_sssp.c:57-169).
+
+struct MsF64 {
+    typedef unsigned long long T;
+    static constexpr unsigned long long INF = 0x7FF0000000000000ull;
+    __device__ static T add(T d, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__)
+    {
+        return (T)__double_as_longlong(__longlong_as_double((long long)d) + ((const double*)w)[p]);
+    }
+};
+
+struct MsU32 {
+    typedef uint32_t T;
+    static constexpr uint32_t INF = 0xffffffffu;
+    __device__ static T add(T d, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
+    {
+        const uint32_t s = d + ((const uint32_t*)w)[p];
+        if (s < d || s == INF) {
+            *ovf = 1u;
+            return INF;
+        }
+        return s;
+    }
+};
+
+template <class P>
+__global__ void k_mss_init(typename P::T* __restrict__ dist, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dist[i] = P::INF;
+}
+
+template <class P>
+__global__ void k_mss_roots(const uint32_t* __restrict__ roots, uint32_t n, typename P::T* __restrict__ dist,
+                            unsigned long long* __restrict__ chg, unsigned long long* __restrict__ bm,
+                            const uint32_t* __restrict__ od, wg_dev_counters* __restrict__ ctr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t r = roots[i];
+    dist[(uint64_t)r * WG_MS_MAX_SOURCES + i] = 0;
+    atomicOr(&chg[r], 1ull << i);
+    const unsigned long long bit = 1ull << (r & 63);
+    if (!(atomicOr(&bm[r >> 6], bit) & bit))
+        atomicAdd(&ctr->slots[0].v[WG_CNT_ACTIVITY], (unsigned long long)od[r]);
+}
+
+// Marks the out-neighbours of the vertices set in `active` in `touched`.
+// Lanes take vertices with few out-edges; longer lists are walked by the
+// whole wavefront.
+__global__ __launch_bounds__(WG_BLOCK) void k_mss_touch(const uint64_t* __restrict__ active, uint64_t nwords,
+                                                         const uint32_t* __restrict__ csri_off,
+                                                         const uint32_t* __restrict__ csri_pos,
+                                                         const uint32_t* __restrict__ dst,
+                                                         unsigned long long* __restrict__ touched)
+{
+    const int lane = wg_lane();
+    const uint64_t nwaves = (uint64_t)gridDim.x * WG_WPB;
+    for (uint64_t w = (uint64_t)blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
+        const uint64_t bits = active[w];
+        if (!bits)
+            continue;
+        const uint32_t u = (uint32_t)(w * 64 + (uint64_t)lane);
+        uint32_t s = 0, e = 0;
+        if ((bits >> lane) & 1ull) {
+            s = csri_off[u];
+            e = csri_off[u + 1];
+        }
+        const bool big = e - s > WG_WEDGE_LANE_MAX;
+        if (!big)
+            for (uint32_t p = s; p < e; ++p) {
+                const uint32_t v = dst[csri_pos[p]];
+                atomicOr(&touched[v >> 6], 1ull << (v & 63));
+            }
+        unsigned long long many = wg_ballot(big);
+        while (many) {
+            const int l = __ffsll((long long)many) - 1;
+            many &= many - 1ull;
+            const uint32_t ls = __shfl(s, l), le = __shfl(e, l);
+            for (uint32_t p = ls + (uint32_t)lane; p < le; p += WG_WAVE) {
+                const uint32_t v = dst[csri_pos[p]];
+                atomicOr(&touched[v >> 6], 1ull << (v & 63));
+            }
+        }
+    }
+}
+
+// Minimum over in-edges [b, e) of dist[u][lane] + w for the sources improved
+// at u (lane = source), four edges in flight.
+template <class P>
+__device__ __forceinline__ typename P::T mss_min(uint32_t b, uint32_t e, const uint32_t* __restrict__ src,
+                                                 const void* __restrict__ w, const typename P::T* __restrict__ dist,
+                                                 const unsigned long long* __restrict__ chg, int lane,
+                                                 unsigned int* __restrict__ ovf)
+{
+    typedef typename P::T T;
+    T m = P::INF;
+    for (uint32_t p = b; p < e; p += 4) {
+        uint32_t u[4];
+        unsigned long long c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            u[k] = p + k < e ? src[p + k] : 0u;
+            c[k] = p + k < e ? chg[u[k]] : 0ull;
+        }
+        T x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            x[k] = ((c[k] >> lane) & 1ull) ? dist[(uint64_t)u[k] * WG_MS_MAX_SOURCES + lane] : P::INF;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (x[k] != P::INF) {
+                const T cand = P::add(x[k], w, p + k, ovf);
+                m = cand < m ? cand : m;
+            }
+    }
+    return m;
+}
+
+// Relaxation over the edge tiles, one wavefront per tile, lane = source.  A
+// light tile relaxes its destinations one after the other and owns them; a
+// heavy tile (part of one destination's in-edges) folds its minimum in with
+// per-lane atomics.  With `sel`, only the destinations set in it are relaxed.
+// Improved sources go to chg_next[v], improved vertices to the bitmask bm.
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_mss_relax(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const void* __restrict__ w, const uint32_t* __restrict__ od,
+                                                         const uint32_t* __restrict__ tile_v,
+                                                         const uint32_t* __restrict__ tile_e, uint32_t ntiles,
+                                                         typename P::T* __restrict__ dist,
+                                                         const unsigned long long* __restrict__ chg,
+                                                         unsigned long long* __restrict__ chg_next,
+                                                         const uint64_t* __restrict__ sel, unsigned long long* __restrict__ bm,
+                                                         unsigned int* __restrict__ ovf, wg_dev_counters* __restrict__ ctr)
+{
+    typedef typename P::T T;
+    const int lane = wg_lane();
+    const uint32_t stride = gridDim.x * WG_WPB;
+    act_tally t = { 0ull, 0ull, 0ull };
+    for (uint32_t ti = blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; ti < ntiles; ti += stride) {
+        const uint32_t tv = tile_v[ti];
+        const uint32_t v0 = tv & ~WG_HEAVY_BIT;
+        if (tv & WG_HEAVY_BIT) {
+            if (sel && !((sel[v0 >> 6] >> (v0 & 63)) & 1ull))
+                continue;
+            const T m = mss_min<P>(tile_e[ti], tile_e[ti + 1], src, w, dist, chg, lane, ovf);
+            const uint64_t slot = (uint64_t)v0 * WG_MS_MAX_SOURCES + lane;
+            bool better = false;
+            if (m < dist[slot]) {
+                const T old = atomic_min_t<T>(&dist[slot], m);
+                better = m < old;
+                if (better && old == P::INF)
+                    t.fresh += 1ull; // counted once: only the atomic that replaced INF sees it
+            }
+            const unsigned long long nm = wg_ballot(better);
+            if (lane == 0 && nm) {
+                const unsigned long long was = atomicOr(&chg_next[v0], nm);
+                if (!was) { // first improvement of v0 this iteration
+                    t.edges += od[v0];
+                    t.verts += 1ull;
+                    atomicOr(&bm[v0 >> 6], 1ull << (v0 & 63));
+                }
+            }
+            continue;
+        }
+        const uint32_t v1 = tile_v[ti + 1] & ~WG_HEAVY_BIT;
+        for (uint32_t v = v0; v < v1; ++v) {
+            if (sel && !((sel[v >> 6] >> (v & 63)) & 1ull))
+                continue;
+            const T m = mss_min<P>(off[v], off[v + 1], src, w, dist, chg, lane, ovf);
+            const uint64_t slot = (uint64_t)v * WG_MS_MAX_SOURCES + lane;
+            const T old = dist[slot];
+            const bool better = m < old;
+            if (better)
+                dist[slot] = m;
+            const unsigned long long nm = wg_ballot(better);
+            if (nm) {
+                const unsigned long long fresh = wg_ballot(better && old == P::INF);
+                if (lane == 0) {
+                    chg_next[v] = nm;
+                    t.edges += od[v];
+                    t.verts += 1ull;
+                    t.fresh += (unsigned long long)__popcll(fresh);
+                    atomicOr(&bm[v >> 6], 1ull << (v & 63));
+                }
+            }
+        }
+    }
+    add_tally(t, ctr); // counts sit in lane 0; the wave sum adds the zeros of the others
+}
+
+template <class P>
+__global__ void k_mss_column(const typename P::T* __restrict__ dist, uint32_t s, uint64_t V, typename P::T* __restrict__ out)
+{
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = dist[v * WG_MS_MAX_SOURCES + s];
+}
+
+template <class P>
+static int mssssp_run(wg_ctx* ctx, const std::vector<uint32_t>& roots, const wg_loop_params* params, uint64_t* iters)
+{
+    typedef typename P::T T;
+    const uint32_t V = (uint32_t)ctx->V, n = (uint32_t)roots.size();
+    const uint64_t nw = ctx->bm_words;
+    void *dist = nullptr, *c0 = nullptr, *c1 = nullptr, *aux = nullptr;
+    int rc;
+    if ((rc = ms_alloc(ctx, (size_t)V * WG_MS_MAX_SOURCES * sizeof(T), &dist)) ||
+        (rc = ms_alloc(ctx, (size_t)V * 8, &c0)) || (rc = ms_alloc(ctx, (size_t)V * 8, &c1)) ||
+        (rc = ms_alloc(ctx, nw * 8 + (size_t)n * 4, &aux))) {
+        (void)hipFree(dist);
+        (void)hipFree(c0);
+        (void)hipFree(c1);
+        (void)hipFree(aux);
+        return rc;
+    }
+    ctx->prop[0] = dist;
+    ctx->ms[0] = c0;
+    ctx->ms[1] = c1;
+    ctx->ms[2] = aux;
+    ctx->ms_n = n;
+    ctx->app = WG_APP_MSSSSP;
+    unsigned long long* touched = (unsigned long long*)aux;
+    uint32_t* droots = (uint32_t*)((char*)aux + nw * 8);
+    unsigned long long *chg = (unsigned long long*)c0, *chg_next = (unsigned long long*)c1;
+    unsigned int* ovf = &ctx->ctr->overflow;
+    WG_HIP(ctx, hipMemcpyAsync(droots, roots.data(), (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_mss_init<P>, dim3(wg_grid(ctx, (uint64_t)V * WG_MS_MAX_SOURCES, 1024, 8)), dim3(256), 0,
+                       ctx->stream, (T*)dist, (uint64_t)V * WG_MS_MAX_SOURCES);
+    if ((rc = wg_check_launch(ctx)) || (rc = wg_clear_u64(ctx, (uint64_t*)chg, V)) ||
+        (rc = wg_clear_u64(ctx, (uint64_t*)chg_next, V)) || (rc = wg_clear_u64(ctx, (uint64_t*)touched, nw)) ||
+        (rc = wg_clear_u64(ctx, ctx->bm_next, nw)) || (rc = wg_reset_counters(ctx, RST_ACTIVITY | RST_OVERFLOW)))
+        return rc;
+    hipLaunchKernelGGL(k_mss_roots<P>, dim3((n + 63) / 64), dim3(64), 0, ctx->stream, droots, n, (T*)dist, chg,
+                       (unsigned long long*)ctx->bm_next, ctx->od, ctx->ctr);
+    if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
+        return rc;
+    unsigned long long act = ctx->hctr->activity;
+    const unsigned long long threshold = (ctx->E * (unsigned long long)params->threshold_pct) / 100ull;
+    const uint64_t cap = 64ull * V + 64ull;
+    uint64_t it = 0;
+    for (;;) {
+        const bool dense = params->schedule == 2 || (params->schedule != 3 && act > threshold);
+        if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
+            return rc;
+        if (!dense) {
+            hipLaunchKernelGGL(k_mss_touch, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream,
+                               ctx->bm_next, nw, ctx->csri_off, ctx->csri_pos, ctx->dst, touched);
+            if ((rc = wg_check_launch(ctx)))
+                return rc;
+        }
+        // the improved-vertex bitmask is rebuilt by the relaxation
+        if ((rc = wg_clear_u64(ctx, ctx->bm_next, nw)))
+            return rc;
+        hipLaunchKernelGGL(k_mss_relax<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
+                           ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, (T*)dist, chg, chg_next,
+                           dense ? (const uint64_t*)nullptr : (const uint64_t*)touched,
+                           (unsigned long long*)ctx->bm_next, ovf, ctx->ctr);
+        if ((rc = wg_check_launch(ctx)) || (!dense && (rc = wg_clear_u64(ctx, (uint64_t*)touched, nw))) ||
+            (rc = wg_read_counters(ctx)))
+            return rc;
+        ++it;
+        if (ctx->hctr->activated == 0)
+            break;
+        if (it > cap || (params->max_iterations && it >= params->max_iterations))
+            break;
+        act = ctx->hctr->activity;
+        // the improved masks become the next iteration's input; the old ones are cleared for reuse
+        unsigned long long* tmp = chg;
+        chg = chg_next;
+        chg_next = tmp;
+        if ((rc = wg_clear_u64(ctx, (uint64_t*)chg_next, V)))
+            return rc;
+    }
+    if (iters)
+        *iters = it;
+    return ctx->hctr->overflow ? WG_E_OVERFLOW : WG_OK;
+}
+
+extern "C" int wg_mssssp(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_params* params,
+                         uint64_t* iterations)
+{
+    if (!ctx || !roots || !params)
+        return WG_E_INVAL;
+    if (!ctx->off || ctx->wkind == WG_WEIGHT_NONE)
+        return WG_E_STATE;
+    if (nroots == 0 || nroots > WG_MS_MAX_SOURCES || params->schedule < 0 || params->schedule > 3)
+        return WG_E_INVAL;
+    std::vector<uint32_t> hroots(nroots);
+    int rc;
+    for (uint32_t i = 0; i < nroots; ++i)
+        if ((rc = wg_to_device_id(ctx, roots[i], &hroots[i])))
+            return rc;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    wg_free_app_state(ctx);
+    rc = ctx->wkind == WG_WEIGHT_F64 ? mssssp_run<MsF64>(ctx, hroots, params, iterations)
+                                     : mssssp_run<MsU32>(ctx, hroots, params, iterations);
+    if (rc == WG_OK || rc == WG_E_OVERFLOW)
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return rc;
+}
+
+extern "C" int wg_read_mssssp(wg_ctx* ctx, uint32_t source, void* host_dist)
+{
+    if (!ctx || !host_dist)
+        return WG_E_INVAL;
+    if (ctx->app != WG_APP_MSSSSP)
+        return WG_E_STATE;
+    if (source >= ctx->ms_n)
+        return WG_E_RANGE;
+    const size_t esz = ctx->wkind == WG_WEIGHT_F64 ? 8 : 4;
+    void* tmp = nullptr;
+    WG_HIP(ctx, hipMalloc(&tmp, ctx->V * esz));
+    const uint32_t grid = wg_grid(ctx, ctx->V, 1024, 8);
+    if (esz == 8)
+        hipLaunchKernelGGL(k_mss_column<MsF64>, dim3(grid), dim3(256), 0, ctx->stream,
+                           (const unsigned long long*)ctx->prop[0], source, ctx->V, (unsigned long long*)tmp);
+    else
+        hipLaunchKernelGGL(k_mss_column<MsU32>, dim3(grid), dim3(256), 0, ctx->stream, (const uint32_t*)ctx->prop[0],
+                           source, ctx->V, (uint32_t*)tmp);
+    int rc = wg_check_launch(ctx);
+    if (!rc)
+        rc = wg_read_vertex_array(ctx, tmp, host_dist, esz);
+    (void)hipFree(tmp);
+    return rc;
+}
