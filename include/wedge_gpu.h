@@ -75,7 +75,8 @@ typedef struct wg_ctx wg_ctx;
 #define WG_K_ADVANCE        9
 #define WG_K_PR_ACC        10  /* stand-alone PageRank edge pull (accumulators) */
 #define WG_K_LOOP          11  /* whole device-resident iteration loop (wg_converge) */
-#define WG_K_COUNT         12
+#define WG_K_BATCH_PR      12  /* whole batched PageRank run (wg_batch_pr) */
+#define WG_K_COUNT         13
 
 /* ---- context -------------------------------------------------------------- */
 
@@ -131,6 +132,11 @@ int wg_init_cc(wg_ctx* ctx);
 /* source/execution_sssp.c:70-89: dist = +inf, dist[root] = 0, root active.
  * Distances are FP64 for WG_WEIGHT_F64 graphs, uint32 for WG_WEIGHT_U32. */
 int wg_init_sssp(wg_ctx* ctx, uint64_t root);
+/* The same with several roots at depth / distance 0 (repeats allowed): on a
+ * graph made of disjoint members, one root per member runs every member's
+ * BFS / SSSP in one traversal (wg_converge runs it). */
+int wg_init_bfs_roots(wg_ctx* ctx, const uint64_t* roots, uint32_t n);
+int wg_init_sssp_roots(wg_ctx* ctx, const uint64_t* roots, uint32_t n);
 
 /* ---- PageRank phases ------------------------------------------------------ */
 
@@ -241,6 +247,49 @@ int wg_read_mssssp(wg_ctx* ctx, uint32_t source, void* host_dist);
 int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping);
 /* Ranks (V entries, FP64) for seed number `seed`. */
 int wg_read_ppr(wg_ctx* ctx, uint32_t seed, double* host_ranks);
+
+/* ---- batches of independent graphs (no reference counterpart) ------------
+ * A batch of small graphs is loaded as one graph, their disjoint union:
+ * member g owns the vertices [first[g], first[g+1]) and every edge stays
+ * inside its member.  Load the union with WG_OPT_RELABEL = WG_RELABEL_OFF so
+ * that the members keep their vertex ranges.  Then
+ *  - BFS / SSSP from one root per member (wg_init_bfs_roots /
+ *    wg_init_sssp_roots + wg_converge) and CC (wg_init_cc + wg_converge) run
+ *    every member in one traversal, with results equal to per-member runs
+ *    shifted by first[g];
+ *  - wg_batch_pr runs the reference's PageRank (source/phase_edge_pull_pr.c,
+ *    source/phase_vertex_pr.c:26-77) on every member at once, each with its
+ *    own N = member size and S = sum of the member's accumulators. */
+int wg_set_members(wg_ctx* ctx, uint32_t n, const uint64_t* first /* n + 1 entries */);
+
+/* A launch shape of the batched PageRank: edges per lane of a tile (a tile
+ * holds up to 64 * edges_per_lane edges; 4, 8 or 16) and workgroups per
+ * compute unit of the tile kernel (1..16). */
+typedef struct wg_batch_shape {
+    uint32_t edges_per_lane;
+    uint32_t blocks_per_cu;
+} wg_batch_shape;
+#define WG_BATCH_NSHAPES 12
+/* The autotuner's candidate number i (0 <= i < WG_BATCH_NSHAPES). */
+int wg_batch_shape_at(uint32_t i, wg_batch_shape* out);
+
+/* `iterations` PageRank iterations on every member from the uniform start.
+ * shape: NULL = the tuned shape (wg_batch_autotune) or {8, 8}.  use_graph: 1 =
+ * all iterations are captured once into a HIP graph and replayed as one
+ * launch (the graph is kept and reused while iterations, damping, precision
+ * and shape stay the same); 0 = plain launches.  Both give bit-identical
+ * ranks.  Returns when the work is queued: wg_sync or wg_read_batch_ranks
+ * waits for it. */
+int wg_batch_pr(wg_ctx* ctx, uint32_t iterations, double damping, int contrib_fp32, const wg_batch_shape* shape,
+                int use_graph);
+/* Times every candidate shape (one graph launch of `iterations` iterations,
+ * best of three after a warm-up), keeps the fastest for shape == NULL and
+ * returns it; ms_per_shape (WG_BATCH_NSHAPES entries, optional) receives each
+ * candidate's time. */
+int wg_batch_autotune(wg_ctx* ctx, uint32_t iterations, double damping, int contrib_fp32, wg_batch_shape* best,
+                      float* ms_per_shape);
+/* Ranks of the last wg_batch_pr (V entries, FP64; each member sums to ~1). */
+int wg_read_batch_ranks(wg_ctx* ctx, double* host_ranks);
 
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
  * the root holds (0 << 32) | root. */

@@ -141,3 +141,29 @@ def test_generator_argument_errors():
         W.rmat(10, 16, seed=1, a=0.9, b=0.2, c=0.1)
     with pytest.raises(W.WedgeError):
         W.grid(1, 5, seed=1)
+
+
+def test_union_keeps_every_member_in_its_range():
+    ms = [W.rmat(5, 4, seed=1), W.from_edges(1, [], []), W.rmat(7, 8, seed=2, symmetrize=True)]
+    u, first = W.union(ms)
+    assert first.tolist() == [0, ms[0].num_vertices, ms[0].num_vertices + 1, u.num_vertices]
+    assert u.num_edges == sum(m.num_edges for m in ms)
+    dst, src = u.destinations().astype(np.int64), u.sources.astype(np.int64)
+    for g, m in enumerate(ms):
+        a, b = int(first[g]), int(first[g + 1])
+        sel = (dst >= a) & (dst < b)
+        assert np.all((src[sel] >= a) & (src[sel] < b))  # no edge leaves its member
+        assert np.array_equal(src[sel] - a, m.sources.astype(np.int64))
+        assert np.array_equal(dst[sel] - a, m.destinations().astype(np.int64))
+    with pytest.raises(ValueError):
+        W.union([W.rmat(4, 4, seed=1), W.rmat(4, 4, seed=1, weights="f64")])
+    with pytest.raises(ValueError):
+        W.union([])
+
+
+def test_batch_shapes_are_listed():
+    shapes = [W.batch_shape_at(i) for i in range(W.BATCH_NSHAPES)]
+    assert len(set(shapes)) == W.BATCH_NSHAPES
+    assert {s[0] for s in shapes} == {4, 8, 16}
+    with pytest.raises(W.WedgeError):
+        W.batch_shape_at(W.BATCH_NSHAPES)

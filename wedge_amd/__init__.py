@@ -49,7 +49,7 @@ MAX_BLOCK_LOG2 = 10
 KERNELS = {
     "pr_tile": 0, "pr_span": 1, "pr_vertex": 2, "min_tile": 3, "min_blocks": 4,
     "bfs_dense": 5, "bfs_heavy": 6, "bfs_blocks": 7, "wedge_build": 8,
-    "advance": 9, "pr_acc": 10, "loop": 11,
+    "advance": 9, "pr_acc": 10, "loop": 11, "batch_pr": 12,
 }
 
 OPT_RELABEL = 1
@@ -124,6 +124,13 @@ GPU_SYMBOLS = [
     ("wg_init_bfs", C.c_int, [_p, C.c_uint64]),
     ("wg_init_cc", C.c_int, [_p]),
     ("wg_init_sssp", C.c_int, [_p, C.c_uint64]),
+    ("wg_init_bfs_roots", C.c_int, [_p, _u64p, C.c_uint32]),
+    ("wg_init_sssp_roots", C.c_int, [_p, _u64p, C.c_uint32]),
+    ("wg_set_members", C.c_int, [_p, C.c_uint32, _u64p]),
+    ("wg_batch_shape_at", C.c_int, [C.c_uint32, _p]),
+    ("wg_batch_pr", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, C.c_int]),
+    ("wg_batch_autotune", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, _f32p]),
+    ("wg_read_batch_ranks", C.c_int, [_p, _f64p]),
     ("wg_edge_pull_pr", C.c_int, [_p, _f64p]),
     ("wg_vertex_pr", C.c_int, [_p]),
     ("wg_pr_iteration", C.c_int, [_p, _f64p]),
@@ -154,6 +161,16 @@ GPU_SYMBOLS = [
     ("wg_mlp_infer", C.c_int, [_p, C.c_int, _f32p, _f32p, _i32p, C.c_uint64, _f32p, _i32p]),
     ("wg_mlp_train", C.c_int, [_p, _f32p, _i32p, _f32p, C.c_uint64, C.POINTER(MlpTrainOpts), _f32p, _f32p]),
 ]
+
+
+class BatchShape(C.Structure):
+    _fields_ = [("edges_per_lane", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+    def as_tuple(self):
+        return int(self.edges_per_lane), int(self.blocks_per_cu)
+
+
+BATCH_NSHAPES = 12
 
 
 class RunOpts(C.Structure):
@@ -371,6 +388,41 @@ def from_edges(num_vertices: int, src, dst, weights=None, dedup=False) -> Graph:
     return Graph(off, src.astype(np.uint32), w)
 
 
+def union(graphs) -> tuple[Graph, np.ndarray]:
+    """Disjoint union of member graphs: member g's vertex v becomes first[g] + v.
+    Returns the union and first (uint64[n + 1]).  All members must carry the
+    same weight kind."""
+    graphs = list(graphs)
+    if not graphs:
+        raise ValueError("union of no graphs")
+    kinds = {g.weight_kind for g in graphs}
+    if len(kinds) != 1:
+        raise ValueError("members differ in weight kind")
+    first = np.zeros(len(graphs) + 1, np.uint64)
+    np.cumsum([g.num_vertices for g in graphs], out=first[1:])
+    ecount = np.zeros(len(graphs) + 1, np.uint64)
+    np.cumsum([g.num_edges for g in graphs], out=ecount[1:])
+    if first[-1] >= 2**32 or ecount[-1] >= 2**32:
+        raise ValueError("union exceeds 32-bit vertex or edge indices")
+    off = np.empty(int(first[-1]) + 1, np.uint32)
+    src = np.empty(int(ecount[-1]), np.uint32)
+    for i, g in enumerate(graphs):
+        v0, e0 = int(first[i]), int(ecount[i])
+        off[v0:v0 + g.num_vertices] = g.offsets[:-1].astype(np.uint64) + np.uint64(e0)
+        src[e0:e0 + g.num_edges] = g.sources.astype(np.uint64) + np.uint64(v0)
+    off[-1] = np.uint32(ecount[-1])
+    w = None
+    if graphs[0].weights is not None:
+        w = np.concatenate([g.weights for g in graphs])
+    return Graph(off, src, w), first
+
+
+def batch_shape_at(i: int) -> tuple[int, int]:
+    s = BatchShape()
+    _check(_lib_gpu().wg_batch_shape_at(i, C.byref(s)), "wg_batch_shape_at")
+    return s.as_tuple()
+
+
 def weight_f64(src: int, dst: int, seed: int) -> float:
     return float(_lib_host().wg_weight_f64(src, dst, seed))
 
@@ -478,6 +530,47 @@ class Context:
 
     def init_bfs(self, root: int):
         self._chk(self._lib.wg_init_bfs(self._h, root), "wg_init_bfs")
+
+    def init_bfs_roots(self, roots):
+        r = np.ascontiguousarray(roots, np.uint64)
+        self._chk(self._lib.wg_init_bfs_roots(self._h, _ptr(r, C.c_uint64), r.size), "wg_init_bfs_roots")
+
+    def init_sssp_roots(self, roots):
+        r = np.ascontiguousarray(roots, np.uint64)
+        self._chk(self._lib.wg_init_sssp_roots(self._h, _ptr(r, C.c_uint64), r.size), "wg_init_sssp_roots")
+
+    def load_batch(self, graphs) -> np.ndarray:
+        """Loads member graphs as one disjoint union (relabelling off, so every
+        member keeps its vertex range) and registers the members.  Returns first."""
+        g, first = union(graphs)
+        self.set_option(OPT_RELABEL, RELABEL_OFF)
+        self.load(g)
+        self.set_members(first)
+        return first
+
+    def set_members(self, first):
+        f = np.ascontiguousarray(first, np.uint64)
+        self._chk(self._lib.wg_set_members(self._h, f.size - 1, _ptr(f, C.c_uint64)), "wg_set_members")
+        self.members = f
+
+    def batch_pr(self, iterations=16, damping=0.85, contrib_fp32=False, shape=None, use_graph=True):
+        """PageRank on every member (queued; read_batch_ranks waits)."""
+        s = None if shape is None else C.byref(BatchShape(*shape))
+        self._chk(self._lib.wg_batch_pr(self._h, iterations, damping, int(contrib_fp32), s, int(use_graph)),
+                  "wg_batch_pr")
+
+    def batch_autotune(self, iterations=16, damping=0.85, contrib_fp32=False):
+        """Times every candidate shape; returns (best shape, {shape: ms})."""
+        best = BatchShape()
+        ms = np.zeros(BATCH_NSHAPES, np.float32)
+        self._chk(self._lib.wg_batch_autotune(self._h, iterations, damping, int(contrib_fp32), C.byref(best),
+                                              _ptr(ms, C.c_float)), "wg_batch_autotune")
+        return best.as_tuple(), {batch_shape_at(i): float(ms[i]) for i in range(BATCH_NSHAPES)}
+
+    def read_batch_ranks(self) -> np.ndarray:
+        out = np.empty(self.V, np.float64)
+        self._chk(self._lib.wg_read_batch_ranks(self._h, _ptr(out, C.c_double)), "wg_read_batch_ranks")
+        return out
 
     def init_cc(self):
         self._chk(self._lib.wg_init_cc(self._h), "wg_init_cc")

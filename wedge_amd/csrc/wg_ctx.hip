@@ -403,12 +403,15 @@ int wg_free_app_state(wg_ctx* ctx)
     free_dev(&ctx->ms[2]);
     ctx->ms_n = 0;
     free_dev((void**)&ctx->acc);
+    free_dev((void**)&ctx->roots);
+    ctx->nroots = 0;
     ctx->app = WG_APP_NONE;
     return WG_OK;
 }
 
 static void free_graph(wg_ctx* ctx)
 {
+    wg_free_members(ctx);
     wg_free_app_state(ctx);
     free_dev((void**)&ctx->off);
     free_dev((void**)&ctx->src);
@@ -464,18 +467,18 @@ extern "C" int wg_sync(wg_ctx* ctx)
     return WG_OK;
 }
 
-// Host-side tiling of the destinations (see wg_internal.h).
-static void build_tiles(const uint32_t* off, uint64_t V, std::vector<uint32_t>& tv, std::vector<uint32_t>& te,
-                        std::vector<uint32_t>& hv, std::vector<uint32_t>& ht, std::vector<uint32_t>& bv,
-                        std::vector<uint32_t>& be)
+// Host-side tiling of the destinations (see wg_internal.h) for tiles of at
+// most tile_edges edges and WG_TILE_VERTS destinations.
+void wg_build_tiles(const uint32_t* off, uint64_t V, uint32_t tile_edges, std::vector<uint32_t>& tv,
+                    std::vector<uint32_t>& te, std::vector<uint32_t>& hv, std::vector<uint32_t>& ht)
 {
     uint64_t v = 0;
     while (v < V) {
         const uint32_t d = off[v + 1] - off[v];
-        if (d > WG_TILE_EDGES) {
+        if (d > tile_edges) {
             hv.push_back((uint32_t)v);
             ht.push_back((uint32_t)tv.size());
-            for (uint64_t e = off[v]; e < off[v + 1]; e += WG_TILE_EDGES) {
+            for (uint64_t e = off[v]; e < off[v + 1]; e += tile_edges) {
                 tv.push_back((uint32_t)v | WG_HEAVY_BIT);
                 te.push_back((uint32_t)e);
             }
@@ -487,7 +490,7 @@ static void build_tiles(const uint32_t* off, uint64_t V, std::vector<uint32_t>& 
         uint32_t ecount = 0, vcount = 0;
         while (v < V && vcount < WG_TILE_VERTS) {
             const uint32_t dd = off[v + 1] - off[v];
-            if (dd > WG_TILE_EDGES || ecount + dd > WG_TILE_EDGES)
+            if (dd > tile_edges || ecount + dd > tile_edges)
                 break;
             ecount += dd;
             ++vcount;
@@ -496,6 +499,13 @@ static void build_tiles(const uint32_t* off, uint64_t V, std::vector<uint32_t>& 
     }
     tv.push_back((uint32_t)V);
     te.push_back(off[V]);
+}
+
+static void build_tiles(const uint32_t* off, uint64_t V, std::vector<uint32_t>& tv, std::vector<uint32_t>& te,
+                        std::vector<uint32_t>& hv, std::vector<uint32_t>& ht, std::vector<uint32_t>& bv,
+                        std::vector<uint32_t>& be)
+{
+    wg_build_tiles(off, V, WG_TILE_EDGES, tv, te, hv, ht);
     for (uint64_t u = 0; u < V; ++u) {
         const uint32_t d = off[u + 1] - off[u];
         if (d > WG_BFS_HEAVY)

@@ -23,6 +23,8 @@
 
 #include <stdint.h>
 
+#include <vector>
+
 #include "wedge_gpu.h"
 
 #define WG_WAVE 64
@@ -171,7 +173,11 @@ struct wg_ctx {
     int mlp_set;              // a model is set for the learned schedule
     wg_dev_counters* ctr;     // device
     wg_dev_counters* hctr;    // pinned host mirror
-    uint32_t root_dev;        // BFS / SSSP root in device numbering
+    uint32_t root_dev;        // BFS / SSSP root in device numbering (the first root)
+    uint32_t* roots;          // BFS / SSSP roots in device numbering, distinct
+    uint32_t nroots;
+    unsigned long long root_activity; // summed out-degree of the roots
+    struct wg_members* members;      // batch of member graphs (wg_set_members), NULL if none
     int iter;                 // iterations run since init
 
     wg_prof prof;
@@ -274,3 +280,8 @@ int wg_fill_u64(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t value);
 int wg_to_device_id(wg_ctx* ctx, uint64_t orig, uint32_t* out);
 // copies V values stored in device order to host memory in original order
 int wg_read_vertex_array(wg_ctx* ctx, const void* dev, void* host, size_t elem_size);
+// edge tiles of at most tile_edges edges (and WG_TILE_VERTS destinations)
+void wg_build_tiles(const uint32_t* off, uint64_t V, uint32_t tile_edges, std::vector<uint32_t>& tv,
+                    std::vector<uint32_t>& te, std::vector<uint32_t>& hv, std::vector<uint32_t>& ht);
+// releases the member batch (with its HIP graph); called when the graph goes
+void wg_free_members(wg_ctx* ctx);

@@ -26,6 +26,9 @@
 // so the edges it reads follow the marked blocks; the word scan itself is
 // E / 2^b / 8 bytes.
 #include "wg_frontier.h"
+
+#include <algorithm>
+#include <vector>
 #include "wg_mlp_wave.h"
 
 // ---- counters -------------------------------------------------------------------
@@ -800,25 +803,74 @@ static int init_common(wg_ctx* ctx, int app)
     return WG_OK;
 }
 
+// Roots (device ids, distinct): BFS state depth 0 / parent = the root itself
+// (original id), SSSP distance 0; every root is set in bm_cur (BFS frontier;
+// for SSSP the set the first frontier build starts from) and, for BFS, in
+// bm_visited; the activity of the root level (summed out-degree) is counted.
+__global__ void k_init_roots(const uint32_t* __restrict__ roots, uint32_t n, int app, int wkind, void* __restrict__ prop,
+                             const uint32_t* __restrict__ perm, uint64_t* __restrict__ cur, uint64_t* __restrict__ visited,
+                             const uint32_t* __restrict__ od, wg_dev_counters* __restrict__ ctr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t r = roots[i];
+    const unsigned long long bit = 1ull << (r & 63);
+    atomicOr((unsigned long long*)&cur[r >> 6], bit);
+    if (app == WG_APP_BFS) {
+        atomicOr((unsigned long long*)&visited[r >> 6], bit);
+        ((uint64_t*)prop)[r] = perm ? perm[r] : r;
+    } else if (wkind == WG_WEIGHT_F64) {
+        ((uint64_t*)prop)[r] = 0ull;
+    } else {
+        ((uint32_t*)prop)[r] = 0u;
+    }
+    atomicAdd(&ctr->slots[0].v[WG_CNT_ACTIVITY], (unsigned long long)od[r]);
+}
+
+static int init_roots(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t n)
+{
+    if (!ctx || !roots || n == 0)
+        return WG_E_INVAL;
+    if (!ctx->off || (app == WG_APP_SSSP && ctx->wkind == WG_WEIGHT_NONE))
+        return WG_E_STATE;
+    std::vector<uint32_t> dev(n);
+    int rc;
+    for (uint32_t i = 0; i < n; ++i)
+        if ((rc = wg_to_device_id(ctx, roots[i], &dev[i])))
+            return rc;
+    std::sort(dev.begin(), dev.end());
+    dev.erase(std::unique(dev.begin(), dev.end()), dev.end());
+    if ((rc = init_common(ctx, app)))
+        return rc;
+    const size_t esz = (app == WG_APP_BFS || ctx->wkind == WG_WEIGHT_F64) ? 8 : 4;
+    WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * esz));
+    WG_HIP(ctx, hipMalloc((void**)&ctx->roots, dev.size() * sizeof(uint32_t)));
+    ctx->nroots = (uint32_t)dev.size();
+    ctx->root_dev = dev[0];
+    WG_HIP(ctx, hipMemcpyAsync(ctx->roots, dev.data(), dev.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    if (esz == 8)
+        rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, app == WG_APP_BFS ? ~0ull : (uint64_t)PolSF::INF);
+    else
+        rc = wg_fill_u32(ctx, (uint32_t*)ctx->prop[0], ctx->V, PolSU::INF);
+    if (rc || (rc = wg_reset_counters(ctx, RST_ACTIVITY)))
+        return rc;
+    hipLaunchKernelGGL(k_init_roots, dim3((ctx->nroots + 255) / 256), dim3(256), 0, ctx->stream, ctx->roots, ctx->nroots,
+                       app, ctx->wkind, ctx->prop[0], ctx->perm, ctx->bm_cur, ctx->bm_visited, ctx->od, ctx->ctr);
+    if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
+        return rc;
+    ctx->root_activity = ctx->hctr->activity;
+    return WG_OK;
+}
+
+extern "C" int wg_init_bfs_roots(wg_ctx* ctx, const uint64_t* roots, uint32_t n)
+{
+    return init_roots(ctx, WG_APP_BFS, roots, n);
+}
+
 extern "C" int wg_init_bfs(wg_ctx* ctx, uint64_t root)
 {
-    int rc = init_common(ctx, WG_APP_BFS);
-    if (rc)
-        return rc;
-    uint32_t r = 0;
-    if ((rc = wg_to_device_id(ctx, root, &r)))
-        return rc;
-    WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
-    if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, ~0ull)))
-        return rc;
-    ctx->root_dev = r;
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], (uint64_t)r, root);
-    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_cur, r);
-    hipLaunchKernelGGL(k_or_bit, dim3(1), dim3(1), 0, ctx->stream, ctx->bm_visited, r);
-    if ((rc = wg_check_launch(ctx)))
-        return rc;
-    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return WG_OK;
+    return init_roots(ctx, WG_APP_BFS, &root, 1);
 }
 
 extern "C" int wg_init_cc(wg_ctx* ctx)
@@ -835,32 +887,14 @@ extern "C" int wg_init_cc(wg_ctx* ctx)
     return WG_OK;
 }
 
+extern "C" int wg_init_sssp_roots(wg_ctx* ctx, const uint64_t* roots, uint32_t n)
+{
+    return init_roots(ctx, WG_APP_SSSP, roots, n);
+}
+
 extern "C" int wg_init_sssp(wg_ctx* ctx, uint64_t root)
 {
-    if (!ctx || !ctx->off || ctx->wkind == WG_WEIGHT_NONE)
-        return WG_E_STATE;
-    int rc = init_common(ctx, WG_APP_SSSP);
-    if (rc)
-        return rc;
-    uint32_t r = 0;
-    if ((rc = wg_to_device_id(ctx, root, &r)))
-        return rc;
-    ctx->root_dev = r;
-    if (ctx->wkind == WG_WEIGHT_F64) {
-        WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint64_t)));
-        if ((rc = wg_fill_u64(ctx, (uint64_t*)ctx->prop[0], ctx->V, PolSF::INF)))
-            return rc;
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, (uint64_t*)ctx->prop[0], (uint64_t)r, 0ull);
-    } else {
-        WG_HIP(ctx, hipMalloc(&ctx->prop[0], ctx->V * sizeof(uint32_t)));
-        if ((rc = wg_fill_u32(ctx, (uint32_t*)ctx->prop[0], ctx->V, PolSU::INF)))
-            return rc;
-        WG_HIP(ctx, hipMemsetAsync((uint32_t*)ctx->prop[0] + r, 0, sizeof(uint32_t), ctx->stream));
-    }
-    if ((rc = wg_check_launch(ctx)))
-        return rc;
-    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return WG_OK;
+    return init_roots(ctx, WG_APP_SSSP, &root, 1);
 }
 
 // Launch shapes.  Frontier pulls: one wavefront per 64 frontier words at the
@@ -1250,21 +1284,22 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
     if (ctx->wedge_log2 >= 0 && (rc = clear_wedge_state(ctx)))
         return rc;
     const unsigned long long threshold = (ctx->E * (unsigned long long)p->threshold_pct) / 100ull;
-    unsigned long long act0 = ctx->E;
-    if (ctx->app != WG_APP_CC) {
-        uint32_t od = 0;
-        WG_HIP(ctx, hipMemcpyAsync(&od, ctx->od + ctx->root_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        act0 = od;
-    }
+    // CC: source/execution_cc.c:95; BFS / SSSP: out-degree of the roots
+    const unsigned long long act0 = ctx->app == WG_APP_CC ? ctx->E : ctx->root_activity;
     hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->mlp_set ? ctx->mlp : nullptr,
                        (unsigned int)p->schedule, threshold, blog2, p->max_iterations, act0,
-                       ctx->app == WG_APP_CC ? (unsigned long long)ctx->V : 1ull, ctx->app == WG_APP_CC ? 1 : 0,
-                       ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - 1), (unsigned int)ctx->app,
+                       ctx->app == WG_APP_CC ? (unsigned long long)ctx->V : (unsigned long long)ctx->nroots,
+                       ctx->app == WG_APP_CC ? 1 : 0,
+                       ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - ctx->nroots), (unsigned int)ctx->app,
                        (unsigned long long)ctx->V, (unsigned long long)ctx->E);
-    if (ctx->app != WG_APP_CC) // the root's blocks, if the first pull is frontier-filtered
-        hipLaunchKernelGGL(k_wedge_single, dim3(16), dim3(256), 0, ctx->stream, ctx->root_dev, ctx->csri_off,
-                           ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+    if (ctx->app != WG_APP_CC) { // the roots' blocks, if the first pull is frontier-filtered
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr, RST_HEAVY);
+        hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_cur,
+                           ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->heavy_cap,
+                           ctx->ctr, WG_GATE_FRONTIER);
+        hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
+                           ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+    }
     if ((rc = wg_check_launch(ctx)))
         return rc;
     int slot = wg_prof_begin(ctx, WG_K_LOOP);
