@@ -6,14 +6,27 @@ Default workload: R-MAT scale 24, edge factor 16, seed 2 (a/b/c/d =
 FP64 edge weights attached for SSSP.  One step = one PageRank iteration
 (edge pull + vertex phase, FP64 contributions, damping 0.85) over the whole
 graph with the inputs resident in HBM; `value` is E / (time per iteration) in
-GTEPS.  The same run reports BFS and SSSP from the max-out-degree root
-(GTEPS = edges whose source was reached / run time), the HBM roofline of the
-PageRank kernels measured with HIP events on the engine's stream, and the CPU
-oracle (a C restatement of Wedge's pull engine) timed on a bounded sample on
-the host cores.  BFS and SSSP run under the plain threshold rule and under the
-learned schedule (the packaged model) on this graph and on a 4096x6144 road
-grid (integer weights, seed 3), with learned/plain speedups and an identity
-check of the outputs.
+GTEPS, and `roofline` prices the PageRank kernels' algorithmic bytes
+(4E + 24V per iteration) per launch, timed with HIP events on the engine's
+stream, against the 8 TB/s HBM peak.
+
+The same run reports:
+  apps      BFS and SSSP from the max-out-degree root under the plain
+            threshold rule and under the learned schedule (the packaged
+            model), on this graph and on a 4096x6144 road grid (integer
+            weights, seed 3): GTEPS, learned/plain speedup, identical outputs;
+  batched   64-source BFS and SSSP and 32-seed personalised PageRank on this
+            graph against one source at a time, and 256 uneven R-MAT graphs
+            (scale U{10..16}, edge factor U{8..32}, seed 4) as one batch
+            (PageRank in one captured HIP graph with the autotuned tile shape,
+            BFS from every member in one traversal) against graph by graph;
+  large     R-MAT scale 27, edge factor 16, seed 5 (2.1 billion edges):
+            PageRank with FP64 and FP32-stored contributions and BFS (plain
+            and learned), each with its HBM roofline fraction, and the L1
+            distance of the FP32 ranks from the FP64 ranks;
+  cpu_baseline  the oracle (a C restatement of Wedge's pull engine) timed on
+            a bounded PageRank sample on the host cores; its ranks also give
+            the L1 error of the HIP path's FP64 and FP32 ranks.
 
 Prints one JSON line.
 """
@@ -39,6 +52,7 @@ CONFIGS = {
     "s24": dict(scale=24, edge_factor=16, seed=2),
     "s20": dict(scale=20, edge_factor=16, seed=1),
     "s22": dict(scale=22, edge_factor=16, seed=2),
+    "s27": dict(scale=27, edge_factor=16, seed=5),
 }
 
 
@@ -98,6 +112,187 @@ def schedule_pair(ctx, g, root, reps, weights):
     return out
 
 
+def best_ms(fn, reps):
+    """Smallest wall time of fn() over reps runs (fn waits for the device)."""
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = fn()
+        dt = (time.perf_counter() - t0) * 1e3
+        if best is None or dt < best[0]:
+            best = (dt, res)
+    return best
+
+
+def batched_sources(ctx, g, pr_ms, damping, reps=3):
+    """64-source BFS and SSSP and 32-seed personalised PageRank on the loaded
+    graph, each against the same work done one source at a time: sources and
+    seeds drawn with seed 2 among the vertices with out-degree > 0."""
+    od = g.outdegrees()
+    rng = np.random.default_rng(2)
+    roots = rng.choice(np.nonzero(od > 0)[0], 64, replace=False).astype(np.uint64)
+    seeds = rng.choice(np.nonzero(od > 0)[0], 32, replace=False).astype(np.uint64)
+    out = {}
+    single = {s: float(np.mean([ctx.run_bfs(int(r), schedule=s)[1]["running_ms"] for r in roots[:16]]))
+              for s in ("plain", "learned")}
+    ms, (levels, reached) = best_ms(lambda: ctx.msbfs(roots), reps)
+    out["msbfs_64"] = {
+        "ms": round(ms, 3), "levels": levels, "gteps": round(reached / (ms * 1e-3) / 1e9, 2),
+        "single_ms_plain": round(single["plain"], 3), "single_ms_learned": round(single["learned"], 3),
+        "speedup_vs_64_plain": round(64 * single["plain"] / ms, 2),
+        "speedup_vs_64_learned": round(64 * single["learned"] / ms, 2),
+    }
+    log(f"MS-BFS 64: {out['msbfs_64']}")
+    iters = 16
+    ms, _ = best_ms(lambda: (ctx.ppr(seeds, iters, damping), ctx.sync()), reps)
+    out["ppr_32"] = {
+        "ms_per_iteration": round(ms / iters, 3), "iterations": iters,
+        "pr_iteration_ms": round(pr_ms, 3), "speedup_vs_32_pr": round(32 * pr_ms / (ms / iters), 2),
+        "gteps": round(32 * g.num_edges / (ms / iters * 1e-3) / 1e9, 2),
+    }
+    log(f"PPR 32: {out['ppr_32']}")
+    single_sssp = float(np.mean([ctx.run_sssp(int(r))[1]["running_ms"] for r in roots[:4]]))
+    ms, it = best_ms(lambda: ctx.mssssp(roots), 1)
+    out["mssssp_64"] = {"ms": round(ms, 1), "iterations": it, "single_ms": round(single_sssp, 2),
+                        "speedup_vs_64": round(64 * single_sssp / ms, 2)}
+    log(f"MS-SSSP 64: {out['mssssp_64']}")
+    return out
+
+
+def batched_graphs(damping, iterations=16, reps=5):
+    """256 uneven R-MAT graphs (scale and edge factor drawn per graph from
+    U{10..16} and U{8..32} with seed 4, graph i generated with seed 4000 + i)
+    as one disjoint union: PageRank on every member in one captured HIP graph
+    with the autotuned tile shape, and BFS from every member's max-out-degree
+    vertex in one traversal; against the same work run graph by graph."""
+    rng = np.random.default_rng(4)
+    scales = rng.integers(10, 17, 256)
+    efs = rng.integers(8, 33, 256)
+    members = [W.rmat(int(s), int(e), seed=4000 + i) for i, (s, e) in enumerate(zip(scales, efs))]
+    E = sum(m.num_edges for m in members)
+    V = sum(m.num_vertices for m in members)
+    locals_ = [m.max_outdegree_vertex() for m in members]
+    ctx = W.Context(0)
+    first = ctx.load_batch(members)
+    shape, times = ctx.batch_autotune(iterations, damping)
+    ctx.batch_pr(iterations, damping)
+    ctx.sync()
+    ms_graph, _ = best_ms(lambda: (ctx.batch_pr(iterations, damping), ctx.sync()), reps)
+    ms_plain, _ = best_ms(lambda: (ctx.batch_pr(iterations, damping, use_graph=False), ctx.sync()), reps)
+    roots = first[:-1] + np.array(locals_, np.uint64)
+    bfs_ms = None
+    for _ in range(reps):
+        ctx.init_bfs_roots(roots)
+        t0 = time.perf_counter()
+        ctx.converge(W.SCHED_PLAIN)
+        dt = (time.perf_counter() - t0) * 1e3
+        bfs_ms = dt if bfs_ms is None else min(bfs_ms, dt)
+    ctx.close()
+    # graph by graph: each member loaded on its own, initialisation not timed
+    one = W.Context(0)
+    seq_pr = seq_bfs = 0.0
+    for m, r in zip(members, locals_):
+        one.load(m)
+        one.init_pr(damping)
+        t0 = time.perf_counter()
+        for _ in range(iterations):
+            one._lib.wg_pr_iteration(one.handle, None)
+        one.sync()
+        seq_pr += (time.perf_counter() - t0) * 1e3
+        one.init_bfs(r)
+        t0 = time.perf_counter()
+        one.converge(W.SCHED_PLAIN)
+        seq_bfs += (time.perf_counter() - t0) * 1e3
+    one.close()
+    out = {
+        "graphs": len(members), "vertices": V, "edges": E, "iterations": iterations,
+        "autotuned_shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1]},
+        "autotune_ms": {f"{k[0]}x{k[1]}": round(v, 4) for k, v in times.items()},
+        "pr_ms_hipgraph": round(ms_graph, 3), "pr_ms_plain_launches": round(ms_plain, 3),
+        "pr_gteps": round(iterations * E / (ms_graph * 1e-3) / 1e9, 2),
+        "pr_ms_one_by_one": round(seq_pr, 2), "pr_speedup": round(seq_pr / ms_graph, 2),
+        "bfs_ms": round(bfs_ms, 3), "bfs_ms_one_by_one": round(seq_bfs, 2),
+        "bfs_speedup": round(seq_bfs / bfs_ms, 2),
+    }
+    log(f"256 graphs: {out}")
+    return out
+
+
+def bfs_algorithmic_bytes(V: int, st: dict) -> int:
+    """Bytes a BFS must move at minimum, Graph500-style: the source index of
+    every edge leaving a reached vertex (4 B), the depth/parent word of every
+    reached vertex (8 B), and per level one read of the current frontier
+    bitmap and one write of the next (2 * V/8 B)."""
+    return 4 * st["edges_reached"] + 8 * st["vertices_reached"] + st["iterations"] * 2 * ((V + 7) // 8)
+
+
+def pr_timed(ctx, damping, fp32, steps, warmup=2):
+    """Wall ms per fused PageRank iteration and the tile + span kernel ms per
+    launch (HIP events on the engine's stream)."""
+    ctx.init_pr(damping, fp32)
+    for _ in range(warmup):
+        ctx.pr_iteration()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx._lib.wg_pr_iteration(ctx.handle, None)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    n_tile, ms_tile = ctx.profile_get("pr_tile")
+    n_span, ms_span = ctx.profile_get("pr_span")
+    ctx.profile(False)
+    return wall, ms_tile / max(n_tile, 1) + ms_span / max(n_span, 1)
+
+
+def large_graph(damping, steps=5, iterations=16, reps=3):
+    """R-MAT scale 27, edge factor 16, seed 5 (2.1 billion edges resident in
+    HBM): PageRank with FP64 and with FP32-stored contributions (time per
+    iteration, roofline of the tile + span kernels, L1 distance of the FP32
+    ranks from the FP64 ranks after 16 iterations), and BFS from the
+    max-out-degree vertex under the plain and the learned schedule."""
+    t0 = time.time()
+    g = W.rmat(27, 16, seed=5)
+    V, E = g.num_vertices, g.num_edges
+    root = g.max_outdegree_vertex()
+    log(f"rmat s27 ef16 seed 5: V={V} E={E} ({time.time() - t0:.1f} s)")
+    ctx = W.Context(0)
+    ctx.load(g)
+    out = {"workload": "rmat-s27-ef16-seed5", "vertices": V, "edges": E, "root": root}
+    ranks = {}
+    for fp32 in (False, True):
+        wall, kern = pr_timed(ctx, damping, fp32, steps)
+        alg = pr_algorithmic_bytes(V, E, fp32)
+        key = "pr_fp32" if fp32 else "pr_fp64"
+        out[key] = {"ms_per_iteration": round(wall, 3), "gteps": round(E / (wall * 1e-3) / 1e9, 2),
+                    "kernel_ms": round(kern, 3), "algorithmic_bytes": alg,
+                    "roofline_frac": round(alg / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        ctx.init_pr(damping, fp32)
+        for _ in range(iterations):
+            ctx._lib.wg_pr_iteration(ctx.handle, None)
+        ranks[fp32] = ctx.read_ranks()
+        log(f"s27 {key}: {out[key]}")
+    out["pr_fp32_l1_vs_fp64"] = float(np.abs(ranks[True] - ranks[False]).sum())
+    out["pr_iterations_compared"] = iterations
+    del ranks
+    W.mlp_load()
+    res = {}
+    for sched in ("plain", "learned"):
+        r, st = run_best(ctx, "bfs", root, reps, schedule=sched)
+        res[sched] = r
+        alg = bfs_algorithmic_bytes(V, st)
+        out[f"bfs_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
+                               "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 2),
+                               "algorithmic_bytes": alg,
+                               "roofline_frac": round(alg / (st["running_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
+    out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
+    out["bfs_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
+    ctx.close()
+    return out
+
+
 def cpu_baseline(g, damping, budget_s=20.0):
     """The oracle's PageRank on the same graph, all host threads, a bounded
     number of iterations (time per iteration -> GTEPS)."""
@@ -110,10 +305,20 @@ def cpu_baseline(g, damping, budget_s=20.0):
     _, st = orc.pr(1, damping)
     per_iter = max(st["running_ms"], 1e-3) / 1e3
     iters = int(max(1, min(8, (budget_s - build_s) / per_iter)))
-    _, st = orc.pr(iters, damping)
+    ref, st = orc.pr(iters, damping)
     ms = st["running_ms"] / iters
     threads = orc.threads
     orc.close()
+    # the oracle's ranks also check the HIP path's ranks after the same iterations
+    errs = {}
+    ctx = W.Context(0)
+    ctx.load(g)
+    for fp32 in (False, True):
+        ctx.init_pr(damping, fp32)
+        for _ in range(iters):
+            ctx._lib.wg_pr_iteration(ctx.handle, None)
+        errs["fp32" if fp32 else "fp64"] = float(np.abs(ctx.read_ranks() - ref).sum())
+    ctx.close()
     return {
         "value": round(g.num_edges / (ms * 1e-3) / 1e9, 4),
         "unit": "GTEPS",
@@ -121,6 +326,8 @@ def cpu_baseline(g, damping, budget_s=20.0):
         "kind": "port",
         "sample": f"oracle PageRank, {iters} iteration(s) on the same graph, {threads} threads, "
                   f"{ms:.1f} ms/iteration (layout build {build_s:.1f} s not timed)",
+        "pr_l1_hip_vs_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
+        "pr_iterations_compared": iters,
     }
 
 
@@ -136,6 +343,9 @@ def main():
     ap.add_argument("--no-apps", action="store_true", help="skip BFS/SSSP")
     ap.add_argument("--bfs-reps", type=int, default=3)
     ap.add_argument("--no-grid", action="store_true", help="skip the 4096x6144 road grid (plain vs learned)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sources and graphs")
+    ap.add_argument("--no-batch-graphs", action="store_true", help="skip the 256-graph batch")
+    ap.add_argument("--no-s27", action="store_true", help="skip the 2.1-billion-edge R-MAT (scale 27)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -221,6 +431,8 @@ def main():
         W.mlp_load()
         root = g.max_outdegree_vertex()
         out["apps"] = {"rmat": schedule_pair(ctx, g, root, args.bfs_reps, "f64")}
+        if not args.no_batch:
+            out["batched"] = batched_sources(ctx, g, ms_per_step, args.damping)
         if not args.no_grid:
             t0 = time.time()
             road = W.grid(4096, 6144, seed=3)
@@ -228,6 +440,10 @@ def main():
             log(f"road grid 4096x6144 seed 3: V={road.num_vertices} E={road.num_edges} ({time.time() - t0:.1f} s)")
             out["apps"]["road_grid_4096x6144"] = schedule_pair(ctx, road, road.max_outdegree_vertex(), 2, "u32")
     ctx.close()
+    if not args.no_batch and not args.no_batch_graphs:
+        out.setdefault("batched", {})["graphs_256"] = batched_graphs(args.damping)
+    if not args.no_s27:
+        out["large"] = large_graph(args.damping)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, args.damping)

@@ -252,7 +252,8 @@ int wg_read_ppr(wg_ctx* ctx, uint32_t seed, double* host_ranks);
  * A batch of small graphs is loaded as one graph, their disjoint union:
  * member g owns the vertices [first[g], first[g+1]) and every edge stays
  * inside its member.  Load the union with WG_OPT_RELABEL = WG_RELABEL_OFF so
- * that the members keep their vertex ranges.  Then
+ * that the members keep their vertex ranges (a single member, n = 1, may be
+ * relabelled).  Then
  *  - BFS / SSSP from one root per member (wg_init_bfs_roots /
  *    wg_init_sssp_roots + wg_converge) and CC (wg_init_cc + wg_converge) run
  *    every member in one traversal, with results equal to per-member runs
@@ -263,18 +264,21 @@ int wg_read_ppr(wg_ctx* ctx, uint32_t seed, double* host_ranks);
 int wg_set_members(wg_ctx* ctx, uint32_t n, const uint64_t* first /* n + 1 entries */);
 
 /* A launch shape of the batched PageRank: edges per lane of a tile (a tile
- * holds up to 64 * edges_per_lane edges; 4, 8 or 16) and workgroups per
- * compute unit of the tile kernel (1..16). */
+ * holds up to 64 * edges_per_lane edges; 4, 8 or 16), workgroups per compute
+ * unit of the tile kernel (1..16), and xcd_split: 1 = each of the 8 XCDs
+ * walks its own contiguous eighth of the edges (so a member's contributions
+ * stay in one XCD's L2), 0 = all workgroups stride over all tiles. */
 typedef struct wg_batch_shape {
     uint32_t edges_per_lane;
     uint32_t blocks_per_cu;
+    uint32_t xcd_split;
 } wg_batch_shape;
-#define WG_BATCH_NSHAPES 12
+#define WG_BATCH_NSHAPES 24
 /* The autotuner's candidate number i (0 <= i < WG_BATCH_NSHAPES). */
 int wg_batch_shape_at(uint32_t i, wg_batch_shape* out);
 
 /* `iterations` PageRank iterations on every member from the uniform start.
- * shape: NULL = the tuned shape (wg_batch_autotune) or {8, 8}.  use_graph: 1 =
+ * shape: NULL = the tuned shape (wg_batch_autotune) or {8, 8, 0}.  use_graph: 1 =
  * all iterations are captured once into a HIP graph and replayed as one
  * launch (the graph is kept and reused while iterations, damping, precision
  * and shape stay the same); 0 = plain launches.  Both give bit-identical

@@ -62,7 +62,7 @@ def test_batch_pr_matches_member_oracle(ctx, fp32):
     first = ctx.load_batch(ms)
     refs = [oracle.Oracle.of(m).pr(16, 0.85)[0] for m in ms]
     tol = PR_TOL_FP32 if fp32 else PR_TOL
-    for shape in (None, (4, 2), (8, 8), (16, 16), (16, 1)):
+    for shape in (None, (4, 2), (8, 8), (16, 16), (16, 1), (4, 4, 1), (8, 16, 1), (16, 2, 1)):
         for use_graph in (False, True):
             ctx.batch_pr(16, 0.85, fp32, shape, use_graph)
             r = ctx.read_batch_ranks()
@@ -84,6 +84,8 @@ def test_batch_pr_graph_replays_equal_plain_launches(ctx):
         ctx.batch_pr(16, 0.85, False, (8, 4), use_graph=True)  # the kept graph after the first
         got = ctx.read_batch_ranks()
         assert np.array_equal(got, plain), f"replay {rep}"
+        ctx.batch_pr(16, 0.85, False, (8, 4, 1), use_graph=True)  # the XCD split sums in the same order
+        assert np.array_equal(ctx.read_batch_ranks(), plain), f"split {rep}"
         # other work between replays: plain launches of another run and a traversal
         ctx.batch_pr(5, 0.5, True, (16, 2), use_graph=False)
         ctx.sync()
@@ -220,7 +222,7 @@ def test_member_errors(ctx):
     with pytest.raises(W.WedgeError) as e:
         ctx.read_batch_ranks()  # nothing ran
     assert e.value.code == W.WG_E_STATE
-    for shape in ((5, 4), (8, 0), (8, 17)):
+    for shape in ((5, 4), (8, 0), (8, 17), (8, 4, 2)):
         with pytest.raises(W.WedgeError) as e:
             ctx.batch_pr(4, 0.85, False, shape)
         assert e.value.code == W.WG_E_INVAL

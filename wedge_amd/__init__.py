@@ -164,13 +164,13 @@ GPU_SYMBOLS = [
 
 
 class BatchShape(C.Structure):
-    _fields_ = [("edges_per_lane", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+    _fields_ = [("edges_per_lane", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("xcd_split", C.c_uint32)]
 
     def as_tuple(self):
-        return int(self.edges_per_lane), int(self.blocks_per_cu)
+        return int(self.edges_per_lane), int(self.blocks_per_cu), int(self.xcd_split)
 
 
-BATCH_NSHAPES = 12
+BATCH_NSHAPES = 24
 
 
 class RunOpts(C.Structure):
@@ -554,7 +554,8 @@ class Context:
         self.members = f
 
     def batch_pr(self, iterations=16, damping=0.85, contrib_fp32=False, shape=None, use_graph=True):
-        """PageRank on every member (queued; read_batch_ranks waits)."""
+        """PageRank on every member (queued; read_batch_ranks waits).  shape:
+        (edges_per_lane, blocks_per_cu[, xcd_split]) or None for the tuned one."""
         s = None if shape is None else C.byref(BatchShape(*shape))
         self._chk(self._lib.wg_batch_pr(self._h, iterations, damping, int(contrib_fp32), s, int(use_graph)),
                   "wg_batch_pr")

@@ -32,13 +32,14 @@ static const uint32_t k_shape_ipt[] = {4, 8, 16};
 static const uint32_t k_shape_bpc[] = {2, 4, 8, 16};
 static constexpr int k_n_ipt = 3;
 static constexpr int k_n_bpc = 4;
-static_assert(k_n_ipt * k_n_bpc == WG_BATCH_NSHAPES, "candidate list");
+static_assert(k_n_ipt * k_n_bpc * 2 == WG_BATCH_NSHAPES, "candidate list");
 
 namespace {
 
 struct shape_tiles {
     uint32_t ntiles = 0, nheavy = 0;
     uint32_t *tile_v = nullptr, *tile_e = nullptr, *heavy_v = nullptr, *heavy_t = nullptr;
+    wg_tile_split split = {}; // edge-balanced XCD ranges of the tiles (split.on unset)
 };
 
 } // namespace
@@ -61,14 +62,14 @@ struct wg_members {
     int c_fp32 = -1;              // precision c[] was allocated for
     int cur = -1;                 // buffer holding the last run's contributions
     int tuned = 0;
-    wg_batch_shape best = {8, 8};
+    wg_batch_shape best = {8, 8, 0};
     // the captured run
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t g_iters = 0;
     double g_damping = 0.0;
     int g_fp32 = -1;
-    wg_batch_shape g_shape = {0, 0};
+    wg_batch_shape g_shape = {0, 0, 0};
 };
 
 static void free_dev_ptr(void* p)
@@ -276,12 +277,17 @@ static int build_members(wg_ctx* ctx, wg_members* m, const uint64_t* first, uint
         shape_tiles& t = m->tiles[i];
         t.ntiles = (uint32_t)(tv.size() - 1);
         t.nheavy = (uint32_t)hv.size();
+        for (int x = 0; x <= WG_NXCD; ++x) { // first tile starting at or after x/8 of the edges
+            const uint64_t target = ctx->E * (uint64_t)x / WG_NXCD;
+            t.split.start[x] = (x == WG_NXCD) ? t.ntiles
+                                              : (uint32_t)(std::lower_bound(te.begin(), te.end() - 1, (uint32_t)target) - te.begin());
+        }
         max_tiles = std::max(max_tiles, t.ntiles);
         if ((rc = upload_vec(ctx, &t.tile_v, tv)) || (rc = upload_vec(ctx, &t.tile_e, te)) ||
             (rc = upload_vec(ctx, &t.heavy_v, hv)) || (rc = upload_vec(ctx, &t.heavy_t, ht)))
             return rc;
     }
-    const size_t nblk = (size_t)ctx->num_cu * 16 + (size_t)ctx->num_cu * 4 + 16;
+    const size_t nblk = (size_t)ctx->num_cu * 16 + WG_NXCD + (size_t)ctx->num_cu * 4 + 16;
     WG_HIP(ctx, hipMalloc((void**)&m->tile_part, (max_tiles + 1) * sizeof(double)));
     WG_HIP(ctx, hipMalloc((void**)&m->blk_part, nblk * sizeof(double)));
     WG_HIP(ctx, hipMalloc((void**)&m->chunk_part, (m->nchunks + 1) * sizeof(double)));
@@ -297,7 +303,7 @@ extern "C" int wg_set_members(wg_ctx* ctx, uint32_t n, const uint64_t* first)
         return WG_E_INVAL;
     if (!ctx->off)
         return WG_E_STATE;
-    if (ctx->perm) // relabelling interleaves the members
+    if (ctx->perm && n > 1) // relabelling interleaves the members
         return WG_E_STATE;
     if (first[0] != 0 || first[n] != ctx->V)
         return WG_E_INVAL;
@@ -320,8 +326,9 @@ extern "C" int wg_batch_shape_at(uint32_t i, wg_batch_shape* out)
 {
     if (i >= WG_BATCH_NSHAPES || !out)
         return WG_E_INVAL;
-    out->edges_per_lane = k_shape_ipt[i / k_n_bpc];
-    out->blocks_per_cu = k_shape_bpc[i % k_n_bpc];
+    out->edges_per_lane = k_shape_ipt[(i / 2) / k_n_bpc];
+    out->blocks_per_cu = k_shape_bpc[(i / 2) % k_n_bpc];
+    out->xcd_split = i & 1;
     return WG_OK;
 }
 
@@ -335,17 +342,22 @@ static int shape_index(const wg_batch_shape& s)
 
 static bool same_shape(const wg_batch_shape& a, const wg_batch_shape& b)
 {
-    return a.edges_per_lane == b.edges_per_lane && a.blocks_per_cu == b.blocks_per_cu;
+    return a.edges_per_lane == b.edges_per_lane && a.blocks_per_cu == b.blocks_per_cu && a.xcd_split == b.xcd_split;
 }
 
 // Queues the whole run: init, then `iters` times edge pull, chunk sums and
 // vertex phase.  Only kernel launches (no copies, fills or waits), so the same
 // sequence is captured into the HIP graph.
 template <typename T, int IPT>
-static int queue_run(wg_ctx* ctx, wg_members* m, const shape_tiles& t, uint32_t bpc, uint32_t iters, double damping)
+static int queue_run(wg_ctx* ctx, wg_members* m, const shape_tiles& t, uint32_t bpc, uint32_t xcd, uint32_t iters,
+                     double damping)
 {
     const uint32_t cgrid = wg_grid(ctx, m->nchunks, 1, 16);
-    const uint32_t grid = wg_grid(ctx, t.ntiles, WG_WPB, bpc);
+    uint32_t grid = wg_grid(ctx, t.ntiles, WG_WPB, bpc);
+    wg_tile_split split = t.split;
+    split.on = xcd;
+    if (xcd)
+        grid = (grid + WG_NXCD - 1) / WG_NXCD * WG_NXCD;
     const uint32_t sgrid = wg_grid(ctx, t.nheavy, 4, 4);
     T* c[2] = {(T*)m->c[0], (T*)m->c[1]};
     hipLaunchKernelGGL(k_bm_init<T>, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->od, c[0], m->chunk_v, m->chunk_m,
@@ -355,7 +367,7 @@ static int queue_run(wg_ctx* ctx, wg_members* m, const shape_tiles& t, uint32_t 
         T* cout = c[(it & 1) ^ 1];
         hipLaunchKernelGGL((k_pr_tile<T, 1, IPT>), dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, cin,
                            (T*)nullptr, m->acc, ctx->od, t.tile_v, t.tile_e, t.ntiles, m->scratch, damping, 1.0,
-                           m->tile_part, m->blk_part);
+                           m->tile_part, m->blk_part, split);
         hipLaunchKernelGGL((k_pr_span<T, 1, IPT>), dim3(sgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, t.heavy_v,
                            t.heavy_t, t.nheavy, m->tile_part, (T*)nullptr, m->acc, ctx->od, m->scratch, damping, 1.0,
                            m->blk_part, grid, m->scratch + 1, &ctx->ctr->ticket);
@@ -373,11 +385,11 @@ static int queue_run_t(wg_ctx* ctx, wg_members* m, const wg_batch_shape& s, uint
     const int i = shape_index(s);
     switch (s.edges_per_lane) {
     case 4:
-        return queue_run<T, 4>(ctx, m, m->tiles[i], s.blocks_per_cu, iters, damping);
+        return queue_run<T, 4>(ctx, m, m->tiles[i], s.blocks_per_cu, s.xcd_split, iters, damping);
     case 8:
-        return queue_run<T, 8>(ctx, m, m->tiles[i], s.blocks_per_cu, iters, damping);
+        return queue_run<T, 8>(ctx, m, m->tiles[i], s.blocks_per_cu, s.xcd_split, iters, damping);
     case 16:
-        return queue_run<T, 16>(ctx, m, m->tiles[i], s.blocks_per_cu, iters, damping);
+        return queue_run<T, 16>(ctx, m, m->tiles[i], s.blocks_per_cu, s.xcd_split, iters, damping);
     default:
         return WG_E_INVAL;
     }
@@ -443,7 +455,7 @@ static int capture(wg_ctx* ctx, wg_members* m, const wg_batch_shape& s, uint32_t
 
 static bool valid_shape(const wg_batch_shape& s)
 {
-    return shape_index(s) >= 0 && s.blocks_per_cu >= 1 && s.blocks_per_cu <= 16;
+    return shape_index(s) >= 0 && s.blocks_per_cu >= 1 && s.blocks_per_cu <= 16 && s.xcd_split <= 1;
 }
 
 static int launch(wg_ctx* ctx, wg_members* m, const wg_batch_shape& s, uint32_t iters, double damping, int fp32,

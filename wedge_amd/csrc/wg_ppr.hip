@@ -164,18 +164,55 @@ __global__ __launch_bounds__(WG_BLOCK) void k_ppr_span(const uint32_t* __restric
 }
 
 // D_next[s] = the per-wave dangling partials in wave order.
-__global__ void k_ppr_reduce(const double* __restrict__ wpart, uint32_t nw, const double* __restrict__ spart, uint32_t ns,
-                             double* __restrict__ Dn)
+// Dangling mass per seed: the n partial rows p[n][32] (wave partials, then
+// span partials) summed per lane in a fixed order.  Block b takes the rows
+// [b n / B, (b + 1) n / B); its thread t sums lane t % 32 over the rows of
+// group t / 32 (every 8th row, four running sums), the 8 group sums are added
+// in group order, and the last block to finish adds the B block sums in block
+// order.
+#define PPR_RED_BLOCKS 64
+__global__ __launch_bounds__(256) void k_ppr_reduce(const double* __restrict__ p, uint32_t n, double* __restrict__ bpart,
+                                                    unsigned int* __restrict__ ticket, double* __restrict__ Dn)
 {
-    const uint32_t s = threadIdx.x;
-    if (s >= PPR_LANES)
-        return;
-    double a = 0.0;
-    for (uint32_t w = 0; w < nw; ++w)
-        a += wpart[(uint64_t)w * PPR_LANES + s];
-    for (uint32_t w = 0; w < ns; ++w)
-        a += spart[(uint64_t)w * PPR_LANES + s];
-    Dn[s] = a;
+    __shared__ double s_grp[8][PPR_LANES];
+    __shared__ bool s_last;
+    const uint32_t lane = threadIdx.x & (PPR_LANES - 1), grp = threadIdx.x / PPR_LANES;
+    const uint32_t r0 = (uint32_t)((uint64_t)n * blockIdx.x / gridDim.x);
+    const uint32_t r1 = (uint32_t)((uint64_t)n * (blockIdx.x + 1) / gridDim.x);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    uint32_t r = r0 + grp;
+    for (; r + 24 < r1; r += 32) {
+        a0 += __builtin_nontemporal_load(&p[(uint64_t)r * PPR_LANES + lane]);
+        a1 += __builtin_nontemporal_load(&p[(uint64_t)(r + 8) * PPR_LANES + lane]);
+        a2 += __builtin_nontemporal_load(&p[(uint64_t)(r + 16) * PPR_LANES + lane]);
+        a3 += __builtin_nontemporal_load(&p[(uint64_t)(r + 24) * PPR_LANES + lane]);
+    }
+    for (; r < r1; r += 8)
+        a0 += __builtin_nontemporal_load(&p[(uint64_t)r * PPR_LANES + lane]);
+    s_grp[grp][lane] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (threadIdx.x < PPR_LANES) {
+        double b = 0.0;
+        for (int g = 0; g < 8; ++g)
+            b += s_grp[g][lane];
+        bpart[(uint64_t)blockIdx.x * PPR_LANES + lane] = b;
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        s_last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        if (threadIdx.x < PPR_LANES) {
+            double d = 0.0;
+            for (uint32_t b = 0; b < gridDim.x; ++b)
+                d += __builtin_nontemporal_load(&bpart[(uint64_t)b * PPR_LANES + lane]);
+            Dn[lane] = d;
+        }
+        if (threadIdx.x == 0)
+            *ticket = 0u;
+    }
 }
 
 __global__ void k_ppr_column(const double* __restrict__ R, uint32_t s, uint64_t V, double* __restrict__ out)
@@ -204,12 +241,13 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
     const uint32_t grid = ctx->pr_grid, sgrid = ctx->span_grid;
     const uint32_t nw = grid * WG_WPB, ns = sgrid * WG_WPB;
     void *C0 = nullptr, *C1 = nullptr, *R = nullptr, *aux = nullptr;
-    // aux: seeds (32 u32) | D[2][32] | tile partials [ntiles][32] | wave partials | span partials
-    const size_t aux_doubles = 2 * PPR_LANES + ((size_t)ctx->ntiles + nw + ns) * PPR_LANES;
+    // aux: seeds (32 u32) | D[2][32] | tile partials [ntiles][32] | wave partials | span partials |
+    //      reduction block partials [PPR_RED_BLOCKS][32] | reduction ticket (u32)
+    const size_t aux_doubles = 2 * PPR_LANES + ((size_t)ctx->ntiles + nw + ns + PPR_RED_BLOCKS) * PPR_LANES;
     if (hipMalloc(&C0, V * PPR_LANES * sizeof(float)) != hipSuccess ||
         hipMalloc(&C1, V * PPR_LANES * sizeof(float)) != hipSuccess ||
         hipMalloc(&R, V * PPR_LANES * sizeof(double)) != hipSuccess ||
-        hipMalloc(&aux, PPR_LANES * sizeof(uint32_t) + aux_doubles * sizeof(double)) != hipSuccess) {
+        hipMalloc(&aux, PPR_LANES * sizeof(uint32_t) + aux_doubles * sizeof(double) + sizeof(unsigned int)) != hipSuccess) {
         (void)hipFree(C0);
         (void)hipFree(C1);
         (void)hipFree(R);
@@ -227,6 +265,9 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
     double* part = D + 2 * PPR_LANES;
     double* wpart = part + (size_t)ctx->ntiles * PPR_LANES;
     double* spart = wpart + (size_t)nw * PPR_LANES;
+    double* rpart = spart + (size_t)ns * PPR_LANES;
+    unsigned int* ticket = (unsigned int*)(rpart + (size_t)PPR_RED_BLOCKS * PPR_LANES);
+    WG_HIP(ctx, hipMemsetAsync(ticket, 0, sizeof(unsigned int), ctx->stream));
     WG_HIP(ctx, hipMemcpyAsync(dseeds, hs.data(), nseeds * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_ppr_init, dim3(wg_grid(ctx, V * PPR_LANES, 1024, 8)), dim3(256), 0, ctx->stream, dseeds, nseeds,
                        ctx->od, (float*)C0, D, V);
@@ -243,7 +284,8 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
                            ctx->tile_e, ctx->ntiles, dseeds, nseeds, cur, nxt, Rout, part, Dcur, wpart, damping);
         hipLaunchKernelGGL(k_ppr_span, dim3(sgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->heavy_v, ctx->heavy_t,
                            ctx->nheavy, ctx->od, dseeds, nseeds, part, nxt, Rout, Dcur, spart, damping);
-        hipLaunchKernelGGL(k_ppr_reduce, dim3(1), dim3(64), 0, ctx->stream, wpart, nw, spart, ns, Dnxt);
+        hipLaunchKernelGGL(k_ppr_reduce, dim3(PPR_RED_BLOCKS), dim3(256), 0, ctx->stream, wpart, nw + ns, rpart, ticket,
+                           Dnxt);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         float* t = cur;

@@ -12,6 +12,19 @@
 
 #include "wg_internal.h"
 
+// XCD-aware tile order.  Workgroups are dispatched to the 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  With the
+// split on, XCD x walks only the tiles [start[x], start[x + 1]) (ranges of
+// about E/8 edges each), so the sources its tiles gather from stay within a
+// few neighbouring destinations' in-neighbourhoods -- for a batch of small
+// graphs, within the members that XCD is working through, whose
+// contributions fit its L2.  Off: all workgroups stride over all tiles.
+#define WG_NXCD 8
+struct wg_tile_split {
+    uint32_t on;
+    uint32_t start[WG_NXCD + 1];
+};
+
 // Deterministic block-wide sum (xor butterfly inside each wave, then the
 // four wave sums in wave order).  All threads receive the result.
 static __device__ __forceinline__ double block_sum_det(double v, double* s_red)
@@ -49,7 +62,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
                                                        const uint32_t* __restrict__ tile_v,
                                                        const uint32_t* __restrict__ tile_e, uint32_t ntiles,
                                                        const double* __restrict__ S_in, double damping, double N,
-                                                       double* __restrict__ tile_part, double* __restrict__ blk_part)
+                                                       double* __restrict__ tile_part, double* __restrict__ blk_part,
+                                                       wg_tile_split split = wg_tile_split{})
 {
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
     constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
@@ -72,8 +86,14 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
     }
     double s_part = 0.0;
 
-    const uint32_t stride = gridDim.x * WG_WPB;
-    for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
+    uint32_t t = blockIdx.x * WG_WPB + wave, tend = ntiles, stride = gridDim.x * WG_WPB;
+    if (split.on) { // gridDim.x is a multiple of WG_NXCD
+        const uint32_t x = blockIdx.x % WG_NXCD;
+        t = split.start[x] + (blockIdx.x / WG_NXCD) * WG_WPB + wave;
+        tend = split.start[x + 1];
+        stride = (gridDim.x / WG_NXCD) * WG_WPB;
+    }
+    for (; t < tend; t += stride) {
         const uint32_t tv = tile_v[t];
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
         const uint32_t v0 = tv & ~WG_HEAVY_BIT;
