@@ -68,6 +68,12 @@ int wg_graph_write(const wg_graph* g, const char* path);
 
 void wg_graph_free(wg_graph* g);
 
+/* Disjoint union of n member graphs (all of one weight kind): member g's
+ * vertex v becomes first[g] + v.  first receives n + 1 entries.  The union is
+ * what wg_set_members expects (include/wedge_gpu.h).  WG_E_RANGE if the union
+ * needs more than 32-bit vertex or edge indices. */
+int wg_graph_union(const wg_graph* members, uint32_t n, wg_graph* out, uint64_t* first);
+
 /* Out-degree of every vertex (out must hold num_vertices entries). */
 int wg_graph_outdegrees(const wg_graph* g, uint32_t* out);
 

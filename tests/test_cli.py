@@ -24,6 +24,11 @@ def run(*args):
     (["-a", "bfs", "-i", "g.bin", "-N"], 3),             # missing value
     (["-a", "bfs", "-i", "g.bin", "-f", "yes"], 4),      # extraneous value
     (["-a", "bfs", "-i", "g.bin", "-g", "rmat:8:8:1"], 6),  # -i and -g together
+    (["-a", "bfs", "-i", "g.bin", "-R", "1,,2"], 2),     # root lists
+    (["-a", "bfs", "-i", "g.bin", "-R", "x"], 2),
+    (["-a", "bfs", "-i", "g.bin", "-R", "3,"], 2),
+    (["-a", "bfs", "-i", "g.bin", "-R", ",".join(["1"] * 65)], 2),
+    (["-a", "bfs", "-i", "g.bin", "-R"], 3),
 ])
 def test_cli_error_codes(args, code):
     assert run(*args).returncode == code
@@ -42,7 +47,8 @@ def test_cli_unknown_application():
 
 
 def test_cli_application_aliases_are_accepted():
-    for name in ("BFS", "breadth_first_search", "PageRank", "connected_components", "SingleSourceShortestPath"):
+    for name in ("BFS", "breadth_first_search", "PageRank", "connected_components", "SingleSourceShortestPath",
+                 "ppr", "personalized_pagerank"):
         r = run("-a", name, "-g", "rmat:6:4:1", "-d", "99")  # device 99 never exists: stops after ingress
         assert "unable to run" not in r.stdout
 
@@ -52,3 +58,13 @@ def test_cli_bad_input_file(tmp_path):
     p.write_bytes(b"\x01\x00\x00\x00\x00\x00\x00\x00")
     r = run("-a", "pr", "-i", str(p))
     assert r.returncode != 0 and "ERROR" in r.stdout
+
+
+def test_cli_ppr_needs_seeds():
+    r = run("-a", "ppr", "-g", "rmat:6:4:1")
+    assert r.returncode != 0 and "needs seeds" in r.stdout
+
+
+def test_cli_batch_reports_member_errors():
+    r = run("-a", "bfs", "-g", "rmat:6:4:1,bogus:1", "-d", "99")
+    assert r.returncode != 0 and "batch of 2 graphs" in r.stdout and "ERROR" in r.stdout

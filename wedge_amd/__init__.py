@@ -222,6 +222,7 @@ HOST_SYMBOLS = [
     ("wg_graph_read", C.c_int, [C.POINTER(GraphStruct), C.c_char_p, C.c_int]),
     ("wg_graph_write", C.c_int, [C.POINTER(GraphStruct), C.c_char_p]),
     ("wg_graph_free", None, [C.POINTER(GraphStruct)]),
+    ("wg_graph_union", C.c_int, [C.POINTER(GraphStruct), C.c_uint32, C.POINTER(GraphStruct), _u64p]),
     ("wg_graph_outdegrees", C.c_int, [C.POINTER(GraphStruct), _u32p]),
     ("wg_graph_max_outdegree_vertex", C.c_uint64, [C.POINTER(GraphStruct)]),
     ("wg_weight_f64", C.c_double, [C.c_uint64, C.c_uint64, C.c_uint64]),
@@ -389,32 +390,21 @@ def from_edges(num_vertices: int, src, dst, weights=None, dedup=False) -> Graph:
 
 
 def union(graphs) -> tuple[Graph, np.ndarray]:
-    """Disjoint union of member graphs: member g's vertex v becomes first[g] + v.
-    Returns the union and first (uint64[n + 1]).  All members must carry the
-    same weight kind."""
+    """Disjoint union of member graphs (wg_graph_union): member g's vertex v
+    becomes first[g] + v.  Returns the union and first (uint64[n + 1]).  All
+    members must carry the same weight kind."""
     graphs = list(graphs)
     if not graphs:
         raise ValueError("union of no graphs")
-    kinds = {g.weight_kind for g in graphs}
-    if len(kinds) != 1:
+    if len({g.weight_kind for g in graphs}) != 1:
         raise ValueError("members differ in weight kind")
-    first = np.zeros(len(graphs) + 1, np.uint64)
-    np.cumsum([g.num_vertices for g in graphs], out=first[1:])
-    ecount = np.zeros(len(graphs) + 1, np.uint64)
-    np.cumsum([g.num_edges for g in graphs], out=ecount[1:])
-    if first[-1] >= 2**32 or ecount[-1] >= 2**32:
-        raise ValueError("union exceeds 32-bit vertex or edge indices")
-    off = np.empty(int(first[-1]) + 1, np.uint32)
-    src = np.empty(int(ecount[-1]), np.uint32)
+    arr = (GraphStruct * len(graphs))()
     for i, g in enumerate(graphs):
-        v0, e0 = int(first[i]), int(ecount[i])
-        off[v0:v0 + g.num_vertices] = g.offsets[:-1].astype(np.uint64) + np.uint64(e0)
-        src[e0:e0 + g.num_edges] = g.sources.astype(np.uint64) + np.uint64(v0)
-    off[-1] = np.uint32(ecount[-1])
-    w = None
-    if graphs[0].weights is not None:
-        w = np.concatenate([g.weights for g in graphs])
-    return Graph(off, src, w), first
+        arr[i] = g._struct()  # keeps g's arrays contiguous and referenced by g
+    first = np.zeros(len(graphs) + 1, np.uint64)
+    st = GraphStruct()
+    _check(_lib_host().wg_graph_union(arr, len(graphs), C.byref(st), _ptr(first, C.c_uint64)), "wg_graph_union")
+    return _adopt(st), first
 
 
 def batch_shape_at(i: int) -> tuple[int, int]:

@@ -20,12 +20,12 @@ static cmdline_opts_t opts;
 
 static int is_recognized(char c)
 {
-    return strchr("ahinNortuVdsbDfgWm", c) != NULL;
+    return strchr("ahinNortuVdsbDfgWmR", c) != NULL;
 }
 
 static int requires_value(char c)
 {
-    return strchr("ainNortudsbDgWm", c) != NULL;
+    return strchr("ainNortudsbDgWmR", c) != NULL;
 }
 
 static void die_unknown(const char* argv0, const char* opt)
@@ -97,6 +97,10 @@ static void usage(const char* argv0)
     printf("  -g generator          rmat:SCALE:EF:SEED[:sym][:w] | grid:ROWS:COLS:SEED instead of -i\n");
     printf("  -W int|f64            SSSP input: third word is an integer (default) or an FP64 weight\n");
     printf("  -m model-file         schedule model for -s learned\n");
+    printf("  -R v[,v...]           BFS/SSSP from up to %d roots in one traversal; seeds of -a ppr (up to 32)\n",
+           CMDLINE_MAX_ROOTS);
+    printf("  -a ppr                personalised PageRank from the -R seeds (-N iterations, -D damping)\n");
+    printf("  -i a,b,... / -g a,b,...  a batch of graphs run together (BFS/SSSP from vertex -r of each)\n");
     exit(0);
 }
 
@@ -202,6 +206,25 @@ static void parse_one(const char* argv0, const char* opt, const char* val)
     case 'm':
         strncpy(opts.model, val, sizeof(opts.model) - 1);
         break;
+    case 'R': {
+        opts.nroots = 0;
+        const char* p = val;
+        while (*p) {
+            char* end = NULL;
+            if (!isdigit((unsigned char)*p) || opts.nroots == CMDLINE_MAX_ROOTS)
+                die_invalid(argv0, opt, val);
+            opts.roots[opts.nroots++] = (uint64_t)strtoull(p, &end, 10);
+            if (*end == ',' && end[1] != '\0')
+                p = end + 1;
+            else if (*end == '\0')
+                p = end;
+            else
+                die_invalid(argv0, opt, val);
+        }
+        if (opts.nroots == 0)
+            die_invalid(argv0, opt, val);
+        break;
+    }
     default:
         die_unknown(argv0, opt);
     }

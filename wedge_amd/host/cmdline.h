@@ -7,7 +7,10 @@
  * CPU threads and NUMA nodes in the reference; here they are accepted and
  * ignored (one GPU does the work).  Additions: -d device, -s schedule,
  * -b block granularity, -D damping, -f FP32 PageRank contributions,
- * -g generator, -W weight type of the input's third word, -m schedule model.
+ * -g generator, -W weight type of the input's third word, -m schedule model,
+ * -R list of roots (BFS / SSSP from several sources at once) or seeds
+ * (application ppr).  A comma-separated -i or -g names the members of a batch
+ * of graphs run together.
  */
 #ifndef WG_CMDLINE_H
 #define WG_CMDLINE_H
@@ -17,9 +20,10 @@
 #define CMDLINE_DEFAULT_NUM_ITERATIONS 1
 #define CMDLINE_DEFAULT_ROOT_VERTEX 0
 #define CMDLINE_DEFAULT_FRONTIER_THRESHOLD_PCT 20
+#define CMDLINE_MAX_ROOTS 64
 
 typedef struct cmdline_opts_t {
-    char graph_input_filename[1024]; /* -i */
+    char graph_input_filename[16384]; /* -i (comma-separated: a batch) */
     char* graph_application_output_filename; /* -o */
     char application[64];            /* -a */
     uint32_t num_iterations;         /* -N */
@@ -31,10 +35,12 @@ typedef struct cmdline_opts_t {
     int block_log2;                  /* -b */
     double damping;                  /* -D */
     int contrib_fp32;                /* -f */
-    char generator[256];             /* -g rmat:SCALE:EF:SEED[:sym] | grid:ROWS:COLS:SEED */
+    char generator[16384];           /* -g rmat:SCALE:EF:SEED[:sym] | grid:ROWS:COLS:SEED (comma-separated: a batch) */
     int weight_kind;                 /* -W int|f64 (SSSP input third word) */
     char model[1024];                /* -m */
     int root_given;
+    uint64_t roots[64];              /* -R */
+    uint32_t nroots;
 } cmdline_opts_t;
 
 /* Parses argv; prints a message and exits with the reference's codes

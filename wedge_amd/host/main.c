@@ -19,6 +19,7 @@
 #include <time.h>
 
 #include "cmdline.h"
+#include "main_batch.h"
 #include "wedge_host.h"
 
 #include <unistd.h>
@@ -55,6 +56,7 @@ static int app_from_string(const char* s)
     static const char* cc[] = { "cc", "connected_components", "connectedcomponents" };
     static const char* pr[] = { "pr", "pagerank" };
     static const char* sssp[] = { "sssp", "single_source_shortest_path", "singlesourceshortestpath" };
+    static const char* ppr[] = { "ppr", "personalized_pagerank", "personalised_pagerank" };
     for (size_t i = 0; i < 3; ++i)
         if (!strcasecmp(s, bfs[i]))
             return WG_APP_BFS;
@@ -67,6 +69,9 @@ static int app_from_string(const char* s)
     for (size_t i = 0; i < 3; ++i)
         if (!strcasecmp(s, sssp[i]))
             return WG_APP_SSSP;
+    for (size_t i = 0; i < 3; ++i)
+        if (!strcasecmp(s, ppr[i]))
+            return WG_APP_PPR;
     return WG_APP_NONE;
 }
 
@@ -77,6 +82,7 @@ static const char* app_name(int app)
     case WG_APP_CC: return "Connected Components";
     case WG_APP_PR: return "PageRank";
     case WG_APP_SSSP: return "Single-Source Shortest Path";
+    case WG_APP_PPR: return "Personalized PageRank";
     default: return "?";
     }
 }
@@ -119,6 +125,15 @@ static int generate(const char* spec, int app, wg_graph* g)
     return WG_E_INVAL;
 }
 
+/* One member of a batch of graphs: a generator spec (-g) or a file (-i). */
+static int load_member(const char* item, int app, wg_graph* g)
+{
+    const cmdline_opts_t* cs = cmdline_get_current_settings();
+    if (cs->generator[0])
+        return generate(item, app, g);
+    return wg_graph_read(g, item, app == WG_APP_SSSP ? cs->weight_kind : WG_WEIGHT_NONE);
+}
+
 int main(int argc, char* argv[])
 {
     cmdline_parse_options_or_die(argc, argv);
@@ -131,6 +146,12 @@ int main(int argc, char* argv[])
         return __LINE__;
     }
     printf("Application: %s on HIP device %d\n", app_name(app), cs->device);
+    if (strchr(cs->generator[0] ? cs->generator : cs->graph_input_filename, ','))
+        return wg_main_members(cs, app, load_member);
+    if (app == WG_APP_PPR && cs->nroots == 0) {
+        printf("Application: ERROR: ppr needs seeds (-R)\n");
+        return __LINE__;
+    }
 
     wg_graph g;
     double t0 = now_ms();
@@ -193,6 +214,14 @@ int main(int argc, char* argv[])
             printf("Schedule:    model \"%s\"\n", model);
     }
 
+    if (app == WG_APP_PPR || cs->nroots > 1) {
+        const int ret = wg_main_sources(cs, app, ctx, &g);
+        wg_close(ctx);
+        wg_graph_free(&g);
+        return ret;
+    }
+    if (cs->nroots == 1)
+        o.root = cs->roots[0];
     const uint64_t V = g.num_vertices;
     void* values = NULL;
     size_t esz = 8;

@@ -822,3 +822,52 @@ uint64_t wg_graph_max_outdegree_vertex(const wg_graph* g)
     free(od);
     return best;
 }
+
+int wg_graph_union(const wg_graph* members, uint32_t n, wg_graph* out, uint64_t* first)
+{
+    if (!members || !out || !first || n == 0)
+        return WG_E_INVAL;
+    memset(out, 0, sizeof(*out));
+    const int kind = members[0].weight_kind;
+    uint64_t V = 0, E = 0;
+    for (uint32_t g = 0; g < n; ++g) {
+        if (members[g].weight_kind != kind || !members[g].offsets)
+            return WG_E_INVAL;
+        first[g] = V;
+        V += members[g].num_vertices;
+        E += members[g].num_edges;
+    }
+    first[n] = V;
+    if (V >= 0xffffffffull || E >= 0xffffffffull)
+        return WG_E_RANGE;
+    const size_t wsz = kind == WG_WEIGHT_F64 ? sizeof(double) : kind == WG_WEIGHT_U32 ? sizeof(uint32_t) : 0;
+    uint32_t* off = (uint32_t*)malloc((V + 1) * sizeof(uint32_t));
+    uint32_t* src = (uint32_t*)malloc((E ? E : 1) * sizeof(uint32_t));
+    char* w = wsz ? (char*)malloc((E ? E : 1) * wsz) : NULL;
+    if (!off || !src || (wsz && !w)) {
+        free(off);
+        free(src);
+        free(w);
+        return WG_E_NOMEM;
+    }
+    uint64_t e0 = 0;
+    for (uint32_t g = 0; g < n; ++g) {
+        const wg_graph* m = &members[g];
+        const uint32_t v0 = (uint32_t)first[g];
+        for (uint64_t v = 0; v < m->num_vertices; ++v)
+            off[v0 + v] = (uint32_t)(e0 + m->offsets[v]);
+        for (uint64_t e = 0; e < m->num_edges; ++e)
+            src[e0 + e] = v0 + m->sources[e];
+        if (wsz && m->num_edges)
+            memcpy(w + e0 * wsz, m->weights, m->num_edges * wsz);
+        e0 += m->num_edges;
+    }
+    off[V] = (uint32_t)E;
+    out->num_vertices = V;
+    out->num_edges = E;
+    out->offsets = off;
+    out->sources = src;
+    out->weights = w;
+    out->weight_kind = kind;
+    return WG_OK;
+}
