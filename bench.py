@@ -293,6 +293,53 @@ def large_graph(damping, steps=5, iterations=16, reps=3):
     return out
 
 
+def measured_copy_peak(gib=4, reps=5):
+    """Device-to-device copy of `gib` GiB (read + write) timed with CUDA events:
+    the bandwidth a streaming kernel reaches on this box, next to the 8 TB/s
+    nominal peak."""
+    import torch
+
+    n = gib * (1 << 30) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return round(2 * gib * (1 << 30) / (best * 1e-3) / 1e9, 1)
+
+
+def host_cpu_info():
+    """CPU model and logical / physical core counts of the host."""
+    model, phys = "unknown", set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "model name":
+                    model = v
+                cur[k] = v
+            elif cur:
+                phys.add((cur.get("physical id"), cur.get("core id")))
+                cur = {}
+        if cur:
+            phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return {"model": model, "logical": os.cpu_count(), "physical": len(phys) or None,
+            "usable": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(g, damping, budget_s=20.0):
     """The oracle's PageRank on the same graph, all host threads, a bounded
     number of iterations (time per iteration -> GTEPS)."""
@@ -326,6 +373,7 @@ def cpu_baseline(g, damping, budget_s=20.0):
         "kind": "port",
         "sample": f"oracle PageRank, {iters} iteration(s) on the same graph, {threads} threads, "
                   f"{ms:.1f} ms/iteration (layout build {build_s:.1f} s not timed)",
+        "host": host_cpu_info(),
         "pr_l1_hip_vs_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "pr_iterations_compared": iters,
     }
@@ -422,6 +470,7 @@ def main():
             "traffic": measured_traffic(args.config, args.fp32),
             "algorithmic_bytes_per_launch": alg,
             "avg_launch_ms": round(kern_ms, 5),
+            "measured_copy_GBs": measured_copy_peak(),
         },
         "pr_rank_sum": rank_sum,
     }

@@ -248,6 +248,26 @@ int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint32_t iterati
 /* Ranks (V entries, FP64) for seed number `seed`. */
 int wg_read_ppr(wg_ctx* ctx, uint32_t seed, double* host_ranks);
 
+/* ---- validation (no reference counterpart) -------------------------------
+ * Graph500-style certificates checked on the device against the loaded graph,
+ * one pass over the edges and one over the vertices:
+ *  BFS   roots: depth 0 and parent = itself; every edge u->v with u reached has
+ *        v reached and depth(v) <= depth(u) + 1; every other reached v has its
+ *        parent as an in-neighbour one level up.
+ *  SSSP  roots: distance 0; every edge u->v with u reached has v reached and
+ *        dist(v) <= dist(u) + w; every other reached v has an in-edge with
+ *        dist(u) + w == dist(v) (with positive weights: the shortest paths).
+ *  CC    every edge u->v has label(v) <= label(u); label(v) <= v; label(v) == v
+ *        or an in-neighbour carries label(v) (the minimum reaching label).
+ * violations = failing edges + failing vertices (0: the result is correct). */
+/* The context's current BFS / CC / SSSP result. */
+int wg_validate(wg_ctx* ctx, uint64_t* violations);
+/* A result computed elsewhere (app WG_APP_BFS / CC / SSSP), in the caller's
+ * numbering and the layouts of wg_read_bfs / wg_read_cc / wg_read_sssp, with
+ * its roots (BFS / SSSP). */
+int wg_validate_values(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t nroots, const void* values,
+                       uint64_t* violations);
+
 /* ---- batches of independent graphs (no reference counterpart) ------------
  * A batch of small graphs is loaded as one graph, their disjoint union:
  * member g owns the vertices [first[g], first[g+1]) and every edge stays

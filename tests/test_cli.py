@@ -29,6 +29,7 @@ def run(*args):
     (["-a", "bfs", "-i", "g.bin", "-R", "3,"], 2),
     (["-a", "bfs", "-i", "g.bin", "-R", ",".join(["1"] * 65)], 2),
     (["-a", "bfs", "-i", "g.bin", "-R"], 3),
+    (["-a", "bfs", "-i", "g.bin", "-c", "1"], 4),        # -c takes no value
 ])
 def test_cli_error_codes(args, code):
     assert run(*args).returncode == code

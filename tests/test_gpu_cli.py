@@ -43,6 +43,14 @@ def test_cli_single_bfs_and_pr(tmp_path):
     assert np.allclose(got, ref, rtol=1e-4, atol=0)
 
 
+@pytest.mark.parametrize("app", ["bfs", "cc", "sssp"])
+def test_cli_validates_results(app):
+    out = run("-a", app, "-g", "rmat:11:8:2", "-r", "0", "-c")
+    assert "Validation:  passed" in out
+    out = run("-a", app, "-g", "rmat:8:8:1,rmat:9:4:2", "-r", "0", "-c")
+    assert "Validation:  passed" in out
+
+
 def test_cli_multi_source_bfs_and_sssp(tmp_path):
     roots = [0, 5, 9, 700]
     g = W.rmat(10, 8, seed=1)

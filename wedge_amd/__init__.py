@@ -127,6 +127,8 @@ GPU_SYMBOLS = [
     ("wg_init_bfs_roots", C.c_int, [_p, _u64p, C.c_uint32]),
     ("wg_init_sssp_roots", C.c_int, [_p, _u64p, C.c_uint32]),
     ("wg_set_members", C.c_int, [_p, C.c_uint32, _u64p]),
+    ("wg_validate", C.c_int, [_p, _u64p]),
+    ("wg_validate_values", C.c_int, [_p, C.c_int, _u64p, C.c_uint32, _p, _u64p]),
     ("wg_batch_shape_at", C.c_int, [C.c_uint32, _p]),
     ("wg_batch_pr", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, C.c_int]),
     ("wg_batch_autotune", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, _f32p]),
@@ -528,6 +530,21 @@ class Context:
     def init_sssp_roots(self, roots):
         r = np.ascontiguousarray(roots, np.uint64)
         self._chk(self._lib.wg_init_sssp_roots(self._h, _ptr(r, C.c_uint64), r.size), "wg_init_sssp_roots")
+
+    def validate(self) -> int:
+        """Violations of the current BFS / CC / SSSP result (0 = correct)."""
+        n = C.c_uint64()
+        self._chk(self._lib.wg_validate(self._h, C.byref(n)), "wg_validate")
+        return int(n.value)
+
+    def validate_values(self, app: int, values: np.ndarray, roots=()) -> int:
+        """Violations of a result computed elsewhere (wg_read_* layouts, caller's numbering)."""
+        r = np.ascontiguousarray(roots, np.uint64)
+        v = np.ascontiguousarray(values)
+        n = C.c_uint64()
+        self._chk(self._lib.wg_validate_values(self._h, app, _ptr(r, C.c_uint64) if r.size else None, r.size,
+                                               v.ctypes.data, C.byref(n)), "wg_validate_values")
+        return int(n.value)
 
     def load_batch(self, graphs) -> np.ndarray:
         """Loads member graphs as one disjoint union (relabelling off, so every

@@ -20,7 +20,7 @@ static cmdline_opts_t opts;
 
 static int is_recognized(char c)
 {
-    return strchr("ahinNortuVdsbDfgWmR", c) != NULL;
+    return strchr("ahinNortuVdsbDfgWmRc", c) != NULL;
 }
 
 static int requires_value(char c)
@@ -100,6 +100,7 @@ static void usage(const char* argv0)
     printf("  -R v[,v...]           BFS/SSSP from up to %d roots in one traversal; seeds of -a ppr (up to 32)\n",
            CMDLINE_MAX_ROOTS);
     printf("  -a ppr                personalised PageRank from the -R seeds (-N iterations, -D damping)\n");
+    printf("  -c                    validate the BFS/CC/SSSP result on the device (exit code 7 if invalid)\n");
     printf("  -i a,b,... / -g a,b,...  a batch of graphs run together (BFS/SSSP from vertex -r of each)\n");
     exit(0);
 }
@@ -191,6 +192,9 @@ static void parse_one(const char* argv0, const char* opt, const char* val)
     }
     case 'f':
         opts.contrib_fp32 = 1;
+        break;
+    case 'c':
+        opts.validate = 1;
         break;
     case 'g':
         strncpy(opts.generator, val, sizeof(opts.generator) - 1);

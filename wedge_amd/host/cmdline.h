@@ -10,7 +10,7 @@
  * -g generator, -W weight type of the input's third word, -m schedule model,
  * -R list of roots (BFS / SSSP from several sources at once) or seeds
  * (application ppr).  A comma-separated -i or -g names the members of a batch
- * of graphs run together.
+ * of graphs run together.  -c validates the result on the device.
  */
 #ifndef WG_CMDLINE_H
 #define WG_CMDLINE_H
@@ -41,6 +41,7 @@ typedef struct cmdline_opts_t {
     int root_given;
     uint64_t roots[64];              /* -R */
     uint32_t nroots;
+    int validate;                    /* -c */
 } cmdline_opts_t;
 
 /* Parses argv; prints a message and exits with the reference's codes

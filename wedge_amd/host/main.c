@@ -317,8 +317,11 @@ int main(int argc, char* argv[])
         printf("----------------------------------------------------\n");
     }
     printf("\n");
+    int ret = 0;
+    if (cs->validate && app != WG_APP_PR)
+        ret = wg_main_validate(ctx);
     free(values);
     wg_close(ctx);
     wg_graph_free(&g);
-    return 0;
+    return ret;
 }

@@ -51,6 +51,22 @@ static void print_dist(FILE* f, int wkind, const void* col, uint64_t v)
     }
 }
 
+int wg_main_validate(wg_ctx* ctx)
+{
+    uint64_t bad = 0;
+    const int rc = wg_validate(ctx, &bad);
+    if (rc) {
+        printf("Validation:  ERROR: %s\n", wg_error_string(rc));
+        return 1;
+    }
+    if (bad) {
+        printf("Validation:  FAILED (%llu violations)\n", (unsigned long long)bad);
+        return 7;
+    }
+    printf("Validation:  passed\n");
+    return 0;
+}
+
 int wg_main_sources(const cmdline_opts_t* cs, int app, wg_ctx* ctx, const wg_graph* g)
 {
     const uint64_t V = g->num_vertices;
@@ -247,7 +263,7 @@ int wg_main_members(const cmdline_opts_t* cs, int app, member_loader load_one)
         }
     }
     print_stats(running, u.num_edges * (iters ? iters : 1), iters, "Graphs", n);
-    ret = 0;
+    ret = (cs->validate && app != WG_APP_PR) ? wg_main_validate(ctx) : 0;
 done:
     free(values);
     if (ctx)

@@ -8,6 +8,10 @@
 #include "cmdline.h"
 #include "wedge_host.h"
 
+/* -c: validates the context's BFS / CC / SSSP result on the device and prints
+ * the verdict; returns 0 if valid, 7 if not, 1 on error. */
+int wg_main_validate(wg_ctx* ctx);
+
 /* Loads one member of a batch: a file name or a generator spec. */
 typedef int (*member_loader)(const char* item, int app, wg_graph* out);
 
