@@ -111,6 +111,18 @@ int wg_set_option(wg_ctx* ctx, int option, int64_t value);
 int wg_load_csc(wg_ctx* ctx, uint64_t num_vertices, uint64_t num_edges,
                 const uint32_t* offsets, const uint32_t* sources,
                 const void* weights, int weight_kind);
+/* Builds the graph on the device from an edge list in any order (no reference
+ * counterpart; the reference reads edges already grouped by destination,
+ * source/graphdata.c:247-585): src[e] -> dst[e] with optional weights[e]
+ * (layout of weight_kind).  The edges are sorted by (destination, source) with
+ * a stable radix sort; flags drop self-loops (WG_EDGES_DROP_SELF_LOOPS) and
+ * repeated (src, dst) pairs (WG_EDGES_DEDUP, the first occurrence's weight
+ * stays).  The result is loaded as by wg_load_csc.  kept_edges (optional)
+ * receives the number of edges kept.  WG_E_RANGE if an endpoint is >= V. */
+#define WG_EDGES_DROP_SELF_LOOPS 1
+#define WG_EDGES_DEDUP 2
+int wg_load_edges(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* src, const uint32_t* dst, const void* weights,
+                  int weight_kind, int flags, uint64_t* kept_edges);
 
 /* Number of vertices / edges loaded. */
 int wg_graph_info(const wg_ctx* ctx, uint64_t* num_vertices, uint64_t* num_edges);

@@ -116,3 +116,65 @@ def test_validate_batched_union():
     with pytest.raises(W.WedgeError):
         c.validate_values(W.APP_PR, np.zeros(c.V), [])
     c.close()
+
+
+# ---- graphs built on the device from unsorted edge lists ---------------------------
+
+
+def _shuffled_edges(g, rng, dup_frac=0.1, loops=50):
+    """g's edges in random order, with repeats and self-loops added."""
+    src, dst = g.sources.astype(np.int64), g.destinations().astype(np.int64)
+    w = g.weights
+    k = int(dup_frac * src.size)
+    pick = rng.integers(0, src.size, k)
+    lv = rng.integers(0, g.num_vertices, loops)
+    s = np.concatenate([src, src[pick], lv])
+    d = np.concatenate([dst, dst[pick], lv])
+    ww = None
+    if w is not None:
+        extra = np.full(k + loops, 7.0 if w.dtype == np.float64 else 7, w.dtype)  # repeats carry other weights
+        ww = np.concatenate([w, extra])
+    # originals in random order first, then the repeats and loops in random order,
+    # so that dedup keeps each original's weight
+    order = np.concatenate([rng.permutation(src.size), src.size + rng.permutation(k + loops)])
+    return s[order], d[order], (ww[order] if ww is not None else None)
+
+
+@pytest.mark.parametrize("kind", ["rmat", "rmat_f64", "grid"])
+def test_load_edges_builds_the_same_graph(kind):
+    rng = np.random.default_rng(7)
+    g = {"rmat": lambda: W.rmat(11, 8, seed=3), "rmat_f64": lambda: W.rmat(11, 8, seed=3, weights="f64"),
+         "grid": lambda: W.grid(40, 50, seed=3)}[kind]()
+    s, d, w = _shuffled_edges(g, rng)
+    c = W.Context(0, relabel=False)
+    kept = c.load_edges(g.num_vertices, s, d, w)
+    assert kept == g.num_edges
+    assert np.array_equal(c.outdegrees(), g.outdegrees())
+    root = g.max_outdegree_vertex()
+    o = oracle.Oracle.of(g)
+    packed, _ = c.run_bfs(root)
+    assert np.array_equal(W.bfs_depth(packed), o.bfs(root)[1])
+    if g.weights is not None:
+        dist, _ = c.run_sssp(root)
+        ref, _ = o.sssp(root)
+        if g.weight_kind == W.WEIGHT_U32:
+            ref = np.where(ref == np.uint64(2**64 - 1), U32_MAX, ref).astype(np.uint32)
+        assert np.array_equal(dist, ref)  # repeats kept their first weight, not the 7s
+    ranks, _ = c.run_pr(iterations=8, damping=0.85)
+    assert float(np.abs(ranks - o.pr(8, 0.85)[0]).sum()) <= 1e-10
+    # keeping repeats and loops: every input edge is loaded
+    assert c.load_edges(g.num_vertices, s, d, w, dedup=False, drop_self_loops=False) == s.size
+    assert int(c.outdegrees().sum()) == s.size
+    c.close()
+
+
+def test_load_edges_errors_and_empty():
+    c = W.Context(0)
+    with pytest.raises(W.WedgeError) as e:
+        c.load_edges(10, [1, 2], [3, 10])  # endpoint out of range
+    assert e.value.code == W.WG_E_RANGE
+    assert c.load_edges(5, [], []) == 0
+    assert c.load_edges(3, [0, 0, 1], [0, 0, 0]) == 1  # a loop, then a repeat: only 1 -> 0 stays
+    c.run_cc()
+    assert c.validate() == 0
+    c.close()

@@ -117,6 +117,7 @@ GPU_SYMBOLS = [
     ("wg_sync", C.c_int, [_p]),
     ("wg_set_option", C.c_int, [_p, C.c_int, C.c_int64]),
     ("wg_load_csc", C.c_int, [_p, C.c_uint64, C.c_uint64, _u32p, _u32p, _p, C.c_int]),
+    ("wg_load_edges", C.c_int, [_p, C.c_uint64, C.c_uint64, _u32p, _u32p, _p, C.c_int, C.c_int, _u64p]),
     ("wg_graph_info", C.c_int, [_p, _u64p, _u64p]),
     ("wg_read_outdegrees", C.c_int, [_p, _u32p]),
     ("wg_vertex_outdegree", C.c_int, [_p, C.c_uint64, _u32p]),
@@ -501,6 +502,31 @@ class Context:
                                         _ptr(src, C.c_uint32), w.ctypes.data if w is not None else None,
                                         g.weight_kind), "wg_load_csc")
         self.graph = g
+        self._wkind = g.weight_kind
+
+    def load_edges(self, num_vertices: int, src, dst, weights=None, dedup=True, drop_self_loops=True) -> int:
+        """Builds the graph on the device from edges in any order (wg_load_edges);
+        returns the number of edges kept.  weights: float64 or uint32 per edge."""
+        s = np.ascontiguousarray(src, np.uint32)
+        d = np.ascontiguousarray(dst, np.uint32)
+        if s.shape != d.shape:
+            raise ValueError("src and dst differ in length")
+        w, wk = None, WEIGHT_NONE
+        if weights is not None:
+            w = np.ascontiguousarray(weights)
+            if w.dtype == np.float64:
+                wk = WEIGHT_F64
+            else:
+                w = np.ascontiguousarray(w, np.uint32)
+                wk = WEIGHT_U32
+        flags = (2 if dedup else 0) | (1 if drop_self_loops else 0)
+        kept = C.c_uint64()
+        self._chk(self._lib.wg_load_edges(self._h, num_vertices, s.size, _ptr(s, C.c_uint32), _ptr(d, C.c_uint32),
+                                          w.ctypes.data if w is not None else None, wk, flags, C.byref(kept)),
+                  "wg_load_edges")
+        self.graph = None
+        self._wkind = wk
+        return int(kept.value)
 
     def sync(self):
         self._chk(self._lib.wg_sync(self._h), "wg_sync")
@@ -647,7 +673,7 @@ class Context:
         return out
 
     def read_sssp(self) -> np.ndarray:
-        f64 = self.graph is not None and self.graph.weight_kind == WEIGHT_F64
+        f64 = getattr(self, "_wkind", WEIGHT_NONE) == WEIGHT_F64
         out = np.empty(self.V, np.float64 if f64 else np.uint32)
         self._chk(self._lib.wg_read_sssp(self._h, out.ctypes.data), "wg_read_sssp")
         return out
@@ -716,7 +742,7 @@ class Context:
         return int(it.value)
 
     def read_mssssp(self, source: int) -> np.ndarray:
-        f64 = self.graph is not None and self.graph.weight_kind == WEIGHT_F64
+        f64 = getattr(self, "_wkind", WEIGHT_NONE) == WEIGHT_F64
         out = np.empty(self.V, np.float64 if f64 else np.uint32)
         self._chk(self._lib.wg_read_mssssp(self._h, source, out.ctypes.data), "wg_read_mssssp")
         return out
@@ -778,7 +804,7 @@ class Context:
 
     def run_sssp(self, root: int, **kw):
         o = run_opts(root=root, **kw)
-        f64 = self.graph is not None and self.graph.weight_kind == WEIGHT_F64
+        f64 = getattr(self, "_wkind", WEIGHT_NONE) == WEIGHT_F64
         out = np.empty(self.V, np.float64 if f64 else np.uint32)
         st = RunStats()
         self._chk(_lib_host().wg_run_sssp(self._h, C.byref(o), out.ctypes.data, C.byref(st)), "wg_run_sssp")

@@ -637,6 +637,11 @@ done:
     return rc;
 }
 
+void wg_drop_graph(wg_ctx* ctx)
+{
+    free_graph(ctx);
+}
+
 extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* off, const uint32_t* src,
                            const void* weights, int weight_kind)
 {

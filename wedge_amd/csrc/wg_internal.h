@@ -285,3 +285,5 @@ void wg_build_tiles(const uint32_t* off, uint64_t V, uint32_t tile_edges, std::v
                     std::vector<uint32_t>& te, std::vector<uint32_t>& hv, std::vector<uint32_t>& ht);
 // releases the member batch (with its HIP graph); called when the graph goes
 void wg_free_members(wg_ctx* ctx);
+// releases the loaded graph and everything built on it
+void wg_drop_graph(wg_ctx* ctx);
