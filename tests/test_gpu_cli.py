@@ -41,6 +41,9 @@ def test_cli_single_bfs_and_pr(tmp_path):
     ref, _ = oracle.Oracle.of(g).pr(16, 0.85)
     got = np.array([float(r[1]) for r in table(out)])
     assert np.allclose(got, ref, rtol=1e-4, atol=0)
+    run("-a", "pr", "-g", "rmat:10:8:1", "-N", "16", "-D", "0.85", "-P", "-o", out)  # full precision
+    got = np.array([float(r[1]) for r in table(out)])
+    assert float(np.abs(got - ref).sum()) <= 1e-10
 
 
 @pytest.mark.parametrize("app", ["bfs", "cc", "sssp"])

@@ -20,7 +20,7 @@ static cmdline_opts_t opts;
 
 static int is_recognized(char c)
 {
-    return strchr("ahinNortuVdsbDfgWmRc", c) != NULL;
+    return strchr("ahinNortuVdsbDfgWmRcP", c) != NULL;
 }
 
 static int requires_value(char c)
@@ -101,6 +101,7 @@ static void usage(const char* argv0)
            CMDLINE_MAX_ROOTS);
     printf("  -a ppr                personalised PageRank from the -R seeds (-N iterations, -D damping)\n");
     printf("  -c                    validate the BFS/CC/SSSP result on the device (exit code 7 if invalid)\n");
+    printf("  -P                    -o writes ranks with 17 significant digits instead of 5\n");
     printf("  -i a,b,... / -g a,b,...  a batch of graphs run together (BFS/SSSP from vertex -r of each)\n");
     exit(0);
 }
@@ -195,6 +196,9 @@ static void parse_one(const char* argv0, const char* opt, const char* val)
         break;
     case 'c':
         opts.validate = 1;
+        break;
+    case 'P':
+        opts.full_precision = 1;
         break;
     case 'g':
         strncpy(opts.generator, val, sizeof(opts.generator) - 1);

@@ -10,7 +10,8 @@
  * -g generator, -W weight type of the input's third word, -m schedule model,
  * -R list of roots (BFS / SSSP from several sources at once) or seeds
  * (application ppr).  A comma-separated -i or -g names the members of a batch
- * of graphs run together.  -c validates the result on the device.
+ * of graphs run together.  -c validates the result on the device; -P writes
+ * ranks at full precision.
  */
 #ifndef WG_CMDLINE_H
 #define WG_CMDLINE_H
@@ -42,6 +43,7 @@ typedef struct cmdline_opts_t {
     uint64_t roots[64];              /* -R */
     uint32_t nroots;
     int validate;                    /* -c */
+    int full_precision;              /* -P */
 } cmdline_opts_t;
 
 /* Parses argv; prints a message and exits with the reference's codes

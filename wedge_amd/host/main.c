@@ -261,7 +261,8 @@ int main(int argc, char* argv[])
             for (uint64_t v = 0; v < V && !ferror(f); ++v) {
                 switch (app) {
                 case WG_APP_PR:
-                    fprintf(f, "%llu %.4le\n", (unsigned long long)v, ((double*)values)[v]);
+                    fprintf(f, cs->full_precision ? "%llu %.17g\n" : "%llu %.4le\n", (unsigned long long)v,
+                            ((double*)values)[v]);
                     break;
                 case WG_APP_BFS: {
                     const uint64_t pk = ((uint64_t*)values)[v];

@@ -129,7 +129,7 @@ int wg_main_sources(const cmdline_opts_t* cs, int app, wg_ctx* ctx, const wg_gra
                 } else if (app == WG_APP_SSSP) {
                     print_dist(f, g->weight_kind, col, v);
                 } else {
-                    fprintf(f, " %.4le", ((const double*)col)[v]);
+                    fprintf(f, cs->full_precision ? " %.17g" : " %.4le", ((const double*)col)[v]);
                 }
             }
             fprintf(f, "\n");
@@ -244,7 +244,7 @@ int wg_main_members(const cmdline_opts_t* cs, int app, member_loader load_one)
             for (uint64_t v = first[g]; v < first[g + 1]; ++v) {
                 fprintf(f, "%u %llu", g, (unsigned long long)(v - first[g]));
                 if (app == WG_APP_PR) {
-                    fprintf(f, " %.4le", ((const double*)values)[v]);
+                    fprintf(f, cs->full_precision ? " %.17g" : " %.4le", ((const double*)values)[v]);
                 } else if (app == WG_APP_BFS) {
                     const uint64_t pk = ((const uint64_t*)values)[v];
                     fprintf(f, " %lld", pk == UINT64_MAX ? -1ll : (long long)((pk & 0xffffffffull) - first[g]));
