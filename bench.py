@@ -228,8 +228,9 @@ def bfs_algorithmic_bytes(V: int, st: dict) -> int:
 
 def pr_timed(ctx, damping, fp32, steps, warmup=2):
     """Wall ms per fused PageRank iteration and the tile + span kernel ms per
-    launch (HIP events on the engine's stream)."""
+    launch (HIP events on the engine's stream), with the autotuned tile shape."""
     ctx.init_pr(damping, fp32)
+    ctx.pr_autotune(2)
     for _ in range(warmup):
         ctx.pr_iteration()
     ctx.sync()
@@ -266,6 +267,7 @@ def large_graph(damping, steps=5, iterations=16, reps=3):
         alg = pr_algorithmic_bytes(V, E, fp32)
         key = "pr_fp32" if fp32 else "pr_fp64"
         out[key] = {"ms_per_iteration": round(wall, 3), "gteps": round(E / (wall * 1e-3) / 1e9, 2),
+                    "tile_shape": list(ctx.pr_shape()),
                     "kernel_ms": round(kern, 3), "algorithmic_bytes": alg,
                     "roofline_frac": round(alg / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         ctx.init_pr(damping, fp32)
@@ -291,31 +293,6 @@ def large_graph(damping, steps=5, iterations=16, reps=3):
     out["bfs_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
     ctx.close()
     return out
-
-
-def measured_copy_peak(gib=4, reps=5):
-    """Device-to-device copy of `gib` GiB (read + write) timed with CUDA events:
-    the bandwidth a streaming kernel reaches on this box, next to the 8 TB/s
-    nominal peak."""
-    import torch
-
-    n = gib * (1 << 30) // 4
-    a = torch.empty(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    best = None
-    for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        b.copy_(a)
-        e1.record()
-        e1.synchronize()
-        ms = e0.elapsed_time(e1)
-        best = ms if best is None else min(best, ms)
-    del a, b
-    torch.cuda.empty_cache()
-    return round(2 * gib * (1 << 30) / (best * 1e-3) / 1e9, 1)
 
 
 def host_cpu_info():
@@ -415,6 +392,8 @@ def main():
 
     # ---- PageRank: the timed steps -------------------------------------------------
     ctx.init_pr(args.damping, args.fp32)
+    shape, _ = ctx.pr_autotune(2)  # tile shape picked before the timed region; PageRank re-initialised
+    log(f"PR tile shape (edges/lane, WG/CU, XCD split): {shape}")
     for _ in range(args.warmup):
         ctx.pr_iteration()
     ctx.sync()
@@ -459,6 +438,7 @@ def main():
                         f"on rmat-s{cfg['scale']}-ef{cfg['edge_factor']}-seed{cfg['seed']}",
             "scale": cfg["scale"], "edge_factor": cfg["edge_factor"], "seed": cfg["seed"],
             "vertices": V, "edges": E, "parallelism": "single GPU",
+            "pr_tile_shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1], "xcd_split": shape[2]},
         },
         "roofline": {
             "bound": "hbm",
@@ -470,7 +450,7 @@ def main():
             "traffic": measured_traffic(args.config, args.fp32),
             "algorithmic_bytes_per_launch": alg,
             "avg_launch_ms": round(kern_ms, 5),
-            "measured_copy_GBs": measured_copy_peak(),
+            "measured_copy_GBs": round(ctx.measure_copy(4 << 30), 1),
         },
         "pr_rank_sum": rank_sum,
     }

@@ -327,6 +327,19 @@ int wg_batch_autotune(wg_ctx* ctx, uint32_t iterations, double damping, int cont
 /* Ranks of the last wg_batch_pr (V entries, FP64; each member sums to ~1). */
 int wg_read_batch_ranks(wg_ctx* ctx, double* host_ranks);
 
+/* The tile shape of the single-graph PageRank (wg_pr_iteration,
+ * wg_edge_pull_pr): the same candidates as the batch.  NULL restores the
+ * default {8, 8, 0}.  The shape stays until the next graph load; results do
+ * not depend on it beyond FP64 summation order (every shape sums in a fixed
+ * order). */
+int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape);
+int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape);
+/* After wg_init_pr: times `iterations` fused iterations per candidate (best of
+ * two after a warm-up), keeps the fastest shape, and initialises PageRank
+ * again with the same damping and precision.  ms_per_shape (optional,
+ * WG_BATCH_NSHAPES entries) receives the time per iteration of each. */
+int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* best, float* ms_per_shape);
+
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;
  * the root holds (0 << 32) | root. */
 int wg_read_bfs(wg_ctx* ctx, uint64_t* host_packed);
@@ -413,6 +426,10 @@ int wg_profile_enable(wg_ctx* ctx, int on);
  * or the last reset (waits for the stream). */
 int wg_profile_get(wg_ctx* ctx, int kernel_id, uint64_t* launches, double* total_ms);
 int wg_profile_reset(wg_ctx* ctx);
+/* Device-to-device streaming copy of `bytes` (read + write), best of five
+ * after a warm-up: the bandwidth an HBM-bound kernel reaches on this device,
+ * reported next to the nominal 8 TB/s. */
+int wg_measure_copy(wg_ctx* ctx, uint64_t bytes, double* gb_per_s);
 
 #ifdef __cplusplus
 }

@@ -116,6 +116,48 @@ def test_pr_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 0), (4, 16, 1), (16, 2, 0), (16, 8, 1), (8, 2, 1)])
+def test_pr_tile_shapes_match_oracle(ctx, shape):
+    """Every PageRank tile shape (edges per lane, workgroups per CU, XCD split)
+    on a power-law graph and on a graph whose hubs exceed every tile size;
+    FP64 and FP32-stored contributions, fused and two-phase iterations."""
+    for g, its in ((W.rmat(13, 16, seed=4), 16), (_hub_graph(), 10)):
+        ctx.load(g)
+        ctx.set_pr_shape(shape)
+        assert ctx.pr_shape() == shape
+        ref, _ = oracle.Oracle.of(g).pr(its, 0.85)
+        ranks, _ = ctx.run_pr(iterations=its, damping=0.85)
+        assert float(np.abs(ranks - ref).sum()) <= PR_TOL, shape
+        ranks32, _ = ctx.run_pr(iterations=its, damping=0.85, contrib_fp32=1)
+        assert float(np.abs(ranks32 - ref).sum()) <= PR_TOL_FP32, shape
+        ctx.init_pr(0.85)
+        for _ in range(its):
+            ctx.edge_pull_pr()
+            ctx.vertex_pr()
+        assert float(np.abs(ctx.read_ranks() - ref).sum()) <= PR_TOL, shape
+    ctx.set_pr_shape(None)
+    assert ctx.pr_shape() == (8, 8, 0)
+
+
+def test_pr_autotune_resets_the_state(ctx):
+    g = W.rmat(13, 16, seed=4)
+    ctx.load(g)
+    ctx.init_pr(0.85)
+    best, times = ctx.pr_autotune(2)
+    assert best in times and times[best] == min(times.values()) and ctx.pr_shape() == best
+    print("PR autotune:", best, {k: round(v, 4) for k, v in times.items()})
+    for _ in range(12):
+        ctx.pr_iteration()
+    tuned = ctx.read_ranks()
+    again, _ = ctx.run_pr(iterations=12, damping=0.85)  # fresh run, same shape
+    assert np.array_equal(tuned, again)
+    ref, _ = oracle.Oracle.of(g).pr(12, 0.85)
+    assert float(np.abs(tuned - ref).sum()) <= PR_TOL
+    with pytest.raises(W.WedgeError):
+        ctx.set_pr_shape((5, 4, 0))
+    ctx.set_pr_shape(None)
+
+
 # ---- BFS -------------------------------------------------------------------------
 
 # device-resident loop (default) and the host-driven loop of the phase API

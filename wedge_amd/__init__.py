@@ -134,6 +134,9 @@ GPU_SYMBOLS = [
     ("wg_batch_pr", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, C.c_int]),
     ("wg_batch_autotune", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, _f32p]),
     ("wg_read_batch_ranks", C.c_int, [_p, _f64p]),
+    ("wg_set_pr_shape", C.c_int, [_p, _p]),
+    ("wg_get_pr_shape", C.c_int, [_p, _p]),
+    ("wg_pr_autotune", C.c_int, [_p, C.c_uint32, _p, _f32p]),
     ("wg_edge_pull_pr", C.c_int, [_p, _f64p]),
     ("wg_vertex_pr", C.c_int, [_p]),
     ("wg_pr_iteration", C.c_int, [_p, _f64p]),
@@ -153,6 +156,7 @@ GPU_SYMBOLS = [
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
+    ("wg_measure_copy", C.c_int, [_p, C.c_uint64, _f64p]),
     ("wg_msbfs", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p, _u64p]),
     ("wg_read_msbfs", C.c_int, [_p, C.c_uint32, _u32p]),
     ("wg_mssssp", C.c_int, [_p, _u64p, C.c_uint32, _p, _u64p]),
@@ -601,6 +605,24 @@ class Context:
                                               _ptr(ms, C.c_float)), "wg_batch_autotune")
         return best.as_tuple(), {batch_shape_at(i): float(ms[i]) for i in range(BATCH_NSHAPES)}
 
+    def set_pr_shape(self, shape=None):
+        """Tile shape of the single-graph PageRank: (edges_per_lane, blocks_per_cu[, xcd_split]) or None."""
+        s = None if shape is None else C.byref(BatchShape(*shape))
+        self._chk(self._lib.wg_set_pr_shape(self._h, s), "wg_set_pr_shape")
+
+    def pr_shape(self):
+        s = BatchShape()
+        self._chk(self._lib.wg_get_pr_shape(self._h, C.byref(s)), "wg_get_pr_shape")
+        return s.as_tuple()
+
+    def pr_autotune(self, iterations=4):
+        """After init_pr: picks the fastest PageRank tile shape and re-initialises;
+        returns (best shape, {shape: ms per iteration})."""
+        best = BatchShape()
+        ms = np.zeros(BATCH_NSHAPES, np.float32)
+        self._chk(self._lib.wg_pr_autotune(self._h, iterations, C.byref(best), _ptr(ms, C.c_float)), "wg_pr_autotune")
+        return best.as_tuple(), {batch_shape_at(i): float(ms[i]) for i in range(BATCH_NSHAPES)}
+
     def read_batch_ranks(self) -> np.ndarray:
         out = np.empty(self.V, np.float64)
         self._chk(self._lib.wg_read_batch_ranks(self._h, _ptr(out, C.c_double)), "wg_read_batch_ranks")
@@ -686,6 +708,12 @@ class Context:
     # -- profiling --
     def profile(self, on=True):
         self._chk(self._lib.wg_profile_enable(self._h, int(on)), "wg_profile_enable")
+
+    def measure_copy(self, bytes_=4 << 30) -> float:
+        """GB/s of a device-to-device streaming copy (read + write)."""
+        g = C.c_double()
+        self._chk(self._lib.wg_measure_copy(self._h, bytes_, C.byref(g)), "wg_measure_copy")
+        return float(g.value)
 
     def profile_reset(self):
         self._chk(self._lib.wg_profile_reset(self._h), "wg_profile_reset")

@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "wg_internal.h"
@@ -428,6 +429,17 @@ static void free_graph(wg_ctx* ctx)
     free_dev((void**)&ctx->bfs_heavy_e);
     free_dev((void**)&ctx->tile_part);
     free_dev((void**)&ctx->blk_part);
+    for (auto& t : ctx->pr_alt) {
+        free_dev((void**)&t.tile_v);
+        free_dev((void**)&t.tile_e);
+        free_dev((void**)&t.heavy_v);
+        free_dev((void**)&t.heavy_t);
+        t.ntiles = t.nheavy = 0;
+    }
+    ctx->pr_alt_built = 0;
+    ctx->pr_ipt = WG_IPT;
+    ctx->pr_bpc = 8;
+    ctx->pr_xcd = 0;
     free_dev((void**)&ctx->bm_cur);
     free_dev((void**)&ctx->bm_next);
     free_dev((void**)&ctx->bm_visited);
@@ -457,6 +469,58 @@ extern "C" void wg_close(wg_ctx* ctx)
     (void)hipHostFree(ctx->hctr);
     (void)hipStreamDestroy(ctx->stream);
     free(ctx);
+}
+
+// Streaming copy, 16 bytes per lane per step: the bandwidth an HBM-bound
+// kernel reaches on this device.
+typedef unsigned int wg_u32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_stream_copy(const wg_u32x4* __restrict__ in, wg_u32x4* __restrict__ out, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+extern "C" int wg_measure_copy(wg_ctx* ctx, uint64_t bytes, double* gb_per_s)
+{
+    if (!ctx || !gb_per_s || bytes < 16)
+        return WG_E_INVAL;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    const uint64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = WG_OK;
+    float best = 0.0f;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return WG_E_NOMEM;
+    }
+    if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess ||
+        hipMemsetAsync(a, 0, n * 16, ctx->stream) != hipSuccess) {
+        rc = WG_E_HIP;
+    } else {
+        const uint32_t grid = ctx->num_cu * 4; // the fastest of 2..32 per CU, with or without non-temporal hints
+        for (int rep = 0; rep < 6 && rc == WG_OK; ++rep) {
+            float t = 0.0f;
+            const hipError_t e0 = hipEventRecord(ev[0], ctx->stream);
+            hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, ctx->stream, (const wg_u32x4*)a,
+                               (wg_u32x4*)b, n);
+            const hipError_t e1 = hipEventRecord(ev[1], ctx->stream);
+            if (e0 != hipSuccess || e1 != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess)
+                rc = WG_E_HIP;
+            else if (rep > 0 && (best == 0.0f || t < best)) // the first run warms up
+                best = t;
+        }
+    }
+    if (ev[0])
+        (void)hipEventDestroy(ev[0]);
+    if (ev[1])
+        (void)hipEventDestroy(ev[1]);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (rc == WG_OK)
+        *gb_per_s = 2.0 * (double)(n * 16) / ((double)best * 1e-3) / 1e9;
+    return rc;
 }
 
 extern "C" int wg_sync(wg_ctx* ctx)
@@ -524,6 +588,50 @@ static int upload(wg_ctx* ctx, T** dptr, const T* host, uint64_t n)
         WG_HIP(ctx, hipMemcpy(*dptr, host, n * sizeof(T), hipMemcpyHostToDevice));
     return WG_OK;
 }
+
+// First tile starting at or after x/8 of the edges, x = 0..8.
+void wg_tile_split_points(const std::vector<uint32_t>& te, uint64_t E, uint32_t start[9])
+{
+    const uint32_t ntiles = (uint32_t)(te.size() - 1);
+    for (int x = 0; x <= 8; ++x)
+        start[x] = (x == 8) ? ntiles
+                            : (uint32_t)(std::lower_bound(te.begin(), te.end() - 1, (uint32_t)(E * (uint64_t)x / 8)) -
+                                         te.begin());
+}
+
+// Tilings with 4 and 16 edges per lane for the PageRank shapes, from the
+// device offsets (in device order).
+int wg_build_pr_tilings(wg_ctx* ctx)
+{
+    if (ctx->pr_alt_built)
+        return WG_OK;
+    std::vector<uint32_t> hoff(ctx->V + 1);
+    WG_HIP(ctx, hipMemcpy(hoff.data(), ctx->off, (ctx->V + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t max_tiles = ctx->ntiles;
+    const uint32_t ipts[2] = {4, 16};
+    for (int i = 0; i < 2; ++i) {
+        std::vector<uint32_t> tv, te, hv, ht;
+        wg_build_tiles(hoff.data(), ctx->V, WG_WAVE * ipts[i], tv, te, hv, ht);
+        wg_tiling& t = ctx->pr_alt[i];
+        t.ntiles = (uint32_t)(tv.size() - 1);
+        t.nheavy = (uint32_t)hv.size();
+        wg_tile_split_points(te, ctx->E, t.split);
+        max_tiles = std::max(max_tiles, t.ntiles);
+        int rc;
+        if ((rc = upload(ctx, &t.tile_v, tv.data(), tv.size())) || (rc = upload(ctx, &t.tile_e, te.data(), te.size())) ||
+            (rc = upload(ctx, &t.heavy_v, hv.data(), hv.size())) || (rc = upload(ctx, &t.heavy_t, ht.data(), ht.size())))
+            return rc;
+    }
+    if (max_tiles > ctx->ntiles) { // heavy-tile partials for the finest tiling
+        free_dev((void**)&ctx->tile_part);
+        int rc = upload<double>(ctx, &ctx->tile_part, nullptr, (uint64_t)max_tiles + 1);
+        if (rc)
+            return rc;
+    }
+    ctx->pr_alt_built = 1;
+    return WG_OK;
+}
+
 
 // Relabelling pays off when a few vertices hold many edges (their values are
 // gathered over and over and end up in one place); on near-regular graphs such
@@ -772,6 +880,7 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
     ctx->bm_words = (V + 63) / 64;
     ctx->wedge_words = (((E + (1ull << WG_MIN_BLOCK_LOG2) - 1) >> WG_MIN_BLOCK_LOG2) + 31) / 32 + 1;
     ctx->pr_grid = wg_grid(ctx, ctx->ntiles, WG_WPB, 8);
+    wg_tile_split_points(te, E, ctx->pr_split8);
     ctx->heavy_cap = (uint32_t)(E / 32 + E / 4096 + 16); // bound on deferred out-edge chunks per build
     ctx->span_grid = wg_grid(ctx, ctx->nheavy, 4, 4);
     if ((rc = upload<uint64_t>(ctx, &ctx->bm_cur, nullptr, ctx->bm_words)) ||
@@ -780,7 +889,7 @@ extern "C" int wg_load_csc(wg_ctx* ctx, uint64_t V, uint64_t E, const uint32_t* 
         (rc = upload<uint32_t>(ctx, &ctx->wedge, nullptr, ctx->wedge_words)) ||
         (rc = upload<uint32_t>(ctx, &ctx->heavy_list, nullptr, 2 * (uint64_t)ctx->heavy_cap)) ||
         (rc = upload<double>(ctx, &ctx->tile_part, nullptr, ctx->ntiles + 1)) ||
-        (rc = upload<double>(ctx, &ctx->blk_part, nullptr, (uint64_t)ctx->num_cu * 16 + 16)))
+        (rc = upload<double>(ctx, &ctx->blk_part, nullptr, (uint64_t)ctx->num_cu * 20 + 32)))
         goto fail;
     if ((rc = wg_clear_u64(ctx, ctx->bm_cur, ctx->bm_words)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
         (rc = wg_clear_u64(ctx, ctx->bm_visited, ctx->bm_words)) || (rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0)))

@@ -116,6 +116,13 @@ struct wg_prof {
     int* kid;         // cap
 };
 
+// Edge tiles of one tile shape (see build_tiles in wg_ctx.hip).
+struct wg_tiling {
+    uint32_t ntiles, nheavy;
+    uint32_t *tile_v, *tile_e, *heavy_v, *heavy_t;
+    uint32_t split[9];        // edge-balanced tile ranges of the 8 XCDs
+};
+
 struct wg_ctx {
     int device;
     hipStream_t stream;
@@ -160,6 +167,12 @@ struct wg_ctx {
     double* blk_part;         // PR: per-block partial sums for S
     uint32_t pr_grid;         // blocks of the tile kernel
     uint32_t span_grid;       // blocks of the span kernel
+    // PageRank tile shape (wg_set_pr_shape / wg_pr_autotune): edges per lane
+    // 4, 8 (the tiles above) or 16, workgroups per CU, XCD split
+    int pr_ipt, pr_bpc, pr_xcd;
+    wg_tiling pr_alt[2];      // tilings with 4 and 16 edges per lane (built on first use)
+    int pr_alt_built;
+    uint32_t pr_split8[9];    // XCD split of the default tiles
     uint64_t bm_words;        // 64-bit words per vertex bitmask
     uint64_t* bm_cur;         // BFS: current frontier
     uint64_t* bm_next;        // activated by the running pull
@@ -287,3 +300,7 @@ void wg_build_tiles(const uint32_t* off, uint64_t V, uint32_t tile_edges, std::v
 void wg_free_members(wg_ctx* ctx);
 // releases the loaded graph and everything built on it
 void wg_drop_graph(wg_ctx* ctx);
+// XCD split points of a tiling (first tile at or after x/8 of the edges)
+void wg_tile_split_points(const std::vector<uint32_t>& te, uint64_t E, uint32_t start[9]);
+// builds the 4- and 16-edges-per-lane PageRank tilings once per graph
+int wg_build_pr_tilings(wg_ctx* ctx);

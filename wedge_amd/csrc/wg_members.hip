@@ -277,11 +277,7 @@ static int build_members(wg_ctx* ctx, wg_members* m, const uint64_t* first, uint
         shape_tiles& t = m->tiles[i];
         t.ntiles = (uint32_t)(tv.size() - 1);
         t.nheavy = (uint32_t)hv.size();
-        for (int x = 0; x <= WG_NXCD; ++x) { // first tile starting at or after x/8 of the edges
-            const uint64_t target = ctx->E * (uint64_t)x / WG_NXCD;
-            t.split.start[x] = (x == WG_NXCD) ? t.ntiles
-                                              : (uint32_t)(std::lower_bound(te.begin(), te.end() - 1, (uint32_t)target) - te.begin());
-        }
+        wg_tile_split_points(te, ctx->E, t.split.start);
         max_tiles = std::max(max_tiles, t.ntiles);
         if ((rc = upload_vec(ctx, &t.tile_v, tv)) || (rc = upload_vec(ctx, &t.tile_e, te)) ||
             (rc = upload_vec(ctx, &t.heavy_v, hv)) || (rc = upload_vec(ctx, &t.heavy_t, ht)))

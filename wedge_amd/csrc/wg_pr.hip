@@ -124,6 +124,55 @@ extern "C" int wg_init_pr(wg_ctx* ctx, double damping, int contrib_fp32)
     return WG_OK;
 }
 
+// The tiling and launch shape of the PageRank edge pull.
+struct pr_shape {
+    const uint32_t *tile_v, *tile_e, *heavy_v, *heavy_t;
+    uint32_t ntiles, nheavy, grid, sgrid;
+    wg_tile_split split;
+};
+
+static pr_shape current_shape(const wg_ctx* ctx)
+{
+    pr_shape s;
+    const uint32_t* split = ctx->pr_split8;
+    if (ctx->pr_ipt == WG_IPT) {
+        s.tile_v = ctx->tile_v;
+        s.tile_e = ctx->tile_e;
+        s.heavy_v = ctx->heavy_v;
+        s.heavy_t = ctx->heavy_t;
+        s.ntiles = ctx->ntiles;
+        s.nheavy = ctx->nheavy;
+    } else {
+        const wg_tiling& t = ctx->pr_alt[ctx->pr_ipt == 4 ? 0 : 1];
+        s.tile_v = t.tile_v;
+        s.tile_e = t.tile_e;
+        s.heavy_v = t.heavy_v;
+        s.heavy_t = t.heavy_t;
+        s.ntiles = t.ntiles;
+        s.nheavy = t.nheavy;
+        split = t.split;
+    }
+    s.grid = wg_grid(ctx, s.ntiles, WG_WPB, (uint32_t)ctx->pr_bpc);
+    s.sgrid = wg_grid(ctx, s.nheavy, 4, 4);
+    s.split.on = ctx->pr_xcd ? 1u : 0u;
+    for (int x = 0; x <= WG_NXCD; ++x)
+        s.split.start[x] = split[x];
+    if (s.split.on) // whole rows of 8 workgroups, one per XCD
+        s.grid = (s.grid + WG_NXCD - 1) / WG_NXCD * WG_NXCD;
+    return s;
+}
+
+template <typename T, int MODE, int IPT>
+static void pr_launch_shape(wg_ctx* ctx, const pr_shape& s, const T* cin, T* cout, double* S_in, double* S_out, double N)
+{
+    hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT>), dim3(s.grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, cin,
+                       cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles, S_in, ctx->damping, N, ctx->tile_part,
+                       ctx->blk_part, s.split);
+    hipLaunchKernelGGL((k_pr_span<T, MODE, IPT>), dim3(s.sgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, s.heavy_v,
+                       s.heavy_t, s.nheavy, ctx->tile_part, cout, ctx->acc, ctx->od, S_in, ctx->damping, N, ctx->blk_part,
+                       s.grid, S_out, &ctx->ctr->ticket);
+}
+
 template <typename T>
 static int pr_launch(wg_ctx* ctx, int mode)
 {
@@ -136,32 +185,120 @@ static int pr_launch(wg_ctx* ctx, int mode)
         S_out = &ctx->ctr->total_rank;
         cout = nullptr;
     }
-    int slot = wg_prof_begin(ctx, mode == 0 ? WG_K_PR_TILE : WG_K_PR_ACC);
-    if (mode == 0)
-        hipLaunchKernelGGL((k_pr_tile<T, 0>), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, cin,
-                           cout, ctx->acc, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, S_in, ctx->damping, N,
-                           ctx->tile_part, ctx->blk_part);
-    else
-        hipLaunchKernelGGL((k_pr_tile<T, 1>), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, cin,
-                           cout, ctx->acc, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, S_in, ctx->damping, N,
-                           ctx->tile_part, ctx->blk_part);
-    int rc = wg_check_launch(ctx);
+    const pr_shape s = current_shape(ctx);
+    // one bracket around tile + span: the iteration's device time
+    const int slot = wg_prof_begin(ctx, mode == 0 ? WG_K_PR_TILE : WG_K_PR_ACC);
+    switch (ctx->pr_ipt * 2 + mode) {
+    case 8: pr_launch_shape<T, 0, 4>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 9: pr_launch_shape<T, 1, 4>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 16: pr_launch_shape<T, 0, 8>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 17: pr_launch_shape<T, 1, 8>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 32: pr_launch_shape<T, 0, 16>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 33: pr_launch_shape<T, 1, 16>(ctx, s, cin, cout, S_in, S_out, N); break;
+    default: return WG_E_STATE;
+    }
+    const int rc = wg_check_launch(ctx);
+    wg_prof_end(ctx, mode == 0 ? WG_K_PR_TILE : WG_K_PR_ACC, slot);
+    return rc;
+}
+
+extern "C" int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape)
+{
+    if (!ctx)
+        return WG_E_INVAL;
+    if (!ctx->off)
+        return WG_E_STATE;
+    if (!shape) {
+        ctx->pr_ipt = WG_IPT;
+        ctx->pr_bpc = 8;
+        ctx->pr_xcd = 0;
+        return WG_OK;
+    }
+    const uint32_t ipt = shape->edges_per_lane;
+    if ((ipt != 4 && ipt != 8 && ipt != 16) || shape->blocks_per_cu < 1 || shape->blocks_per_cu > 16 ||
+        shape->xcd_split > 1)
+        return WG_E_INVAL;
+    if (ipt != WG_IPT) {
+        WG_HIP(ctx, hipSetDevice(ctx->device));
+        WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int rc = wg_build_pr_tilings(ctx);
+        if (rc)
+            return rc;
+    }
+    ctx->pr_ipt = (int)ipt;
+    ctx->pr_bpc = (int)shape->blocks_per_cu;
+    ctx->pr_xcd = (int)shape->xcd_split;
+    return WG_OK;
+}
+
+extern "C" int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape)
+{
+    if (!ctx || !shape)
+        return WG_E_INVAL;
+    shape->edges_per_lane = (uint32_t)(ctx->pr_ipt ? ctx->pr_ipt : WG_IPT);
+    shape->blocks_per_cu = (uint32_t)(ctx->pr_bpc ? ctx->pr_bpc : 8);
+    shape->xcd_split = (uint32_t)ctx->pr_xcd;
+    return WG_OK;
+}
+
+extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* best, float* ms_per_shape)
+{
+    if (!ctx || iterations == 0)
+        return WG_E_INVAL;
+    if (ctx->app != WG_APP_PR)
+        return WG_E_STATE;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    const double damping = ctx->damping;
+    const int fp32 = ctx->pr_fp32;
+    hipEvent_t ev[2];
+    WG_HIP(ctx, hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return WG_E_HIP;
+    }
+    int rc = WG_OK;
+    float best_ms = 0.0f;
+    wg_batch_shape best_s = {WG_IPT, 8, 0};
+    for (uint32_t i = 0; i < WG_BATCH_NSHAPES && rc == WG_OK; ++i) {
+        wg_batch_shape s;
+        wg_batch_shape_at(i, &s);
+        if ((rc = wg_set_pr_shape(ctx, &s)) || (rc = wg_pr_iteration(ctx, nullptr))) // warm-up
+            break;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && rc == WG_OK; ++rep) {
+            if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) {
+                rc = WG_E_HIP;
+                break;
+            }
+            for (uint32_t it = 0; it < iterations && rc == WG_OK; ++it)
+                rc = wg_pr_iteration(ctx, nullptr);
+            float t = 0.0f;
+            if (rc || hipEventRecord(ev[1], ctx->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess) {
+                rc = rc ? rc : WG_E_HIP;
+                break;
+            }
+            t /= (float)iterations;
+            ms = (rep == 0 || t < ms) ? t : ms;
+        }
+        if (rc)
+            break;
+        if (ms_per_shape)
+            ms_per_shape[i] = ms;
+        if (i == 0 || ms < best_ms) {
+            best_ms = ms;
+            best_s = s;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
     if (rc)
         return rc;
-    wg_prof_end(ctx, WG_K_PR_TILE, slot);
-    slot = wg_prof_begin(ctx, WG_K_PR_SPAN);
-    if (mode == 0)
-        hipLaunchKernelGGL((k_pr_span<T, 0>), dim3(ctx->span_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->heavy_v,
-                           ctx->heavy_t, ctx->nheavy, ctx->tile_part, cout, ctx->acc, ctx->od, S_in, ctx->damping, N,
-                           ctx->blk_part, ctx->pr_grid, S_out, &ctx->ctr->ticket);
-    else
-        hipLaunchKernelGGL((k_pr_span<T, 1>), dim3(ctx->span_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->heavy_v,
-                           ctx->heavy_t, ctx->nheavy, ctx->tile_part, cout, ctx->acc, ctx->od, S_in, ctx->damping, N,
-                           ctx->blk_part, ctx->pr_grid, S_out, &ctx->ctr->ticket);
-    rc = wg_check_launch(ctx);
-    if (rc)
+    // the timed iterations advanced the state: start again from the uniform ranks
+    if ((rc = wg_set_pr_shape(ctx, &best_s)) || (rc = wg_init_pr(ctx, damping, fp32)))
         return rc;
-    wg_prof_end(ctx, WG_K_PR_SPAN, slot);
+    if (best)
+        *best = best_s;
     return WG_OK;
 }
 
