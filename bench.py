@@ -14,7 +14,8 @@ The same run reports:
   apps      BFS and SSSP from the max-out-degree root under the plain
             threshold rule and under the learned schedule (the packaged
             model), on this graph and on a 4096x6144 road grid (integer
-            weights, seed 3): GTEPS, learned/plain speedup, identical outputs;
+            weights, seed 3), and CC on the symmetrised graph: GTEPS,
+            learned/plain speedup, identical outputs;
   batched   64-source BFS and SSSP and 32-seed personalised PageRank on this
             graph against one source at a time, and 256 uneven R-MAT graphs
             (scale U{10..16}, edge factor U{8..32}, seed 4) as one batch
@@ -25,8 +26,9 @@ The same run reports:
             and learned), each with its HBM roofline fraction, and the L1
             distance of the FP32 ranks from the FP64 ranks;
   cpu_baseline  the oracle (a C restatement of Wedge's pull engine) timed on
-            a bounded PageRank sample on the host cores; its ranks also give
-            the L1 error of the HIP path's FP64 and FP32 ranks.
+            a bounded PageRank sample on the host cores, and its BFS from the
+            same root; its ranks also give the L1 error of the HIP path's FP64
+            and FP32 ranks.
 
 Prints one JSON line.
 """
@@ -317,7 +319,37 @@ def host_cpu_info():
             "usable": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(g, damping, budget_s=20.0):
+def cc_pair(scale, edge_factor, seed, reps=2):
+    """CC on the symmetrised R-MAT (the reference's CC input) under the plain
+    rule and the learned schedule: run time, iterations, reference-style
+    E * iterations / time, identical labels."""
+    t0 = time.time()
+    g = W.rmat(scale, edge_factor, seed=seed, symmetrize=True)
+    log(f"rmat s{scale} ef{edge_factor} seed {seed} symmetrised: V={g.num_vertices} E={g.num_edges} "
+        f"({time.time() - t0:.1f} s)")
+    ctx = W.Context(0)
+    ctx.load(g)
+    out = {"vertices": g.num_vertices, "edges": g.num_edges}
+    res = {}
+    for sched in ("plain", "learned"):
+        best = None
+        for _ in range(reps):
+            lab, st = ctx.run_cc(schedule=sched)
+            if best is None or st["running_ms"] < best[1]["running_ms"]:
+                best = (lab, st)
+        res[sched] = best[0]
+        st = best[1]
+        out[f"cc_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
+                              "frontier_iterations": st["frontier_iterations"],
+                              "bedges_per_s": round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2)}
+    out["cc_learned_vs_plain"] = round(out["cc_plain"]["ms"] / out["cc_learned"]["ms"], 3)
+    out["cc_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
+    ctx.close()
+    log(f"cc: {out}")
+    return out
+
+
+def cpu_baseline(g, damping, budget_s=20.0, root=None):
     """The oracle's PageRank on the same graph, all host threads, a bounded
     number of iterations (time per iteration -> GTEPS)."""
     import oracle
@@ -332,6 +364,10 @@ def cpu_baseline(g, damping, budget_s=20.0):
     ref, st = orc.pr(iters, damping)
     ms = st["running_ms"] / iters
     threads = orc.threads
+    bfs = None
+    if root is not None:  # the oracle's BFS from the bench's root, same threads
+        _, _, bst = orc.bfs(root)
+        bfs = {"ms": round(bst["running_ms"], 2), "iterations": int(bst["iterations"])}
     orc.close()
     # the oracle's ranks also check the HIP path's ranks after the same iterations
     errs = {}
@@ -351,6 +387,7 @@ def cpu_baseline(g, damping, budget_s=20.0):
         "sample": f"oracle PageRank, {iters} iteration(s) on the same graph, {threads} threads, "
                   f"{ms:.1f} ms/iteration (layout build {build_s:.1f} s not timed)",
         "host": host_cpu_info(),
+        "bfs": bfs,
         "pr_l1_hip_vs_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
         "pr_iterations_compared": iters,
     }
@@ -460,6 +497,7 @@ def main():
         W.mlp_load()
         root = g.max_outdegree_vertex()
         out["apps"] = {"rmat": schedule_pair(ctx, g, root, args.bfs_reps, "f64")}
+        out["apps"]["rmat_symmetrised_cc"] = None  # filled after the road grid (own context)
         if not args.no_batch:
             out["batched"] = batched_sources(ctx, g, ms_per_step, args.damping)
         if not args.no_grid:
@@ -469,13 +507,18 @@ def main():
             log(f"road grid 4096x6144 seed 3: V={road.num_vertices} E={road.num_edges} ({time.time() - t0:.1f} s)")
             out["apps"]["road_grid_4096x6144"] = schedule_pair(ctx, road, road.max_outdegree_vertex(), 2, "u32")
     ctx.close()
+    if not args.no_apps:
+        if cfg["scale"] <= 24:  # the symmetrised scale-27 graph would pass 2^32 edges
+            out["apps"]["rmat_symmetrised_cc"] = cc_pair(cfg["scale"], cfg["edge_factor"], cfg["seed"])
+        else:
+            del out["apps"]["rmat_symmetrised_cc"]
     if not args.no_batch and not args.no_batch_graphs:
         out.setdefault("batched", {})["graphs_256"] = batched_graphs(args.damping)
     if not args.no_s27:
         out["large"] = large_graph(args.damping)
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, args.damping)
+        out["cpu_baseline"] = cpu_baseline(g, args.damping, root=None if args.no_apps else g.max_outdegree_vertex())
         log(f"CPU baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
