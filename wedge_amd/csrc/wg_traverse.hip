@@ -79,15 +79,12 @@ int wg_reset_counters(wg_ctx* ctx, unsigned int flags)
 // up to WG_BFS_SEQ in-edges of its own vertex; vertices with more edges are
 // finished by the whole wave, 64 edges per step, stopping at the first hit.
 // Vertices with more than WG_BFS_HEAVY in-edges are left to k_bfs_heavy.
-__global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
-                                                         const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
-                                                         const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
-                                                         uint64_t* __restrict__ packed, uint32_t V,
-                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
-                                                         int gate)
+__device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                               const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                               const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                               uint64_t* __restrict__ packed, uint32_t V, const uint32_t* __restrict__ perm,
+                               wg_dev_counters* __restrict__ ctr)
 {
-    if (wg_gated_off(ctr, gate))
-        return;
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
@@ -158,6 +155,18 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
     add_tally(act, ctr);
 }
 
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                                                         const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                                         uint64_t* __restrict__ packed, uint32_t V,
+                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
+                                                         int gate)
+{
+    if (wg_gated_off(ctr, gate))
+        return;
+    bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr);
+}
+
 // Dense level for one chunk of WG_BFS_HEAVY in-edges of a high in-degree vertex.
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
                                                          uint32_t n, const uint32_t* __restrict__ off,
@@ -212,16 +221,13 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
 }
 
 // Frontier-mode BFS level over the marked edge blocks.
-__global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
-                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
-                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
-                                                          uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
-                                                          const uint32_t* __restrict__ perm, int gate)
+__device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                                const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
+                                const uint32_t* __restrict__ perm)
 {
-    if (wg_gated_off(ctr, gate))
-        return;
-    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     act_tally act = { 0ull, 0ull, 0ull };
@@ -250,14 +256,24 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __rest
     add_tally(act, ctr);
 }
 
-// End of a BFS level: visited |= next, the new frontier replaces the old one
-// (copied, so kernels need no pointer swap), next is cleared, the new
-// frontier is counted and the level advances.
-__global__ void k_bfs_advance(uint64_t* __restrict__ visited, uint64_t* __restrict__ next, uint64_t* __restrict__ cur,
-                              uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
+                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                                          uint64_t* __restrict__ packed, uint32_t* __restrict__ wedge,
+                                                          const uint32_t* __restrict__ perm, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
+    bfs_blocks_body(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
+}
+
+// End of a BFS level: visited |= next, the new frontier replaces the old one
+// (copied, so kernels need no pointer swap), next is cleared, the new
+// frontier is counted and the level advances.
+__device__ void bfs_advance_body(uint64_t* __restrict__ visited, uint64_t* __restrict__ next, uint64_t* __restrict__ cur,
+                                 uint64_t nwords, wg_dev_counters* __restrict__ ctr)
+{
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t n = next[w];
@@ -277,10 +293,16 @@ __global__ void k_bfs_advance(uint64_t* __restrict__ visited, uint64_t* __restri
         atomicAdd(&ctr->level, 1u);
 }
 
-__global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
+__global__ void k_bfs_advance(uint64_t* __restrict__ visited, uint64_t* __restrict__ next, uint64_t* __restrict__ cur,
+                              uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
+    bfs_advance_body(visited, next, cur, nwords, ctr);
+}
+
+__device__ void count_clear_body(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev_counters* __restrict__ ctr)
+{
     unsigned long long cnt = 0;
     for (uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; w < nwords; w += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t b = bm[w];
@@ -290,6 +312,13 @@ __global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev
         }
     }
     add_count(cnt, ctr, WG_CNT_FRONTIER);
+}
+
+__global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev_counters* __restrict__ ctr, int gate)
+{
+    if (wg_gated_off(ctr, gate))
+        return;
+    count_clear_body(bm, nwords, ctr);
 }
 
 // ---- CC / SSSP (min propagation) -------------------------------------------------
@@ -360,15 +389,12 @@ __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint
 // meet through atomics (the reference's idempotent merge,
 // source/phases.c:54-65).
 template <class P>
-__global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
-                                                        const void* __restrict__ w, typename P::T* __restrict__ prop,
-                                                        const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
-                                                        const uint32_t* __restrict__ tile_e, uint32_t ntiles,
-                                                        uint64_t* __restrict__ next, wg_dev_counters* __restrict__ ctr,
-                                                        unsigned int* __restrict__ ovf, int gate)
+__device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                              const void* __restrict__ w, typename P::T* __restrict__ prop,
+                              const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
+                              const uint32_t* __restrict__ tile_e, uint32_t ntiles, uint64_t* __restrict__ next,
+                              wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf)
 {
-    if (wg_gated_off(ctr, gate))
-        return;
     typedef typename P::T T;
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
     constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
@@ -464,18 +490,27 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
     add_tally(act, ctr);
 }
 
-// Frontier-mode pull over the marked edge blocks.
 template <class P>
-__global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
-                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
-                                                          const void* __restrict__ w, typename P::T* __restrict__ prop,
-                                                          const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
-                                                          uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf,
-                                                          int gate)
+__global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                        const void* __restrict__ w, typename P::T* __restrict__ prop,
+                                                        const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
+                                                        const uint32_t* __restrict__ tile_e, uint32_t ntiles,
+                                                        uint64_t* __restrict__ next, wg_dev_counters* __restrict__ ctr,
+                                                        unsigned int* __restrict__ ovf, int gate)
 {
     if (wg_gated_off(ctr, gate))
         return;
-    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
+    min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf);
+}
+
+// Frontier-mode pull over the marked edge blocks.
+template <class P>
+__device__ void min_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
+                                const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                const void* __restrict__ w, typename P::T* __restrict__ prop,
+                                const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
+                                uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf)
+{
     typedef typename P::T T;
     act_tally act = { 0ull, 0ull, 0ull };
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool) {
@@ -486,6 +521,19 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __rest
         }
     });
     add_tally(act, ctr);
+}
+
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
+                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
+                                                          const void* __restrict__ w, typename P::T* __restrict__ prop,
+                                                          const uint32_t* __restrict__ od, uint64_t* __restrict__ next,
+                                                          uint32_t* __restrict__ wedge, unsigned int* __restrict__ ovf,
+                                                          int gate)
+{
+    if (wg_gated_off(ctr, gate))
+        return;
+    min_blocks_body<P>(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
 }
 
 // ---- Wedge frontier ---------------------------------------------------------------
@@ -539,15 +587,10 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
 }
 
 // Out-edge chunks deferred by k_wedge_build: one wave per chunk.
-__global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy, uint32_t heavy_cap,
-                                                           const uint32_t* __restrict__ csri_off,
-                                                           const uint32_t* __restrict__ csri_pos, int blog2_arg,
-                                                           uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr,
-                                                           int gate)
+__device__ void wedge_heavy_body(const uint32_t* __restrict__ heavy, uint32_t heavy_cap,
+                                 const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos, int blog2,
+                                 uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr)
 {
-    if (wg_gated_off(ctr, gate))
-        return;
-    const int blog2 = gate < 0 ? blog2_arg : ctr->blog2;
     const uint32_t n = ctr->heavy_len < heavy_cap ? ctr->heavy_len : heavy_cap;
     const uint32_t nwaves = gridDim.x * WG_WPB;
     for (uint32_t i = blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; i < n; i += nwaves) {
@@ -557,6 +600,17 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __rest
         for (uint32_t p = s + (uint32_t)wg_lane(); p < e; p += WG_WAVE)
             mark_block(csri_pos[p] >> blog2, wedge);
     }
+}
+
+__global__ __launch_bounds__(WG_BLOCK) void k_wedge_heavy(const uint32_t* __restrict__ heavy, uint32_t heavy_cap,
+                                                           const uint32_t* __restrict__ csri_off,
+                                                           const uint32_t* __restrict__ csri_pos, int blog2_arg,
+                                                           uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr,
+                                                           int gate)
+{
+    if (wg_gated_off(ctr, gate))
+        return;
+    wedge_heavy_body(heavy, heavy_cap, csri_off, csri_pos, gate < 0 ? blog2_arg : ctr->blog2, wedge, ctr);
 }
 
 // source/wedge.c:52-59
@@ -724,9 +778,9 @@ __global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, u
 
 // End of a pull (one wavefront): fold the counts, account the iteration,
 // stop at zero activity, choose the next schedule.
-__global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
+__device__ void decide_wave(wg_dev_counters* c, const float* __restrict__ mlp)
 {
-    if (blockIdx.x != 0 || threadIdx.x >= WG_WAVE || c->done)
+    if (c->done)
         return;
     const unsigned long long act0 = c->activity, nact0 = c->activated, fresh0 = c->fresh;
     const unsigned int iters = c->iters + 1u, fiters = c->fiters + (c->mode == WG_MODE_FRONTIER ? 1u : 0u);
@@ -760,6 +814,72 @@ __global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
     }
     c->mode = (unsigned int)mode;
     c->blog2 = blog2;
+}
+
+__global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
+{
+    if (blockIdx.x == 0 && threadIdx.x < WG_WAVE)
+        decide_wave(c, mlp);
+}
+
+// One level of the device-resident loop in four launches: the pull the
+// control block selected (dense or frontier, one kernel), the decision for
+// the next level, the frontier build for it, and the advance (with the hub
+// out-lists the build deferred).  Every launch costs a dispatch, gated off or
+// not (consecutive kernels start about 8 us apart on this device), which
+// dominates levels with small frontiers.  The decision stays a kernel of its
+// own: taking it in the last workgroup of the pull needs device-scope fences
+// in every wave, which cost more than the launch.
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ dst, const uint32_t* __restrict__ od,
+                                                         const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
+                                                         uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
+                                                         uint32_t V, uint32_t E, uint32_t* __restrict__ wedge,
+                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr)
+{
+    if (wg_gated_off(ctr, WG_GATE_LIVE))
+        return;
+    if (ctr->mode == WG_MODE_DENSE)
+        bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr);
+    else
+        bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
+}
+
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ dst, const void* __restrict__ w,
+                                                         typename P::T* __restrict__ prop, const uint32_t* __restrict__ od,
+                                                         const uint32_t* __restrict__ tile_v,
+                                                         const uint32_t* __restrict__ tile_e, uint32_t ntiles, uint32_t E,
+                                                         uint64_t* __restrict__ next, uint32_t* __restrict__ wedge,
+                                                         wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf)
+{
+    if (wg_gated_off(ctr, WG_GATE_LIVE))
+        return;
+    if (ctr->mode == WG_MODE_DENSE)
+        min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf);
+    else
+        min_blocks_body<P>(ctr, ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
+}
+
+// Advance (BFS) or count-and-clear (CC, SSSP) of the activated vertices, and
+// first the hub out-lists the frontier build deferred (those marks are read
+// only by the next pull).
+__global__ __launch_bounds__(WG_BLOCK) void k_level_advance(uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                                             uint64_t* __restrict__ cur, uint64_t nwords, int bfs,
+                                                             const uint32_t* __restrict__ heavy, uint32_t heavy_cap,
+                                                             const uint32_t* __restrict__ csri_off,
+                                                             const uint32_t* __restrict__ csri_pos,
+                                                             uint32_t* __restrict__ wedge, wg_dev_counters* __restrict__ ctr)
+{
+    if (wg_gated_off(ctr, WG_GATE_LIVE))
+        return;
+    if (ctr->mode == WG_MODE_FRONTIER)
+        wedge_heavy_body(heavy, heavy_cap, csri_off, csri_pos, ctr->blog2, wedge, ctr);
+    if (bfs)
+        bfs_advance_body(visited, next, cur, nwords, ctr);
+    else
+        count_clear_body(next, nwords, ctr);
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -1213,59 +1333,45 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
     return WG_OK;
 }
 
-// One iteration of the device-resident loop: every kernel is enqueued, the
-// gates pick the ones that run.
+// One iteration of the device-resident loop: [dense pull of the hub
+// destinations, gated], the level pull, the decision, the frontier build
+// (gated) and the advance (with the build's deferred hub out-lists).
 static int enqueue_iteration(wg_ctx* ctx)
 {
     uint64_t* packed = (uint64_t*)ctx->prop[0];
     const uint32_t qgrid = wg_frontier_grid(ctx);
+    const float* mlp = ctx->mlp_set ? ctx->mlp : nullptr;
     if (ctx->app == WG_APP_BFS) {
-        hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
-                           ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_DENSE);
         if (ctx->nbfs_heavy)
             hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                                ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
                                ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
                                WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_bfs_blocks, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
-                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           ctx->wedge, ctx->perm, WG_GATE_FRONTIER);
-    } else if (ctx->app == WG_APP_CC) {
-        hipLaunchKernelGGL(k_min_tile<PolCC>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
-                           (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
-                           &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolCC>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
-                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
-                           ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
-    } else if (ctx->wkind == WG_WEIGHT_F64) {
-        hipLaunchKernelGGL(k_min_tile<PolSF>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
-                           (unsigned long long*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
-                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolSF>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
-                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (unsigned long long*)ctx->prop[0], ctx->od,
-                           ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
+        const uint32_t dgrid = wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8);
+        hipLaunchKernelGGL(k_bfs_level, dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
+                           ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
+                           (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr);
     } else {
-        hipLaunchKernelGGL(k_min_tile<PolSU>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
-                           (uint32_t*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next, ctx->ctr,
-                           &ctx->ctr->overflow, WG_GATE_DENSE);
-        hipLaunchKernelGGL(k_min_blocks<PolSU>, dim3(qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr, 0,
-                           (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (uint32_t*)ctx->prop[0], ctx->od, ctx->bm_next,
-                           ctx->wedge, &ctx->ctr->overflow, WG_GATE_FRONTIER);
+        const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
+#define WG_MIN_LEVEL(P, T)                                                                                            \
+    hipLaunchKernelGGL(k_min_level<P>, dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, ctx->w, \
+                       (T*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, (uint32_t)ctx->E, ctx->bm_next,   \
+                       ctx->wedge, ctx->ctr, &ctx->ctr->overflow)
+        if (ctx->app == WG_APP_CC)
+            WG_MIN_LEVEL(PolCC, uint32_t);
+        else if (ctx->wkind == WG_WEIGHT_F64)
+            WG_MIN_LEVEL(PolSF, unsigned long long);
+        else
+            WG_MIN_LEVEL(PolSU, uint32_t);
+#undef WG_MIN_LEVEL
     }
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->mlp_set ? ctx->mlp : nullptr);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, mlp);
     hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
                        ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->heavy_cap, ctx->ctr,
                        WG_GATE_FRONTIER);
-    hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                       ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
-    const uint32_t agrid = wg_advance_grid(ctx);
-    if (ctx->app == WG_APP_BFS)
-        hipLaunchKernelGGL(k_bfs_advance, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_visited, ctx->bm_next,
-                           ctx->bm_cur, ctx->bm_words, ctx->ctr, WG_GATE_LIVE);
-    else
-        hipLaunchKernelGGL(k_count_clear, dim3(agrid), dim3(256), 0, ctx->stream, ctx->bm_next, ctx->bm_words, ctx->ctr,
-                           WG_GATE_LIVE);
+    hipLaunchKernelGGL(k_level_advance, dim3(wg_advance_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_visited,
+                       ctx->bm_next, ctx->bm_cur, ctx->bm_words, ctx->app == WG_APP_BFS ? 1 : 0, ctx->heavy_list,
+                       ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, ctx->wedge, ctx->ctr);
     return wg_check_launch(ctx);
 }
 
