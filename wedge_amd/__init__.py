@@ -153,6 +153,8 @@ GPU_SYMBOLS = [
     ("wg_read_sssp", C.c_int, [_p, _p]),
     ("wg_reached", C.c_int, [_p, _u64p, _u64p]),
     ("wg_converge", C.c_int, [_p, _p, _u64p, _u64p]),
+    ("wg_set_fused_levels", C.c_int, [_p, C.c_uint32, C.c_uint32]),
+    ("wg_fused_level_count", C.c_int, [_p, _u64p]),
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
@@ -683,6 +685,20 @@ class Context:
         it, fit = C.c_uint64(), C.c_uint64()
         self._chk(self._lib.wg_converge(self._h, C.byref(p), C.byref(it), C.byref(fit)), "wg_converge")
         return int(it.value), int(fit.value)
+
+    def set_fused_levels(self, workgroups=32, apps=(APP_BFS,)):
+        """Workgroups of the fused small-frontier levels of converge (0 = off)
+        and the applications that use them."""
+        mask = 0
+        for a in apps:
+            mask |= 1 << int(a)
+        self._chk(self._lib.wg_set_fused_levels(self._h, int(workgroups), mask), "wg_set_fused_levels")
+
+    def fused_level_count(self) -> int:
+        """Iterations of the last converge run by fused small-frontier levels."""
+        n = C.c_uint64()
+        self._chk(self._lib.wg_fused_level_count(self._h, C.byref(n)), "wg_fused_level_count")
+        return int(n.value)
 
     def read_bfs(self) -> np.ndarray:
         out = np.empty(self.V, np.uint64)

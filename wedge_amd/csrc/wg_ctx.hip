@@ -383,6 +383,8 @@ extern "C" int wg_open(wg_ctx** out, int device)
         return WG_E_NOMEM;
     }
     memset(ctx->hctr, 0, sizeof(wg_dev_counters));
+    ctx->run_wgs = WG_RUN_DEFAULT_WGS;
+    ctx->run_apps = 1u << WG_APP_BFS;
     *out = ctx;
     return WG_OK;
 }
@@ -445,6 +447,7 @@ static void free_graph(wg_ctx* ctx)
     free_dev((void**)&ctx->bm_visited);
     free_dev((void**)&ctx->wedge);
     free_dev((void**)&ctx->heavy_list);
+    free_dev((void**)&ctx->run_spare);
     free_dev((void**)&ctx->perm);
     free_dev((void**)&ctx->inv);
     ctx->V = ctx->E = 0;
@@ -465,6 +468,8 @@ extern "C" void wg_close(wg_ctx* ctx)
         free(ctx->prof.kid);
     }
     (void)hipFree(ctx->mlp);
+    (void)hipFree(ctx->run_ctl);
+    (void)hipFree(ctx->run_q);
     (void)hipFree(ctx->ctr);
     (void)hipHostFree(ctx->hctr);
     (void)hipStreamDestroy(ctx->stream);

@@ -56,7 +56,7 @@ struct wg_dev_counters {
     unsigned long long activated;       // number of vertices activated this pull
     unsigned long long fresh;           // of which reached for the first time (BFS, SSSP)
     unsigned long long frontier_verts;  // popcount of the frontier made current by advance
-    unsigned int pad1;
+    unsigned int run_err;               // a fused-level run gave up waiting at its grid barrier
     unsigned int heavy_len;             // high out-degree vertices deferred by the frontier build
     unsigned int overflow;              // integer SSSP distance reached 2^32-1
     unsigned int ticket;                // last-block-done ticket (PR span kernel)
@@ -74,7 +74,7 @@ struct wg_dev_counters {
     unsigned int sched;                 // schedule kind (wg_loop_params.schedule)
     int plain_blog2;                    // granularity the plain rule uses
     unsigned int max_iters;             // 0 = no cap
-    unsigned int pad0;
+    unsigned int run_levels;            // iterations run by fused small-frontier runs
     unsigned long long threshold;       // plain rule: dense iff activity > threshold
     unsigned long long last_activity;   // activity of the last completed pull
     unsigned long long last_activated;
@@ -104,6 +104,32 @@ __device__ __forceinline__ bool wg_gated_off(const wg_dev_counters* c, int gate)
         return false;
     return c->mode != (unsigned int)gate;
 }
+
+// Fused small-frontier levels (k_bfs_run): a queue entry is up to WG_RUN_CH
+// out-edges of one vertex; a run continues while a level activates at most
+// WG_RUN_NACT_MAX vertices with at most WG_RUN_ACT_MAX out-edges, so that the
+// next queue (at most activity / WG_RUN_CH + activated entries) fits.
+#define WG_RUN_CH 8u
+#define WG_RUN_ACT_MAX (1ull << 17)
+#define WG_RUN_NACT_MAX (1ull << 17)
+#define WG_RUN_QCAP (1u << 18)
+#define WG_RUN_DEFAULT_WGS 32u
+#define WG_RUN_MAX_WGS 128u
+
+struct __attribute__((aligned(256))) wg_run_line {
+    unsigned long long v;
+    unsigned long long pad[31];
+};
+// Control block of a run: its grid barrier and, by level mod 3, the queue
+// tails and the level's counts (one 256-byte line each).
+struct wg_run_ctl {
+    wg_run_line count;   // barrier arrivals
+    wg_run_line gen;     // barrier generation
+    wg_run_line tail[3];
+    wg_run_line act[3];
+    wg_run_line nact[3];
+    wg_run_line fresh[3];
+};
 
 struct wg_prof {
     int enabled;
@@ -191,6 +217,12 @@ struct wg_ctx {
     uint32_t nroots;
     unsigned long long root_activity; // summed out-degree of the roots
     struct wg_members* members;      // batch of member graphs (wg_set_members), NULL if none
+    // fused small-frontier levels of the device loop (k_bfs_run)
+    uint32_t run_wgs;         // workgroups of a run, 0 = off
+    uint32_t run_apps;        // apps that use runs, bit 1 << WG_APP_*
+    struct wg_run_ctl* run_ctl;
+    uint4* run_q;             // two queues of WG_RUN_QCAP entries
+    uint64_t* run_spare;      // CC / SSSP runs: second activation mask (bm_words), zero between runs
     int iter;                 // iterations run since init
 
     wg_prof prof;

@@ -20,6 +20,13 @@ struct wg_mlp_lane {
     float a1, h1, a2, h2, y;
 };
 
+// Value of x in lane k (k the same in every lane): a scalar read of the lane's
+// register, cheaper than a cross-lane permute.
+__device__ __forceinline__ float wg_lane_value(float x, int k)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k));
+}
+
 __device__ __forceinline__ int wg_mlp_choose_wave(const float* __restrict__ P, int app, const float* x,
                                                   const float* base, int plain, float* y, wg_mlp_lane* keep = nullptr)
 {
@@ -34,13 +41,15 @@ __device__ __forceinline__ int wg_mlp_choose_wave(const float* __restrict__ P, i
         a1 = a1 + P[WG_MLP_OFF_W1 + i * WG_MLP_HIDDEN + lane] * z[i];
     const float h1 = wg_mlp_relu(a1);
     float a2 = P[WG_MLP_OFF_B2 + lane];
+#pragma unroll 16
     for (int k = 0; k < WG_MLP_HIDDEN; ++k)
-        a2 = a2 + P[WG_MLP_OFF_W2 + k * WG_MLP_HIDDEN + lane] * __shfl(h1, k);
+        a2 = a2 + P[WG_MLP_OFF_W2 + k * WG_MLP_HIDDEN + lane] * wg_lane_value(h1, k);
     const float h2 = wg_mlp_relu(a2);
     const int o = lane < WG_MLP_ACTIONS ? lane : WG_MLP_ACTIONS - 1;
     float a3 = P[WG_MLP_OFF_B3 + o];
+#pragma unroll 16
     for (int j = 0; j < WG_MLP_HIDDEN; ++j)
-        a3 = a3 + P[WG_MLP_OFF_W3 + j * WG_MLP_ACTIONS + o] * __shfl(h2, j);
+        a3 = a3 + P[WG_MLP_OFF_W3 + j * WG_MLP_ACTIONS + o] * wg_lane_value(h2, j);
     const float pred = base ? a3 + base[o] : a3;
     if (y && lane < WG_MLP_ACTIONS)
         y[lane] = pred;
@@ -52,14 +61,14 @@ __device__ __forceinline__ int wg_mlp_choose_wave(const float* __restrict__ P, i
         keep->y = a3;
     }
     int best = 0;
-    float bv = __shfl(pred, 0);
+    float bv = wg_lane_value(pred, 0);
     for (int q = 1; q < WG_MLP_ACTIONS; ++q) {
-        const float v = __shfl(pred, q);
+        const float v = wg_lane_value(pred, q);
         if (wg_mlp_allowed(app, q) && v < bv) {
             best = q;
             bv = v;
         }
     }
-    const float py = __shfl(pred, plain >= 0 ? plain : 0);
+    const float py = wg_lane_value(pred, plain >= 0 ? plain : 0);
     return wg_mlp_settle(P, app, best, bv, plain, py);
 }

@@ -699,7 +699,7 @@ __device__ __forceinline__ loop_cfg read_cfg(const wg_dev_counters* c)
 
 // Schedule of the next pull from the state after the last one.  Called by all
 // lanes of one wavefront (the learned rule evaluates the model across it).
-__device__ void loop_choose_wave(const loop_cfg& g, const float* __restrict__ mlp, unsigned long long act,
+__device__ __forceinline__ void loop_choose_wave(const loop_cfg& g, const float* __restrict__ mlp, unsigned long long act,
                                  unsigned long long nact, unsigned int iteration, unsigned long long unvisited, int* mode,
                                  int* blog2)
 {
@@ -763,6 +763,8 @@ __global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, u
     c->plain_blog2 = plain_blog2;
     c->max_iters = max_iters;
     c->done = 0u;
+    c->run_err = 0u;
+    c->run_levels = 0u;
     c->iters = 0u;
     c->fiters = 0u;
     c->activity = 0ull;
@@ -816,10 +818,19 @@ __device__ void decide_wave(wg_dev_counters* c, const float* __restrict__ mlp)
     c->blog2 = blog2;
 }
 
-__global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp)
+// Also clears the counts of the fused-level run that may follow (rc, if any).
+__global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp, wg_run_ctl* __restrict__ rc)
 {
-    if (blockIdx.x == 0 && threadIdx.x < WG_WAVE)
+    if (blockIdx.x != 0)
+        return;
+    if (threadIdx.x < WG_WAVE)
         decide_wave(c, mlp);
+    if (rc && threadIdx.x < 3) {
+        rc->tail[threadIdx.x].v = 0ull;
+        rc->act[threadIdx.x].v = 0ull;
+        rc->nact[threadIdx.x].v = 0ull;
+        rc->fresh[threadIdx.x].v = 0ull;
+    }
 }
 
 // One level of the device-resident loop in four launches: the pull the
@@ -880,6 +891,508 @@ __global__ __launch_bounds__(WG_BLOCK) void k_level_advance(uint64_t* __restrict
         bfs_advance_body(visited, next, cur, nwords, ctr);
     else
         count_clear_body(next, nwords, ctr);
+}
+
+// ---- fused small-frontier levels ---------------------------------------------------
+//
+// Levels whose frontier is small cost the loop four dispatches each, while
+// their edges keep a handful of workgroups busy for a few microseconds.  A run
+// executes such levels back to back in one launch of a few resident
+// workgroups, with one grid barrier per level:
+//   - the frontier is a queue of entries (vertex, out-edge range of at most
+//     WG_RUN_CH CSRI positions, first-chunk flag), so a level touches only
+//     the out-edges of its active vertices;
+//   - BFS: each out-edge u -> v does the frontier pull's atomicMin of the
+//     packed (depth, parent) word, so the smallest parent on the previous
+//     level wins as in every other pull; the one thread whose atomic found v
+//     unvisited appends v's out-edges to the next queue and sets v in next;
+//   - CC / SSSP: each out-edge offers v the value of u (plus the edge
+//     weight) with the pulls' atomicMin; a vertex whose value drops is
+//     activated once per level (a bit in one of two alternating bitmasks) and
+//     appended to the next queue;
+//   - every workgroup then takes the decision k_decide takes (the same counts
+//     and schedule code, so all take the same one) and the run goes on while
+//     the schedule stays frontier-filtered and the level stays small
+//     (WG_RUN_ACT_MAX, WG_RUN_NACT_MAX).
+// A run starts after k_decide chose a frontier-filtered pull for a small
+// activated set (the bits of next) and ends in the state a pull followed by
+// k_decide leaves: next holds the last level's activated vertices, the
+// control block the counts and the next schedule; the frontier build and the
+// advance follow as after any pull.  BFS levels, parents and iteration counts
+// are those of the level-per-launch loop.  CC labels and SSSP distances are
+// too (the fixpoint is unique), but a run activates exactly the vertices whose
+// value dropped while a frontier-filtered pull reads whole edge blocks, so
+// the two may take different numbers of iterations to get there.  bm_cur is
+// not kept up to date during a BFS run (nothing reads it before the advance
+// rewrites it); visited gains each vertex when its first entry is expanded.
+
+// Grid barrier of a run.  A run has at most WG_RUN_MAX_WGS workgroups (no
+// more than the device has CUs), launched after the previous kernel of the
+// stream has finished, so all are resident together and an arrival counter
+// with a generation word suffices.  The wait is bounded: a workgroup that has
+// polled WG_RUN_SPIN times sets run_err and done in the control block and
+// returns, so the kernel ends whatever happens and the host reports the
+// error.
+#define WG_RUN_SPIN (1u << 19)
+
+__device__ bool run_barrier(wg_run_ctl* __restrict__ rc, unsigned int G, wg_dev_counters* __restrict__ ctr)
+{
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        unsigned int* cnt = (unsigned int*)&rc->count.v;
+        unsigned int* gen = (unsigned int*)&rc->gen.v;
+        const unsigned int g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        if (atomicAdd(cnt, 1u) == G - 1u) {
+            __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned int n = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (++n > WG_RUN_SPIN) {
+                    ok = 0;
+                    atomicOr(&ctr->run_err, 1u);
+                    atomicOr(&ctr->done, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// Exclusive prefix sum over the workgroup; *total receives the sum.
+__device__ uint32_t block_excl_scan(uint32_t x, uint32_t* total)
+{
+    __shared__ uint32_t s_w[WG_WPB];
+    const int lane = wg_lane(), wv = (int)(threadIdx.x / WG_WAVE);
+    uint32_t inc = x;
+    for (int d = 1; d < WG_WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d);
+        if (lane >= d)
+            inc += y;
+    }
+    if (lane == WG_WAVE - 1)
+        s_w[wv] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int i = 0; i < WG_WPB; ++i) {
+        const uint32_t t = s_w[i];
+        before += (i < wv) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + inc - x;
+}
+
+// Workgroup-wide sum (every thread gets it).
+__device__ unsigned long long block_sum_u64(unsigned long long x)
+{
+    __shared__ unsigned long long s_w[WG_WPB];
+    x = wg_wave_sum(x);
+    if (wg_lane() == 0)
+        s_w[threadIdx.x / WG_WAVE] = x;
+    __syncthreads();
+    unsigned long long t = 0ull;
+    for (int i = 0; i < WG_WPB; ++i)
+        t += s_w[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ uint32_t run_entries(uint32_t len)
+{
+    return len ? (len + WG_RUN_CH - 1u) / WG_RUN_CH : 1u;
+}
+
+// Writes the entries of vertex v (out-edges [s, e) of the CSRI) from idx on.
+__device__ __forceinline__ uint32_t run_put(uint4* __restrict__ q, uint32_t idx, uint32_t v, uint32_t s, uint32_t e)
+{
+    uint32_t c = s;
+    do {
+        const uint32_t ce = (e - c > WG_RUN_CH) ? c + WG_RUN_CH : e;
+        if (idx < WG_RUN_QCAP)
+            q[idx] = make_uint4(v, c, ce, c == s ? 1u : 0u);
+        ++idx;
+        c = ce;
+    } while (c < e);
+    return idx;
+}
+
+// Appends the entries of the k vertices a thread activated (k_i entries for
+// vertex vv[i]) to the queue whose tail is *tail: one atomic per workgroup.
+template <int N>
+__device__ __forceinline__ void run_append(uint4* __restrict__ q, unsigned int* __restrict__ tail, const uint32_t* vv,
+                                           const uint32_t* so, const uint32_t* eo, const bool* nw, uint32_t k)
+{
+    __shared__ uint32_t s_base;
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(k, &tot);
+    if (threadIdx.x == 0)
+        s_base = tot ? atomicAdd(tail, tot) : 0u;
+    __syncthreads();
+    uint32_t idx = s_base + pre;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (nw[j])
+            idx = run_put(q, idx, vv[j], so[j], eo[j]);
+    __syncthreads();
+}
+
+// The run's copy of the loop state (the same in every thread).
+struct run_state {
+    loop_cfg g;
+    unsigned int max_iters, level0, iters, fiters;
+    unsigned long long unvisited;
+};
+
+// Gate of a run, the same in every workgroup: the control block changes only
+// after the last barrier of a run.
+__device__ __forceinline__ bool run_gate(const wg_dev_counters* ctr, run_state& st)
+{
+    if (ctr->done || ctr->mode != WG_MODE_FRONTIER || ctr->last_activity > WG_RUN_ACT_MAX ||
+        ctr->last_activated > WG_RUN_NACT_MAX)
+        return false;
+    st.g = read_cfg(ctr);
+    st.max_iters = ctr->max_iters;
+    st.level0 = ctr->level;
+    st.iters = ctr->iters;
+    st.fiters = ctr->fiters;
+    st.unvisited = ctr->unvisited;
+    return true;
+}
+
+// Copies the schedule model to LDS for the run's per-level decisions, if the
+// learned schedule is on and has a model; returns whether it is.
+__device__ __forceinline__ bool run_stage_model(const float* __restrict__ mlp, const run_state& st, float* s_mlp)
+{
+    if (!mlp || st.g.sched != LOOP_SCHED_LEARNED)
+        return false;
+    for (int i = threadIdx.x; i < WG_MLP_OFF_END; i += WG_BLOCK)
+        s_mlp[i] = mlp[i];
+    __syncthreads();
+    return true;
+}
+
+// Entry: the vertices the last pull activated (bits of act) form queue q.
+// Each thread reads 16 words at a time, strided by the grid, so that the
+// loads are coalesced and many are in flight.
+#define WG_RUN_SCAN_BATCH 16
+__device__ void run_entry(const uint64_t* __restrict__ act, uint64_t nwords, const uint32_t* __restrict__ csri_off,
+                          uint4* __restrict__ q, wg_run_ctl* __restrict__ rc)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * WG_BLOCK;
+    const uint64_t t = (uint64_t)blockIdx.x * WG_BLOCK + threadIdx.x;
+    uint32_t k = 0;
+    for (uint64_t w0 = t; w0 < nwords; w0 += WG_RUN_SCAN_BATCH * stride) {
+        uint64_t x[WG_RUN_SCAN_BATCH];
+#pragma unroll
+        for (int j = 0; j < WG_RUN_SCAN_BATCH; ++j) {
+            const uint64_t w = w0 + (uint64_t)j * stride;
+            x[j] = (w < nwords) ? act[w] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < WG_RUN_SCAN_BATCH; ++j)
+            for (uint64_t b = x[j]; b; b &= b - 1ull) {
+                const uint32_t v = (uint32_t)((w0 + (uint64_t)j * stride) * 64u + (uint64_t)(__ffsll((long long)b) - 1));
+                k += run_entries(csri_off[v + 1] - csri_off[v]);
+            }
+    }
+    __shared__ uint32_t s_base;
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(k, &tot);
+    if (threadIdx.x == 0)
+        s_base = tot ? atomicAdd((unsigned int*)&rc->tail[0].v, tot) : 0u;
+    __syncthreads();
+    uint32_t idx = s_base + pre;
+    if (k)
+        for (uint64_t w0 = t; w0 < nwords; w0 += WG_RUN_SCAN_BATCH * stride) {
+            uint64_t x[WG_RUN_SCAN_BATCH];
+#pragma unroll
+            for (int j = 0; j < WG_RUN_SCAN_BATCH; ++j) {
+                const uint64_t w = w0 + (uint64_t)j * stride;
+                x[j] = (w < nwords) ? act[w] : 0ull;
+            }
+#pragma unroll
+            for (int j = 0; j < WG_RUN_SCAN_BATCH; ++j)
+                for (uint64_t b = x[j]; b; b &= b - 1ull) {
+                    const uint32_t v =
+                        (uint32_t)((w0 + (uint64_t)j * stride) * 64u + (uint64_t)(__ffsll((long long)b) - 1));
+                    idx = run_put(q, idx, v, csri_off[v], csri_off[v + 1]);
+                }
+        }
+}
+
+// End of level r: the workgroup's counts into the level's totals, the
+// barrier, then the decision k_decide would take after this pull (with the
+// model in LDS, s_mlp, if learned).  Returns whether the run goes on; when it
+// does not, the control block holds the state after this level.  *ok turns
+// false if the barrier gave up.
+__device__ __forceinline__ bool run_level_end(wg_run_ctl* __restrict__ rc, wg_dev_counters* __restrict__ ctr,
+                                              unsigned int r, unsigned long long act, unsigned long long nact,
+                                              unsigned long long fresh, run_state& st, const float* s_mlp, bool learned,
+                                              bool* ok)
+{
+    const unsigned int rn = (r + 1u) % 3u, rz = (r + 2u) % 3u;
+    __shared__ int s_dec[2];
+    act = block_sum_u64(act);
+    nact = block_sum_u64(nact);
+    fresh = block_sum_u64(fresh);
+    if (threadIdx.x == 0) {
+        if (act)
+            atomicAdd(&rc->act[rn].v, act);
+        if (nact)
+            atomicAdd(&rc->nact[rn].v, nact);
+        if (fresh)
+            atomicAdd(&rc->fresh[rn].v, fresh);
+        if (blockIdx.x == 0) { // every workgroup read these before the last barrier
+            __hip_atomic_store(&rc->tail[rz].v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&rc->act[rz].v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&rc->nact[rz].v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&rc->fresh[rz].v, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (!run_barrier(rc, gridDim.x, ctr)) {
+        *ok = false;
+        return false;
+    }
+    const unsigned long long A = __hip_atomic_load(&rc->act[rn].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long N = __hip_atomic_load(&rc->nact[rn].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long F = __hip_atomic_load(&rc->fresh[rn].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long T = __hip_atomic_load(&rc->tail[rn].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st.iters += 1u;
+    st.fiters += 1u;
+    st.unvisited = st.unvisited >= F ? st.unvisited - F : 0ull;
+    const bool stop = A == 0ull || (st.max_iters && st.iters >= st.max_iters);
+    if (!stop && threadIdx.x < WG_WAVE) {
+        int mode = 0, blog2 = st.g.plain_blog2;
+        if (learned) // s_mlp is the caller's __shared__ array: LDS reads
+            loop_choose_wave(st.g, s_mlp, A, N, st.iters, st.unvisited, &mode, &blog2);
+        else
+            loop_choose_wave(st.g, nullptr, A, N, st.iters, st.unvisited, &mode, &blog2);
+        if (threadIdx.x == 0) {
+            s_dec[0] = mode;
+            s_dec[1] = blog2;
+        }
+    }
+    __syncthreads();
+    const int mode = stop ? 0 : s_dec[0];
+    const int blog2 = stop ? st.g.plain_blog2 : s_dec[1];
+    __syncthreads();
+    const bool cont =
+        !stop && mode == WG_MODE_FRONTIER && A <= WG_RUN_ACT_MAX && N <= WG_RUN_NACT_MAX && T <= WG_RUN_QCAP;
+    if (!cont && blockIdx.x == 0 && threadIdx.x == 0) {
+        ctr->iters = st.iters;
+        ctr->fiters = st.fiters;
+        ctr->last_activity = A;
+        ctr->last_activated = N;
+        ctr->unvisited = st.unvisited;
+        ctr->level = st.level0 + r + 1u;
+        ctr->run_levels += r + 1u;
+        if (stop) {
+            ctr->done = 1u;
+        } else {
+            ctr->mode = (unsigned int)mode;
+            ctr->blog2 = blog2;
+        }
+    }
+    return cont;
+}
+
+__device__ __forceinline__ uint32_t run_queue_len(const wg_run_ctl* rc, unsigned int r)
+{
+    const uint32_t n = __hip_atomic_load((const unsigned int*)&rc->tail[r % 3u].v, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return n < WG_RUN_QCAP ? n : WG_RUN_QCAP;
+}
+
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict__ csri_off,
+                                                       const uint32_t* __restrict__ csri_pos,
+                                                       const uint32_t* __restrict__ dst, uint64_t* __restrict__ next,
+                                                       uint64_t* __restrict__ visited, uint64_t* __restrict__ packed,
+                                                       uint64_t nwords, const uint32_t* __restrict__ perm,
+                                                       uint4* __restrict__ qa, uint4* __restrict__ qb,
+                                                       wg_run_ctl* __restrict__ rc, wg_dev_counters* __restrict__ ctr,
+                                                       const float* __restrict__ mlp)
+{
+    run_state st;
+    if (!run_gate(ctr, st))
+        return;
+    __shared__ float s_mlp[WG_MLP_OFF_END];
+    const bool learned = run_stage_model(mlp, st, s_mlp);
+    run_entry(next, nwords, csri_off, qa, rc);
+    if (!run_barrier(rc, gridDim.x, ctr))
+        return;
+    uint4* qc = qa;
+    uint4* qn = qb;
+    for (unsigned int r = 0;; ++r) {
+        const uint32_t nc = run_queue_len(rc, r);
+        const unsigned long long newdepth = (unsigned long long)(st.level0 + 2u + r);
+        unsigned int* tn = (unsigned int*)&rc->tail[(r + 1u) % 3u].v;
+        unsigned long long act = 0ull, nact = 0ull;
+        for (uint32_t i0 = blockIdx.x * WG_BLOCK; i0 < nc; i0 += gridDim.x * WG_BLOCK) {
+            const uint32_t i = i0 + threadIdx.x;
+            uint32_t vv[WG_RUN_CH], so[WG_RUN_CH], eo[WG_RUN_CH];
+            bool nw[WG_RUN_CH];
+#pragma unroll
+            for (int j = 0; j < (int)WG_RUN_CH; ++j) {
+                vv[j] = so[j] = eo[j] = 0u;
+                nw[j] = false;
+            }
+            uint32_t k = 0;
+            if (i < nc) {
+                const uint4 en = qc[i];
+                const uint32_t u = en.x;
+                if (en.w) { // u leaves next for visited, as at an advance
+                    const unsigned long long m = 1ull << (u & 63u);
+                    atomicOr((unsigned long long*)&visited[u >> 6], m);
+                    atomicAnd((unsigned long long*)&next[u >> 6], ~m);
+                }
+                const uint32_t pid = perm ? perm[u] : u;
+                const unsigned long long val = (newdepth << 32) | (unsigned long long)pid;
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (en.y + (uint32_t)j < en.z)
+                        vv[j] = dst[csri_pos[en.y + (uint32_t)j]];
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (en.y + (uint32_t)j < en.z)
+                        nw[j] = atomicMin((unsigned long long*)&packed[vv[j]], val) == ~0ull;
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (nw[j]) {
+                        so[j] = csri_off[vv[j]];
+                        eo[j] = csri_off[vv[j] + 1];
+                        atomicOr((unsigned long long*)&next[vv[j] >> 6], 1ull << (vv[j] & 63u));
+                    }
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (nw[j]) {
+                        k += run_entries(eo[j] - so[j]);
+                        act += eo[j] - so[j];
+                        nact += 1ull;
+                    }
+            }
+            run_append<WG_RUN_CH>(qn, tn, vv, so, eo, nw, k);
+        }
+        bool ok = true;
+        // a BFS activation is always a first visit
+        if (!run_level_end(rc, ctr, r, act, nact, nact, st, s_mlp, learned, &ok))
+            return;
+        uint4* t = qc;
+        qc = qn;
+        qn = t;
+    }
+}
+
+// CC / SSSP.  Level r expands queue r, whose vertices are set in mask
+// D[r & 1], and activates into D[(r + 1) & 1] (D[0] = next, D[1] = spare):
+// a vertex leaves D[r & 1] when its first entry is expanded, and nothing
+// sets D[r & 1] during level r, so a vertex whose value drops during level r
+// (even after some of its entries were expanded with the old value) is
+// appended again.  A run that ends after an even level moves its activated
+// set from spare to next.
+template <class P>
+__global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict__ csri_off,
+                                                       const uint32_t* __restrict__ csri_pos,
+                                                       const uint32_t* __restrict__ dst, const void* __restrict__ w,
+                                                       typename P::T* __restrict__ prop, uint64_t* __restrict__ next,
+                                                       uint64_t* __restrict__ spare, uint64_t nwords,
+                                                       uint4* __restrict__ qa, uint4* __restrict__ qb,
+                                                       wg_run_ctl* __restrict__ rc, wg_dev_counters* __restrict__ ctr,
+                                                       unsigned int* __restrict__ ovf, const float* __restrict__ mlp)
+{
+    typedef typename P::T T;
+    run_state st;
+    if (!run_gate(ctr, st))
+        return;
+    __shared__ float s_mlp[WG_MLP_OFF_END];
+    const bool learned = run_stage_model(mlp, st, s_mlp);
+    run_entry(next, nwords, csri_off, qa, rc);
+    if (!run_barrier(rc, gridDim.x, ctr))
+        return;
+    uint4* qc = qa;
+    uint4* qn = qb;
+    for (unsigned int r = 0;; ++r) {
+        uint64_t* dc = (r & 1u) ? spare : next;
+        uint64_t* dn = (r & 1u) ? next : spare;
+        const uint32_t nc = run_queue_len(rc, r);
+        unsigned int* tn = (unsigned int*)&rc->tail[(r + 1u) % 3u].v;
+        unsigned long long act = 0ull, nact = 0ull, fresh = 0ull;
+        for (uint32_t i0 = blockIdx.x * WG_BLOCK; i0 < nc; i0 += gridDim.x * WG_BLOCK) {
+            const uint32_t i = i0 + threadIdx.x;
+            uint32_t vv[WG_RUN_CH], so[WG_RUN_CH], eo[WG_RUN_CH], pos[WG_RUN_CH];
+            bool nw[WG_RUN_CH];
+#pragma unroll
+            for (int j = 0; j < (int)WG_RUN_CH; ++j) {
+                vv[j] = so[j] = eo[j] = pos[j] = 0u;
+                nw[j] = false;
+            }
+            uint32_t k = 0;
+            if (i < nc) {
+                const uint4 en = qc[i];
+                const uint32_t u = en.x;
+                if (en.w)
+                    atomicAnd((unsigned long long*)&dc[u >> 6], ~(1ull << (u & 63u)));
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (en.y + (uint32_t)j < en.z) {
+                        pos[j] = csri_pos[en.y + (uint32_t)j];
+                        vv[j] = dst[pos[j]];
+                    }
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (en.y + (uint32_t)j < en.z) {
+                        const T m = P::val(prop, w, u, pos[j], ovf);
+                        if (m < prop[vv[j]]) {
+                            const T old = atomic_min_t<T>(&prop[vv[j]], m);
+                            if (old > m) {
+                                const unsigned long long b = 1ull << (vv[j] & 63u);
+                                nw[j] = !(atomicOr((unsigned long long*)&dn[vv[j] >> 6], b) & b);
+                                fresh += (old == P::INF) ? 1ull : 0ull;
+                            }
+                        }
+                    }
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (nw[j]) {
+                        so[j] = csri_off[vv[j]];
+                        eo[j] = csri_off[vv[j] + 1];
+                    }
+#pragma unroll
+                for (int j = 0; j < (int)WG_RUN_CH; ++j)
+                    if (nw[j]) {
+                        k += run_entries(eo[j] - so[j]);
+                        act += eo[j] - so[j];
+                        nact += 1ull;
+                    }
+            }
+            run_append<WG_RUN_CH>(qn, tn, vv, so, eo, nw, k);
+        }
+        bool ok = true;
+        if (!run_level_end(rc, ctr, r, act, nact, fresh, st, s_mlp, learned, &ok)) {
+            if (ok && (r & 1u) == 0u) // the last activations are in spare
+                for (uint64_t wd = (uint64_t)blockIdx.x * WG_BLOCK + threadIdx.x; wd < nwords;
+                     wd += (uint64_t)gridDim.x * WG_BLOCK) {
+                    const uint64_t x = spare[wd];
+                    if (x) {
+                        next[wd] = x;
+                        spare[wd] = 0ull;
+                    }
+                }
+            return;
+        }
+        uint4* t = qc;
+        qc = qn;
+        qn = t;
+    }
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -1365,7 +1878,30 @@ static int enqueue_iteration(wg_ctx* ctx)
             WG_MIN_LEVEL(PolSU, uint32_t);
 #undef WG_MIN_LEVEL
     }
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, mlp);
+    const bool run = ctx->run_wgs && ctx->run_q && ((ctx->run_apps >> ctx->app) & 1u) &&
+                     (ctx->app == WG_APP_BFS || ctx->run_spare);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, mlp, run ? ctx->run_ctl : nullptr);
+    if (run) {
+        const dim3 rg(ctx->run_wgs), rb(WG_BLOCK);
+        uint4* qb = ctx->run_q + WG_RUN_QCAP;
+        if (ctx->app == WG_APP_BFS) {
+            hipLaunchKernelGGL(k_bfs_run, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->bm_next,
+                               ctx->bm_visited, packed, ctx->bm_words, ctx->perm, ctx->run_q, qb, ctx->run_ctl, ctx->ctr,
+                               mlp);
+        } else {
+#define WG_MIN_RUN(P, T)                                                                                               \
+    hipLaunchKernelGGL(k_min_run<P>, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->w,            \
+                       (T*)ctx->prop[0], ctx->bm_next, ctx->run_spare, ctx->bm_words, ctx->run_q, qb, ctx->run_ctl,     \
+                       ctx->ctr, &ctx->ctr->overflow, mlp)
+            if (ctx->app == WG_APP_CC)
+                WG_MIN_RUN(PolCC, uint32_t);
+            else if (ctx->wkind == WG_WEIGHT_F64)
+                WG_MIN_RUN(PolSF, unsigned long long);
+            else
+                WG_MIN_RUN(PolSU, uint32_t);
+#undef WG_MIN_RUN
+        }
+    }
     hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_next,
                        ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->heavy_cap, ctx->ctr,
                        WG_GATE_FRONTIER);
@@ -1373,6 +1909,27 @@ static int enqueue_iteration(wg_ctx* ctx)
                        ctx->bm_next, ctx->bm_cur, ctx->bm_words, ctx->app == WG_APP_BFS ? 1 : 0, ctx->heavy_list,
                        ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, ctx->wedge, ctx->ctr);
     return wg_check_launch(ctx);
+}
+
+extern "C" int wg_set_fused_levels(wg_ctx* ctx, uint32_t workgroups, uint32_t apps)
+{
+    if (!ctx)
+        return WG_E_INVAL;
+    if (workgroups > WG_RUN_MAX_WGS || workgroups > (uint32_t)ctx->num_cu)
+        return WG_E_INVAL;
+    if (apps & ~((1u << WG_APP_BFS) | (1u << WG_APP_CC) | (1u << WG_APP_SSSP)))
+        return WG_E_INVAL;
+    ctx->run_wgs = workgroups;
+    ctx->run_apps = apps;
+    return WG_OK;
+}
+
+extern "C" int wg_fused_level_count(wg_ctx* ctx, uint64_t* levels)
+{
+    if (!ctx || !levels)
+        return WG_E_INVAL;
+    *levels = ctx->hctr->run_levels;
+    return WG_OK;
 }
 
 extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* iterations, uint64_t* frontier_iterations)
@@ -1389,6 +1946,19 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
     int rc;
     if (ctx->wedge_log2 >= 0 && (rc = clear_wedge_state(ctx)))
         return rc;
+    if (ctx->run_wgs && ((ctx->run_apps >> ctx->app) & 1u)) {
+        if (!ctx->run_q) {
+            WG_HIP(ctx, hipMalloc((void**)&ctx->run_ctl, sizeof(wg_run_ctl)));
+            WG_HIP(ctx, hipMalloc((void**)&ctx->run_q, 2 * (size_t)WG_RUN_QCAP * sizeof(uint4)));
+        }
+        WG_HIP(ctx, hipMemsetAsync(ctx->run_ctl, 0, sizeof(wg_run_ctl), ctx->stream));
+        if (ctx->app != WG_APP_BFS) {
+            if (!ctx->run_spare)
+                WG_HIP(ctx, hipMalloc((void**)&ctx->run_spare, ctx->bm_words * sizeof(uint64_t)));
+            if ((rc = wg_clear_u64(ctx, ctx->run_spare, ctx->bm_words)))
+                return rc;
+        }
+    }
     const unsigned long long threshold = (ctx->E * (unsigned long long)p->threshold_pct) / 100ull;
     // CC: source/execution_cc.c:95; BFS / SSSP: out-degree of the roots
     const unsigned long long act0 = ctx->app == WG_APP_CC ? ctx->E : ctx->root_activity;
@@ -1422,6 +1992,8 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
                 return rc;
         WG_HIP(ctx, hipMemcpyAsync(ctx->hctr, ctx->ctr, WG_CTR_HOST_BYTES, hipMemcpyDeviceToHost, ctx->stream));
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->hctr->run_err)
+            return WG_E_STATE;
         if (ctx->hctr->done)
             break;
         if (ctx->hctr->iters <= seen || ctx->hctr->iters > cap)
