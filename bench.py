@@ -105,8 +105,15 @@ def schedule_pair(ctx, g, root, reps, weights):
             out[f"{app}_{sched}"] = {
                 "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
                 "ms": round(st["running_ms"], 3), "iterations": st["iterations"],
-                "frontier_iterations": st["frontier_iterations"],
+                "frontier_iterations": st["frontier_iterations"], "fused_levels": ctx.fused_level_count(),
             }
+        if app == "bfs":  # the same runs with one launch sequence per level (fused levels off)
+            ctx.set_fused_levels(0)
+            for sched in ("plain", "learned"):
+                r, st = run_best(ctx, app, root, reps, schedule=sched)
+                out[f"bfs_{sched}"]["ms_level_per_launch"] = round(st["running_ms"], 3)
+                assert np.array_equal(r, res[sched][0])
+            ctx.set_fused_levels(32)
         out[f"{app}_learned_vs_plain"] = round(res["plain"][1]["running_ms"] / res["learned"][1]["running_ms"], 3)
         out[f"{app}_identical"] = bool(np.array_equal(res["plain"][0], res["learned"][0]))
         log(f"{app} ({weights} weights for SSSP): plain {out[f'{app}_plain']}, learned {out[f'{app}_learned']}, "
