@@ -258,3 +258,24 @@ def test_sssp_cap_inside_a_run(ctx):
         ctx.init_sssp(root)
         ctx.converge(W.SCHED_FRONTIER)
         assert np.array_equal(ctx.read_sssp(), ref)
+
+
+def test_hub_overflows_the_queue(ctx):
+    # root -> a -> hub -> 2.2M leaves: the level that reaches the hub appends
+    # more entries than a queue holds; the run ends there and the loop goes
+    # on with ordinary pulls
+    n = (1 << 21) + 100000
+    leaves = np.arange(3, n + 3, dtype=np.int64)
+    src = np.concatenate([[0, 1], np.full(n, 2, np.int64)])
+    dst = np.concatenate([[1, 2], leaves])
+    g = W.from_edges(n + 3, src, dst)
+    ctx.load(g)
+    for sched in (W.SCHED_PLAIN, W.SCHED_FRONTIER):
+        (p0, it0, _), (p1, it1, f1) = _pair(ctx, lambda: ctx.init_bfs(0), sched)
+        assert f1 > 0 and it0 == it1
+        assert np.array_equal(p0, p1)
+    depth = W.bfs_depth(p1)
+    assert depth[0] == 0 and depth[1] == 1 and depth[2] == 2 and np.all(depth[3:] == 3)
+    assert np.all(W.bfs_parent(p1)[3:] == 2)
+    (c0, _, _), (c1, _, _) = _pair_values(ctx, ctx.init_cc, ctx.read_cc, W.SCHED_FRONTIER)
+    assert np.array_equal(c0, c1) and np.all(c1 == 0)

@@ -1013,21 +1013,29 @@ __device__ __forceinline__ uint32_t run_entries(uint32_t len)
 }
 
 // Writes the entries of vertex v (out-edges [s, e) of the CSRI) from idx on.
+// Entries past WG_RUN_QCAP are dropped: the level that produced them has more
+// activity than a run takes, so the run ends and never reads the queue.
 __device__ __forceinline__ uint32_t run_put(uint4* __restrict__ q, uint32_t idx, uint32_t v, uint32_t s, uint32_t e)
 {
     uint32_t c = s;
     do {
+        if (idx >= WG_RUN_QCAP)
+            return idx;
         const uint32_t ce = (e - c > WG_RUN_CH) ? c + WG_RUN_CH : e;
-        if (idx < WG_RUN_QCAP)
-            q[idx] = make_uint4(v, c, ce, c == s ? 1u : 0u);
+        q[idx] = make_uint4(v, c, ce, c == s ? 1u : 0u);
         ++idx;
         c = ce;
     } while (c < e);
     return idx;
 }
 
-// Appends the entries of the k vertices a thread activated (k_i entries for
-// vertex vv[i]) to the queue whose tail is *tail: one atomic per workgroup.
+// Appends the entries of the k vertices a thread activated (vertex vv[i]
+// with out-edges [so[i], eo[i]) if nw[i]) to the queue whose tail is *tail:
+// one atomic per workgroup.  A vertex with more than WG_RUN_SERIAL entries
+// is written by its whole wavefront, one entry per lane.  A workgroup whose
+// entries would pass WG_RUN_QCAP writes none: the level has more activity
+// than a run takes, so the run ends without reading them.
+#define WG_RUN_SERIAL 8u
 template <int N>
 __device__ __forceinline__ void run_append(uint4* __restrict__ q, unsigned int* __restrict__ tail, const uint32_t* vv,
                                            const uint32_t* so, const uint32_t* eo, const bool* nw, uint32_t k)
@@ -1038,11 +1046,42 @@ __device__ __forceinline__ void run_append(uint4* __restrict__ q, unsigned int* 
     if (threadIdx.x == 0)
         s_base = tot ? atomicAdd(tail, tot) : 0u;
     __syncthreads();
-    uint32_t idx = s_base + pre;
+    const uint32_t base = s_base;
+    if (tot && (uint64_t)base + tot <= WG_RUN_QCAP) { // the same in the whole workgroup
+        const int lane = wg_lane();
+        uint32_t idx = base + pre;
+        uint32_t bidx[N];
+        bool big[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-        if (nw[j])
-            idx = run_put(q, idx, vv[j], so[j], eo[j]);
+        for (int j = 0; j < N; ++j) {
+            bidx[j] = idx;
+            big[j] = false;
+            if (nw[j]) {
+                const uint32_t n = run_entries(eo[j] - so[j]);
+                if (n <= WG_RUN_SERIAL)
+                    run_put(q, idx, vv[j], so[j], eo[j]);
+                else
+                    big[j] = true;
+                idx += n;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            unsigned long long m = wg_ballot(big[j]);
+            while (m) {
+                const int l = __ffsll((long long)m) - 1;
+                m &= m - 1ull;
+                const uint32_t v = __shfl(vv[j], l), s = __shfl(so[j], l), e = __shfl(eo[j], l);
+                const uint32_t b = __shfl(bidx[j], l);
+                const uint32_t n = run_entries(e - s);
+                for (uint32_t c = (uint32_t)lane; c < n; c += WG_WAVE) {
+                    const uint32_t cs = s + c * WG_RUN_CH;
+                    const uint32_t ce = (e - cs > WG_RUN_CH) ? cs + WG_RUN_CH : e;
+                    q[b + c] = make_uint4(v, cs, ce, c == 0 ? 1u : 0u);
+                }
+            }
+        }
+    }
     __syncthreads();
 }
 
