@@ -222,16 +222,17 @@ typedef struct wg_loop_params {
 int wg_converge(wg_ctx* ctx, const wg_loop_params* params, uint64_t* iterations, uint64_t* frontier_iterations);
 /* Fused small-frontier levels.  No reference counterpart: an execution
  * option of wg_converge.  While the chosen pull is frontier-filtered and a
- * level activates at most 2^17 vertices with at most 2^17 out-edges, the
- * loop runs its levels back to back in one launch of `workgroups` resident
- * workgroups (one grid barrier per level) instead of four launches per
- * level, each level touching only the out-edges of its active vertices.
- * apps: bit (1 << WG_APP_BFS | WG_APP_CC | WG_APP_SSSP) per application that
- * uses it.  BFS results, iteration counts and schedule decisions are
- * unchanged; CC labels and SSSP distances are unchanged, their iteration
- * counts may differ (a frontier-filtered pull reads whole edge blocks, which
- * can settle values in fewer iterations).  workgroups 0 turns it off; at
- * most 128 and at most the device's CU count.  Default: 32 workgroups, BFS. */
+ * level is small (at most 1024 activated vertices and 8192 out-edges per
+ * workgroup of the run), the loop runs its levels back to back in one launch
+ * of `workgroups` resident workgroups (one grid barrier per level) instead of
+ * four launches per level, each level touching only the out-edges of its
+ * active vertices.  Sets the workgroup count for every application whose bit
+ * (1 << WG_APP_BFS, WG_APP_CC, WG_APP_SSSP) is in apps; 0 turns the option
+ * off for them; at most 256 and at most the device's CU count.  BFS results,
+ * iteration counts and schedule decisions are unchanged; CC labels and SSSP
+ * distances are unchanged, their iteration counts may differ (a
+ * frontier-filtered pull reads whole edge blocks, which can settle values in
+ * fewer iterations).  Default: BFS with 32 workgroups, CC and SSSP off. */
 int wg_set_fused_levels(wg_ctx* ctx, uint32_t workgroups, uint32_t apps);
 /* Iterations of the last wg_converge that ran fused (included in its
  * iteration counts). */

@@ -143,7 +143,7 @@ def test_empty_frontier_and_isolated_root(ctx):
 def test_setter_bounds():
     c = W.Context(0)
     with pytest.raises(W.WedgeError):
-        c.set_fused_levels(129)
+        c.set_fused_levels(257)
     with pytest.raises(W.WedgeError):
         c.set_fused_levels(32, (W.APP_PR,))
     c.set_fused_levels(0)
@@ -179,6 +179,7 @@ def _pair_values(ctx, init, read, sched, wgs=32, max_iterations=0):
         init()
         it = ctx.converge(sched, max_iterations=max_iterations)
         out.append((read(), it, ctx.fused_level_count()))
+    ctx.set_fused_levels(0, (W.APP_CC, W.APP_SSSP))
     ctx.set_fused_levels(32)
     return out
 
@@ -227,7 +228,7 @@ def test_cc_and_sssp_fused_match(ctx, model, kind, sched):
     ctx.set_schedule_model(None)
 
 
-@pytest.mark.parametrize("wgs", [1, 5, 128])
+@pytest.mark.parametrize("wgs", [1, 5, 128, 256])
 def test_sssp_several_roots_and_workgroup_counts(ctx, wgs):
     g = W.grid(90, 140, seed=9)
     ctx.load(g)
@@ -258,6 +259,7 @@ def test_sssp_cap_inside_a_run(ctx):
         ctx.init_sssp(root)
         ctx.converge(W.SCHED_FRONTIER)
         assert np.array_equal(ctx.read_sssp(), ref)
+    ctx.set_fused_levels(0, (W.APP_CC, W.APP_SSSP))
 
 
 def test_hub_overflows_the_queue(ctx):

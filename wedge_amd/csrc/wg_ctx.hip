@@ -383,8 +383,7 @@ extern "C" int wg_open(wg_ctx** out, int device)
         return WG_E_NOMEM;
     }
     memset(ctx->hctr, 0, sizeof(wg_dev_counters));
-    ctx->run_wgs = WG_RUN_DEFAULT_WGS;
-    ctx->run_apps = 1u << WG_APP_BFS;
+    ctx->run_wgs[WG_APP_BFS] = WG_RUN_DEFAULT_WGS;
     *out = ctx;
     return WG_OK;
 }

@@ -105,16 +105,18 @@ __device__ __forceinline__ bool wg_gated_off(const wg_dev_counters* c, int gate)
     return c->mode != (unsigned int)gate;
 }
 
-// Fused small-frontier levels (k_bfs_run): a queue entry is up to WG_RUN_CH
-// out-edges of one vertex; a run continues while a level activates at most
-// WG_RUN_NACT_MAX vertices with at most WG_RUN_ACT_MAX out-edges, so that the
-// next queue (at most activity / WG_RUN_CH + activated entries) fits.
+// Fused small-frontier levels (k_bfs_run, k_min_run): a queue entry is up to
+// WG_RUN_CH out-edges of one vertex.  A run of G workgroups takes levels that
+// activate at most WG_RUN_NACT_PER_WG * G vertices with at most
+// WG_RUN_ACT_PER_WG * G out-edges: at most 2048 entries (8 rounds of its
+// threads) per workgroup, so a queue of WG_RUN_QCAP entries holds any level a
+// run of WG_RUN_MAX_WGS workgroups continues with.
 #define WG_RUN_CH 8u
-#define WG_RUN_ACT_MAX (1ull << 17)
-#define WG_RUN_NACT_MAX (1ull << 17)
-#define WG_RUN_QCAP (1u << 18)
+#define WG_RUN_NACT_PER_WG 1024ull
+#define WG_RUN_ACT_PER_WG 8192ull
+#define WG_RUN_MAX_WGS 256u
+#define WG_RUN_QCAP (1u << 19)
 #define WG_RUN_DEFAULT_WGS 32u
-#define WG_RUN_MAX_WGS 128u
 
 struct __attribute__((aligned(256))) wg_run_line {
     unsigned long long v;
@@ -218,8 +220,7 @@ struct wg_ctx {
     unsigned long long root_activity; // summed out-degree of the roots
     struct wg_members* members;      // batch of member graphs (wg_set_members), NULL if none
     // fused small-frontier levels of the device loop (k_bfs_run)
-    uint32_t run_wgs;         // workgroups of a run, 0 = off
-    uint32_t run_apps;        // apps that use runs, bit 1 << WG_APP_*
+    uint32_t run_wgs[8];      // workgroups of a run, by WG_APP_*, 0 = off
     struct wg_run_ctl* run_ctl;
     uint4* run_q;             // two queues of WG_RUN_QCAP entries
     uint64_t* run_spare;      // CC / SSSP runs: second activation mask (bm_words), zero between runs

@@ -1096,8 +1096,8 @@ struct run_state {
 // after the last barrier of a run.
 __device__ __forceinline__ bool run_gate(const wg_dev_counters* ctr, run_state& st)
 {
-    if (ctr->done || ctr->mode != WG_MODE_FRONTIER || ctr->last_activity > WG_RUN_ACT_MAX ||
-        ctr->last_activated > WG_RUN_NACT_MAX)
+    if (ctr->done || ctr->mode != WG_MODE_FRONTIER || ctr->last_activity > WG_RUN_ACT_PER_WG * gridDim.x ||
+        ctr->last_activated > WG_RUN_NACT_PER_WG * gridDim.x)
         return false;
     st.g = read_cfg(ctr);
     st.max_iters = ctr->max_iters;
@@ -1226,7 +1226,8 @@ __device__ __forceinline__ bool run_level_end(wg_run_ctl* __restrict__ rc, wg_de
     const int blog2 = stop ? st.g.plain_blog2 : s_dec[1];
     __syncthreads();
     const bool cont =
-        !stop && mode == WG_MODE_FRONTIER && A <= WG_RUN_ACT_MAX && N <= WG_RUN_NACT_MAX && T <= WG_RUN_QCAP;
+        !stop && mode == WG_MODE_FRONTIER && A <= WG_RUN_ACT_PER_WG * gridDim.x &&
+        N <= WG_RUN_NACT_PER_WG * gridDim.x && T <= WG_RUN_QCAP;
     if (!cont && blockIdx.x == 0 && threadIdx.x == 0) {
         ctr->iters = st.iters;
         ctr->fiters = st.fiters;
@@ -1917,11 +1918,11 @@ static int enqueue_iteration(wg_ctx* ctx)
             WG_MIN_LEVEL(PolSU, uint32_t);
 #undef WG_MIN_LEVEL
     }
-    const bool run = ctx->run_wgs && ctx->run_q && ((ctx->run_apps >> ctx->app) & 1u) &&
-                     (ctx->app == WG_APP_BFS || ctx->run_spare);
+    const uint32_t run_wgs = ctx->run_wgs[ctx->app & 7];
+    const bool run = run_wgs && ctx->run_q && (ctx->app == WG_APP_BFS || ctx->run_spare);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, mlp, run ? ctx->run_ctl : nullptr);
     if (run) {
-        const dim3 rg(ctx->run_wgs), rb(WG_BLOCK);
+        const dim3 rg(run_wgs), rb(WG_BLOCK);
         uint4* qb = ctx->run_q + WG_RUN_QCAP;
         if (ctx->app == WG_APP_BFS) {
             hipLaunchKernelGGL(k_bfs_run, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->bm_next,
@@ -1958,8 +1959,9 @@ extern "C" int wg_set_fused_levels(wg_ctx* ctx, uint32_t workgroups, uint32_t ap
         return WG_E_INVAL;
     if (apps & ~((1u << WG_APP_BFS) | (1u << WG_APP_CC) | (1u << WG_APP_SSSP)))
         return WG_E_INVAL;
-    ctx->run_wgs = workgroups;
-    ctx->run_apps = apps;
+    for (int a = 0; a < 8; ++a)
+        if ((apps >> a) & 1u)
+            ctx->run_wgs[a] = workgroups;
     return WG_OK;
 }
 
@@ -1985,7 +1987,7 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
     int rc;
     if (ctx->wedge_log2 >= 0 && (rc = clear_wedge_state(ctx)))
         return rc;
-    if (ctx->run_wgs && ((ctx->run_apps >> ctx->app) & 1u)) {
+    if (ctx->run_wgs[ctx->app & 7]) {
         if (!ctx->run_q) {
             WG_HIP(ctx, hipMalloc((void**)&ctx->run_ctl, sizeof(wg_run_ctl)));
             WG_HIP(ctx, hipMalloc((void**)&ctx->run_q, 2 * (size_t)WG_RUN_QCAP * sizeof(uint4)));
