@@ -143,13 +143,55 @@ __device__ __forceinline__ uint32_t wedge_nwords(uint32_t E, int blog2)
     return (uint32_t)((nblk + 31ull) >> 5);
 }
 
+// Item k of a wavefront whose lane l holds cnt_l items (inc: the inclusive
+// prefix sum of cnt over the lanes, total: its last value, k < total): the
+// owning lane and the rank of k among that lane's items.  Every lane calls
+// it (a uniform loop of shuffles); lanes may ask for different k.
+__device__ __forceinline__ void wave_item_owner(uint32_t inc, uint32_t cnt, uint32_t k, int* owner, uint32_t* rank)
+{
+    int lo = 0, hi = WG_WAVE - 1;
+#pragma unroll
+    for (int it = 0; it < 6; ++it) {
+        const int mid = (lo + hi) >> 1;
+        if ((uint32_t)__shfl((int)inc, mid) > k)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    *owner = lo;
+    *rank = k - (uint32_t)__shfl((int)(inc - cnt), lo);
+}
+
+// Position of the rank-th set bit of w (w has more than rank set bits).
+__device__ __forceinline__ uint32_t nth_set_bit(uint64_t w, uint32_t rank)
+{
+    for (uint32_t i = 0; i < rank; ++i)
+        w &= w - 1ull;
+    return (uint32_t)(__ffsll((long long)w) - 1);
+}
+
+// Inclusive prefix sum of x over the lanes of the wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+    const int lane = wg_lane();
+#pragma unroll
+    for (int d = 1; d < WG_WAVE; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d);
+        if (lane >= d)
+            x += y;
+    }
+    return x;
+}
+
 // Walks the marked edge blocks (source/phase_edge_pull_bfs.c:245-268): each
 // wavefront takes 64 consecutive frontier words, one per lane, zeroes the
-// non-zero ones, and for each of them hands the marked blocks to lane groups
-// of 2^min(b,6) lanes, 64 >> min(b,6) blocks per step.  f(p, ok, first) is
-// called by every lane of the wavefront for each 64-edge step: p is the edge
-// position, ok whether the lane holds an edge, first whether the lane is the
-// first of its block's group.
+// non-zero ones, and hands their marked blocks, in order, to lane groups of
+// 2^min(b,6) lanes, 64 >> min(b,6) blocks per step, whichever words they
+// come from (a prefix sum of the words' bit counts), so sparse words do not
+// cost a step each.  f(p, ok, first) is called by every lane of the
+// wavefront for each 64-edge step: p is the edge position, ok whether the
+// lane holds an edge, first whether the lane is the first of its block's
+// group.
 template <class F>
 __device__ __forceinline__ void walk_marked_blocks(uint32_t* __restrict__ wedge, int blog2, uint32_t E, F&& f)
 {
@@ -167,24 +209,21 @@ __device__ __forceinline__ void walk_marked_blocks(uint32_t* __restrict__ wedge,
             if (word)
                 wedge[wi] = 0u; // consumed, source/phase_edge_pull_bfs.c:252-256
         }
-        unsigned long long nz = wg_ballot(word != 0u);
-        while (nz) {
-            const int l = __ffsll((long long)nz) - 1;
-            nz &= nz - 1ull;
-            uint32_t bits = (uint32_t)__shfl((int)word, l); // wave-uniform
-            const uint64_t base = (uint64_t)(w0 + (uint32_t)l) * 32u;
-            while (bits) {
-                uint32_t mine = bits; // this lane group's block: set bit number 'slot'
-                for (uint32_t i = 0; i < slot && mine; ++i)
-                    mine &= mine - 1u;
-                const bool have = mine != 0u;
-                const uint64_t blk = base + (have ? (uint64_t)(__ffs((int)mine) - 1) : 0ull);
-                for (uint32_t st = 0; st < bw.steps; ++st) {
-                    const uint64_t p64 = (blk << blog2) + (uint64_t)st * 64u + sub;
-                    f((uint32_t)p64, have && p64 < E, sub == 0u);
-                }
-                for (uint32_t i = 0; i < bw.per_wave && bits; ++i)
-                    bits &= bits - 1u;
+        const uint32_t cnt = (uint32_t)__popc(word);
+        const uint32_t inc = wave_incl_scan(cnt);
+        const uint32_t total = (uint32_t)__shfl((int)inc, WG_WAVE - 1);
+        for (uint32_t k0 = 0; k0 < total; k0 += bw.per_wave) {
+            const uint32_t k = k0 + slot;
+            const bool have = k < total;
+            int owner;
+            uint32_t rank;
+            wave_item_owner(inc, cnt, have ? k : 0u, &owner, &rank);
+            const uint32_t ow = (uint32_t)__shfl((int)word, owner);
+            const uint64_t blk =
+                (uint64_t)(w0 + (uint32_t)owner) * 32u + (have ? (uint64_t)nth_set_bit(ow, rank) : 0ull);
+            for (uint32_t st = 0; st < bw.steps; ++st) {
+                const uint64_t p64 = (blk << blog2) + (uint64_t)st * 64u + sub;
+                f((uint32_t)p64, have && p64 < E, sub == 0u);
             }
         }
     }

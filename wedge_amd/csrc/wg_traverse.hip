@@ -540,9 +540,10 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __rest
 
 
 // source/wedge.c:63-155: every set vertex marks the blocks of its out-edges.
-// Each lane loads one bitmask word; the wavefront then takes the non-zero
-// words one at a time with one lane per vertex of the word, so a dense word
-// costs one step and an empty one nothing.  Vertices with more than
+// Each lane loads one bitmask word; the set vertices of the wavefront's 64
+// words are then handed out 64 at a time, one per lane, whichever words they
+// come from (a prefix sum of the words' bit counts), so a sparse bitmask does
+// not cost a step per non-zero word.  Vertices with more than
 // WG_WEDGE_LANE_MAX out-edges are deferred to k_wedge_heavy as chunks of
 // WG_WEDGE_CHUNK out-edges (pairs vertex, first position), so that a hub's
 // list is spread over many wavefronts.  At most E/32 + E/WG_WEDGE_CHUNK + 1
@@ -562,14 +563,18 @@ __global__ __launch_bounds__(WG_BLOCK) void k_wedge_build(const uint64_t* __rest
          w0 += nwaves * WG_WAVE) {
         const uint64_t wi = w0 + (uint64_t)lane;
         const uint64_t word = (wi < nwords) ? active[wi] : 0ull;
-        unsigned long long nz = wg_ballot(word != 0ull);
-        while (nz) {
-            const int l = __ffsll((long long)nz) - 1;
-            nz &= nz - 1ull;
-            const uint64_t bits = __shfl(word, l);
-            if (!((bits >> lane) & 1ull))
+        const uint32_t cnt = (uint32_t)__popcll(word);
+        const uint32_t inc = wave_incl_scan(cnt);
+        const uint32_t total = (uint32_t)__shfl((int)inc, WG_WAVE - 1);
+        for (uint32_t k0 = 0; k0 < total; k0 += WG_WAVE) {
+            const uint32_t k = k0 + (uint32_t)lane;
+            int owner;
+            uint32_t rank;
+            wave_item_owner(inc, cnt, k < total ? k : 0u, &owner, &rank);
+            const uint64_t ow = __shfl(word, owner);
+            if (k >= total)
                 continue;
-            const uint32_t u = (uint32_t)((w0 + (uint64_t)l) * 64u + (uint64_t)lane);
+            const uint32_t u = (uint32_t)((w0 + (uint64_t)owner) * 64u + (uint64_t)nth_set_bit(ow, rank));
             const uint32_t s0 = csri_off[u], e0 = csri_off[u + 1];
             if (e0 - s0 > WG_WEDGE_LANE_MAX) {
                 const uint32_t nch = (e0 - s0 + WG_WEDGE_CHUNK - 1) / WG_WEDGE_CHUNK;
