@@ -165,9 +165,12 @@ static pr_shape current_shape(const wg_ctx* ctx)
 template <typename T, int MODE, int IPT>
 static void pr_launch_shape(wg_ctx* ctx, const pr_shape& s, const T* cin, T* cout, double* S_in, double* S_out, double N)
 {
-    hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT>), dim3(s.grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, cin,
-                       cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles, S_in, ctx->damping, N, ctx->tile_part,
-                       ctx->blk_part, s.split);
+    // the hottest contributions in LDS, if the vertices are in degree order
+    const uint64_t hub_max = WG_PR_HUB_BYTES / sizeof(T);
+    const uint32_t nhub = ctx->perm ? (uint32_t)(ctx->V < hub_max ? ctx->V : hub_max) : 0u;
+    hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT, true>), dim3(s.grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
+                       cin, cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles, S_in, ctx->damping, N, ctx->tile_part,
+                       ctx->blk_part, s.split, nhub);
     hipLaunchKernelGGL((k_pr_span<T, MODE, IPT>), dim3(s.sgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, s.heavy_v,
                        s.heavy_t, s.nheavy, ctx->tile_part, cout, ctx->acc, ctx->od, S_in, ctx->damping, N, ctx->blk_part,
                        s.grid, S_out, &ctx->ctr->ticket);

@@ -55,7 +55,15 @@ static __device__ __forceinline__ double load_contrib(const T* __restrict__ p, u
 // first dependent use, the source stream is read non-temporally (it is read
 // once per iteration and should not displace contributions from L2), and
 // wavefronts never wait for each other.
-template <typename T, int MODE, int IPT = WG_IPT>
+//
+// HUB: the contributions of vertices [0, nhub) (nhub <= WG_PR_HUB_BYTES /
+// sizeof(T)) are copied to LDS first and gathered from there.  After the
+// degree relabelling these are the highest-degree vertices, the sources of a
+// large share of the edges (rmat-s24: 11% for the first 2048, 15% for the
+// first 4096), whose gathers then skip the L2.  16 KB keeps four workgroups
+// per CU; 32 and 64 KB measured slower (fewer resident waves).
+#define WG_PR_HUB_BYTES 16384u
+template <typename T, int MODE, int IPT = WG_IPT, bool HUB = false>
 __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                        const T* __restrict__ cin, T* __restrict__ cout,
                                                        double* __restrict__ acc_out, const uint32_t* __restrict__ od,
@@ -63,7 +71,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
                                                        const uint32_t* __restrict__ tile_e, uint32_t ntiles,
                                                        const double* __restrict__ S_in, double damping, double N,
                                                        double* __restrict__ tile_part, double* __restrict__ blk_part,
-                                                       wg_tile_split split = wg_tile_split{})
+                                                       wg_tile_split split = wg_tile_split{}, uint32_t nhub = 0)
 {
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
     constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
@@ -72,6 +80,17 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
     __shared__ double s_fp_all[WG_WPB][WG_WAVE];
     __shared__ double s_lp_all[WG_WPB][WG_WAVE];
     __shared__ double s_red[WG_WPB];
+    __shared__ T s_hub[HUB ? WG_PR_HUB_BYTES / sizeof(T) : 1];
+    if (HUB) {
+        for (uint32_t i = threadIdx.x; i < nhub; i += WG_BLOCK)
+            s_hub[i] = cin[i];
+        __syncthreads();
+    }
+    auto gather = [&](uint32_t sj) -> double {
+        if (HUB && sj < nhub)
+            return (double)s_hub[sj];
+        return load_contrib(cin, sj);
+    };
 
     const int wave = threadIdx.x / WG_WAVE;
     const int lane = wg_lane();
@@ -108,7 +127,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
 #pragma unroll
             for (int j = 0; j < IPT; ++j)
                 if (a + j < e1)
-                    s += load_contrib(cin, s_idx[j]);
+                    s += gather(s_idx[j]);
             s = wg_wave_sum(s);
             if (lane == 0)
                 tile_part[t] = s;
@@ -131,8 +150,11 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
         }
         double g[IPT];
 #pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            g[j] = (a + j < e1) ? load_contrib(cin, s_idx[j]) : 0.0;
+        for (int j = 0; j < IPT; ++j) {
+            g[j] = 0.0;
+            if (a + j < e1)
+                g[j] = gather(s_idx[j]);
+        }
 #pragma unroll
         for (int k = 0; k < NOFF; ++k) {
             const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
