@@ -167,17 +167,15 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
     bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr);
 }
 
-// Dense level for one chunk of WG_BFS_HEAVY in-edges of a high in-degree vertex.
-__global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
-                                                         uint32_t n, const uint32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
-                                                         const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
-                                                         uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
-                                                         const uint32_t* __restrict__ perm,
-                                                         wg_dev_counters* __restrict__ ctr, int gate)
+// Dense level for chunks of WG_BFS_HEAVY in-edges of high in-degree vertices,
+// one wave per chunk.
+__device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he, uint32_t n,
+                               const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                               const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
+                               const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                               uint64_t* __restrict__ packed, const uint32_t* __restrict__ perm,
+                               wg_dev_counters* __restrict__ ctr)
 {
-    if (wg_gated_off(ctr, gate))
-        return;
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
@@ -218,6 +216,19 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
     }
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+}
+
+__global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
+                                                         uint32_t n, const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
+                                                         const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
+                                                         uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
+                                                         const uint32_t* __restrict__ perm,
+                                                         wg_dev_counters* __restrict__ ctr, int gate)
+{
+    if (wg_gated_off(ctr, gate))
+        return;
+    bfs_heavy_body(hv, he, n, off, src, od, cur, visited, next, packed, perm, ctr);
 }
 
 // Frontier-mode BFS level over the marked edge blocks.
@@ -851,13 +862,17 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
                                                          const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
                                                          uint64_t* __restrict__ next, uint64_t* __restrict__ packed,
                                                          uint32_t V, uint32_t E, uint32_t* __restrict__ wedge,
-                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr)
+                                                         const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
+                                                         const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
+                                                         uint32_t nheavy)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
-    if (ctr->mode == WG_MODE_DENSE)
+    if (ctr->mode == WG_MODE_DENSE) {
         bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr);
-    else
+        // and the high in-degree destinations the dense body leaves out
+        bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
+    } else
         bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
 
@@ -1891,24 +1906,20 @@ extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)
     return WG_OK;
 }
 
-// One iteration of the device-resident loop: [dense pull of the hub
-// destinations, gated], the level pull, the decision, the frontier build
-// (gated) and the advance (with the build's deferred hub out-lists).
+// One iteration of the device-resident loop: the level pull (BFS dense: with
+// the hub destinations' chunks), the decision, [a fused run], the frontier
+// build (gated) and the advance (with the build's deferred hub out-lists).
 static int enqueue_iteration(wg_ctx* ctx)
 {
     uint64_t* packed = (uint64_t*)ctx->prop[0];
     const uint32_t qgrid = wg_frontier_grid(ctx);
     const float* mlp = ctx->mlp_set ? ctx->mlp : nullptr;
     if (ctx->app == WG_APP_BFS) {
-        if (ctx->nbfs_heavy)
-            hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
-                               ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
-                               ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
-                               WG_GATE_DENSE);
         const uint32_t dgrid = wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8);
         hipLaunchKernelGGL(k_bfs_level, dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr);
+                           (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
+                           ctx->bfs_heavy_e, ctx->nbfs_heavy);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL(P, T)                                                                                            \
