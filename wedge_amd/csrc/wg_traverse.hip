@@ -358,41 +358,51 @@ __global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev
 
 // ---- CC / SSSP (min propagation) -------------------------------------------------
 
+// val_from(du, w, p): the value edge p offers its destination when its
+// source holds du; val reads du from prop.
 struct PolCC {
     typedef uint32_t T;
     static constexpr uint32_t INF = 0xffffffffu;
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
+    static constexpr bool HUB = true; // dense pulls read the first labels from LDS
+    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
         (void)w;
         (void)p;
         (void)ovf;
-        return prop[u];
+        return du;
+    }
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        return val_from(prop[u], w, p, ovf);
     }
 };
 
 struct PolSF { // FP64 distances as bit patterns
     typedef unsigned long long T;
     static constexpr unsigned long long INF = 0x7FF0000000000000ull;
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
+    static constexpr bool HUB = false;
+    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
         (void)ovf;
-        const T du = prop[u];
         if (du >= INF)
             return INF;
         const double s = __longlong_as_double((long long)du) + ((const double*)w)[p];
         return (T)__double_as_longlong(s);
+    }
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        return val_from(prop[u], w, p, ovf);
     }
 };
 
 struct PolSU { // 32-bit integer distances
     typedef uint32_t T;
     static constexpr uint32_t INF = 0xffffffffu;
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
+    static constexpr bool HUB = false;
+    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
-        const uint32_t du = prop[u];
         if (du == INF)
             return INF;
         const uint32_t s = du + ((const uint32_t*)w)[p];
@@ -402,7 +412,33 @@ struct PolSU { // 32-bit integer distances
         }
         return s;
     }
+    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
+                            unsigned int* __restrict__ ovf)
+    {
+        return val_from(prop[u], w, p, ovf);
+    }
 };
+
+// Values of the first vertices in LDS for the dense pulls (policies with HUB):
+// after the degree relabelling they are the sources of a large share of the
+// edges.  The copy is taken when the pull starts; a value that drops during the
+// pull activates its vertex, whose out-edges the next pull reads again, so a
+// copy that is older than prop changes when values settle, not what they
+// settle to.  CC labels: 5 iterations either way on the symmetrised
+// rmat-s24, 12.0 -> 11.2 ms.  SSSP distances on rmat-s24 took 23 iterations
+// instead of 20 with the copy (37.4 -> 44.5 ms), and still 23 with only the
+// copy's workgroup barrier left in (how far a pull propagates in place depends
+// on how its waves interleave), so SSSP keeps the plain path, without either.
+#define WG_MIN_HUB_BYTES 16384u
+template <class P>
+__device__ __forceinline__ void stage_hub(const typename P::T* __restrict__ prop, uint32_t nhub, typename P::T* s_hub)
+{
+    if constexpr (P::HUB) {
+        for (uint32_t i = threadIdx.x; i < nhub; i += blockDim.x)
+            s_hub[i] = prop[i];
+        __syncthreads();
+    }
+}
 
 template <class P>
 __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint64_t* __restrict__ next,
@@ -428,7 +464,8 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
                               const void* __restrict__ w, typename P::T* __restrict__ prop,
                               const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
                               const uint32_t* __restrict__ tile_e, uint32_t ntiles, uint64_t* __restrict__ next,
-                              wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf)
+                              wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf,
+                              const typename P::T* s_hub, uint32_t nhub)
 {
     typedef typename P::T T;
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
@@ -450,7 +487,15 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
 #pragma unroll
         for (int j = 0; j < WG_IPT; ++j) {
             const uint32_t e = a + j;
-            g[j] = (e < e1) ? P::val(prop, w, __builtin_nontemporal_load(&src[e]), e, ovf) : P::INF;
+            if constexpr (P::HUB) {
+                g[j] = P::INF;
+                if (e < e1) {
+                    const uint32_t u = __builtin_nontemporal_load(&src[e]);
+                    g[j] = P::val_from(u < nhub ? s_hub[u] : prop[u], w, e, ovf);
+                }
+            } else {
+                g[j] = (e < e1) ? P::val(prop, w, __builtin_nontemporal_load(&src[e]), e, ovf) : P::INF;
+            }
         }
         if (tv & WG_HEAVY_BIT) {
             T m = P::INF;
@@ -531,11 +576,13 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
                                                         const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
                                                         const uint32_t* __restrict__ tile_e, uint32_t ntiles,
                                                         uint64_t* __restrict__ next, wg_dev_counters* __restrict__ ctr,
-                                                        unsigned int* __restrict__ ovf, int gate)
+                                                        unsigned int* __restrict__ ovf, int gate, uint32_t nhub)
 {
     if (wg_gated_off(ctr, gate))
         return;
-    min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf);
+    __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
+    stage_hub<P>(prop, nhub, s_hub);
+    min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
 }
 
 // Frontier-mode pull over the marked edge blocks.
@@ -909,13 +956,16 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restri
                                                          const uint32_t* __restrict__ tile_v,
                                                          const uint32_t* __restrict__ tile_e, uint32_t ntiles, uint32_t E,
                                                          uint64_t* __restrict__ next, uint32_t* __restrict__ wedge,
-                                                         wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf)
+                                                         wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf,
+                                                         uint32_t nhub)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
-    if (ctr->mode == WG_MODE_DENSE)
-        min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf);
-    else
+    if (ctr->mode == WG_MODE_DENSE) {
+        __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
+        stage_hub<P>(prop, nhub, s_hub);
+        min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
+    } else
         min_blocks_body<P>(ctr, ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
 }
 
@@ -1797,6 +1847,15 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
     return finish_pull(ctx, edges_activated);
 }
 
+// Vertices whose values a dense pull reads from LDS: the first ones, if the
+// vertices are in degree order.
+template <class P>
+static uint32_t min_hub_count(const wg_ctx* ctx)
+{
+    const uint64_t cap = WG_MIN_HUB_BYTES / sizeof(typename P::T);
+    return (P::HUB && ctx->perm) ? (uint32_t)(ctx->V < cap ? ctx->V : cap) : 0u;
+}
+
 template <class P>
 static int min_pull(wg_ctx* ctx, int mode, void* prop)
 {
@@ -1805,7 +1864,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
         hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
                            (typename P::T*)prop, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
-                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS);
+                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
@@ -1951,7 +2010,7 @@ static int enqueue_iteration(wg_ctx* ctx)
 #define WG_MIN_LEVEL(P, T)                                                                                            \
     hipLaunchKernelGGL(k_min_level<P>, dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, ctx->w, \
                        (T*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, (uint32_t)ctx->E, ctx->bm_next,   \
-                       ctx->wedge, ctx->ctr, &ctx->ctr->overflow)
+                       ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx))
         if (ctx->app == WG_APP_CC)
             WG_MIN_LEVEL(PolCC, uint32_t);
         else if (ctx->wkind == WG_WEIGHT_F64)
