@@ -40,16 +40,26 @@ __device__ __forceinline__ int wg_mlp_choose_wave(const float* __restrict__ P, i
     for (int i = 0; i < WG_MLP_IN; ++i)
         a1 = a1 + P[WG_MLP_OFF_W1 + i * WG_MLP_HIDDEN + lane] * z[i];
     const float h1 = wg_mlp_relu(a1);
+    // a layer's activations reach every lane through a per-wave LDS row (reads
+    // of one address by all lanes are broadcasts, issued 16 at a time)
+    __shared__ float s_act[16][WG_MLP_HIDDEN]; // up to 16 wavefronts per workgroup
+    float* act = s_act[(threadIdx.x / WG_WAVE) & 15];
+    act[lane] = h1;
+    wg_wave_sync();
     float a2 = P[WG_MLP_OFF_B2 + lane];
 #pragma unroll 16
     for (int k = 0; k < WG_MLP_HIDDEN; ++k)
-        a2 = a2 + P[WG_MLP_OFF_W2 + k * WG_MLP_HIDDEN + lane] * wg_lane_value(h1, k);
+        a2 = a2 + P[WG_MLP_OFF_W2 + k * WG_MLP_HIDDEN + lane] * act[k];
     const float h2 = wg_mlp_relu(a2);
+    wg_wave_sync();
+    act[lane] = h2;
+    wg_wave_sync();
     const int o = lane < WG_MLP_ACTIONS ? lane : WG_MLP_ACTIONS - 1;
     float a3 = P[WG_MLP_OFF_B3 + o];
 #pragma unroll 16
     for (int j = 0; j < WG_MLP_HIDDEN; ++j)
-        a3 = a3 + P[WG_MLP_OFF_W3 + j * WG_MLP_ACTIONS + o] * wg_lane_value(h2, j);
+        a3 = a3 + P[WG_MLP_OFF_W3 + j * WG_MLP_ACTIONS + o] * act[j];
+    wg_wave_sync(); // the row is free again
     const float pred = base ? a3 + base[o] : a3;
     if (y && lane < WG_MLP_ACTIONS)
         y[lane] = pred;
