@@ -314,7 +314,9 @@ int wg_set_members(wg_ctx* ctx, uint32_t n, const uint64_t* first /* n + 1 entri
 
 /* A launch shape of the batched PageRank: edges per lane of a tile (a tile
  * holds up to 64 * edges_per_lane edges; 4, 8 or 16), workgroups per compute
- * unit of the tile kernel (1..16), and xcd_split: 1 = each of the 8 XCDs
+ * unit of the tile kernel (1..16; for the single-graph PageRank, 1 means one
+ * workgroup of 16 wavefronts per CU with a 64 KB LDS contribution cache,
+ * the others workgroups of 4 wavefronts), and xcd_split: 1 = each of the 8 XCDs
  * walks its own contiguous eighth of the edges (so a member's contributions
  * stay in one XCD's L2), 0 = all workgroups stride over all tiles. */
 typedef struct wg_batch_shape {
@@ -353,8 +355,10 @@ int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape);
 int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape);
 /* After wg_init_pr: times `iterations` fused iterations per candidate (best of
  * two after a warm-up), keeps the fastest shape, and initialises PageRank
- * again with the same damping and precision.  ms_per_shape (optional,
- * WG_BATCH_NSHAPES entries) receives the time per iteration of each. */
+ * again with the same damping and precision.  The candidates are the batch's
+ * WG_BATCH_NSHAPES shapes and {4|8|16, 1, 0} (one 16-wavefront workgroup per
+ * CU).  ms_per_shape (optional, WG_BATCH_NSHAPES entries) receives the time
+ * per iteration of each of the batch's shapes. */
 int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* best, float* ms_per_shape);
 
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;

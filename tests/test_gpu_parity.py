@@ -116,11 +116,12 @@ def test_pr_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 0), (4, 16, 1), (16, 2, 0), (16, 8, 1), (8, 2, 1)])
+@pytest.mark.parametrize("shape", [(4, 4, 0), (4, 16, 1), (16, 2, 0), (16, 8, 1), (8, 2, 1), (4, 1, 0), (16, 1, 1)])
 def test_pr_tile_shapes_match_oracle(ctx, shape):
-    """Every PageRank tile shape (edges per lane, workgroups per CU, XCD split)
-    on a power-law graph and on a graph whose hubs exceed every tile size;
-    FP64 and FP32-stored contributions, fused and two-phase iterations."""
+    """Every PageRank tile shape (edges per lane, workgroups per CU, XCD split;
+    1 workgroup per CU: 16 wavefronts with a 64 KB contribution cache) on a
+    power-law graph and on a graph whose hubs exceed every tile size; FP64 and
+    FP32-stored contributions, fused and two-phase iterations."""
     for g, its in ((W.rmat(13, 16, seed=4), 16), (_hub_graph(), 10)):
         ctx.load(g)
         ctx.set_pr_shape(shape)

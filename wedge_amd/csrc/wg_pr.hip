@@ -152,7 +152,8 @@ static pr_shape current_shape(const wg_ctx* ctx)
         s.nheavy = t.nheavy;
         split = t.split;
     }
-    s.grid = wg_grid(ctx, s.ntiles, WG_WPB, (uint32_t)ctx->pr_bpc);
+    // blocks_per_cu 1: one workgroup of WG_PR_BIG_WPB wavefronts per CU
+    s.grid = ctx->pr_bpc == 1 ? wg_grid(ctx, s.ntiles, WG_PR_BIG_WPB, 1) : wg_grid(ctx, s.ntiles, WG_WPB, (uint32_t)ctx->pr_bpc);
     s.sgrid = wg_grid(ctx, s.nheavy, 4, 4);
     s.split.on = ctx->pr_xcd ? 1u : 0u;
     for (int x = 0; x <= WG_NXCD; ++x)
@@ -166,11 +167,17 @@ template <typename T, int MODE, int IPT>
 static void pr_launch_shape(wg_ctx* ctx, const pr_shape& s, const T* cin, T* cout, double* S_in, double* S_out, double N)
 {
     // the hottest contributions in LDS, if the vertices are in degree order
-    const uint64_t hub_max = WG_PR_HUB_BYTES / sizeof(T);
+    const bool big = ctx->pr_bpc == 1;
+    const uint64_t hub_max = big ? wg_pr_hub_max<T, WG_PR_BIG_WPB>() : wg_pr_hub_max<T, WG_WPB>();
     const uint32_t nhub = ctx->perm ? (uint32_t)(ctx->V < hub_max ? ctx->V : hub_max) : 0u;
-    hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT, true>), dim3(s.grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
-                       cin, cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles, S_in, ctx->damping, N, ctx->tile_part,
-                       ctx->blk_part, s.split, nhub);
+    if (big)
+        hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT, true, WG_PR_BIG_WPB>), dim3(s.grid), dim3(WG_PR_BIG_WPB * WG_WAVE), 0,
+                           ctx->stream, ctx->off, ctx->src, cin, cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles,
+                           S_in, ctx->damping, N, ctx->tile_part, ctx->blk_part, s.split, nhub);
+    else
+        hipLaunchKernelGGL((k_pr_tile<T, MODE, IPT, true>), dim3(s.grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
+                           ctx->src, cin, cout, ctx->acc, ctx->od, s.tile_v, s.tile_e, s.ntiles, S_in, ctx->damping, N,
+                           ctx->tile_part, ctx->blk_part, s.split, nhub);
     hipLaunchKernelGGL((k_pr_span<T, MODE, IPT>), dim3(s.sgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, s.heavy_v,
                        s.heavy_t, s.nheavy, ctx->tile_part, cout, ctx->acc, ctx->od, S_in, ctx->damping, N, ctx->blk_part,
                        s.grid, S_out, &ctx->ctr->ticket);
@@ -262,9 +269,13 @@ extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* 
     int rc = WG_OK;
     float best_ms = 0.0f;
     wg_batch_shape best_s = {WG_IPT, 8, 0};
-    for (uint32_t i = 0; i < WG_BATCH_NSHAPES && rc == WG_OK; ++i) {
+    // the batch's candidates, then one 16-wave workgroup per CU for each tile size
+    for (uint32_t i = 0; i < WG_BATCH_NSHAPES + 3 && rc == WG_OK; ++i) {
         wg_batch_shape s;
-        wg_batch_shape_at(i, &s);
+        if (i < WG_BATCH_NSHAPES)
+            wg_batch_shape_at(i, &s);
+        else
+            s = wg_batch_shape{4u << (i - WG_BATCH_NSHAPES), 1u, 0u};
         if ((rc = wg_set_pr_shape(ctx, &s)) || (rc = wg_pr_iteration(ctx, nullptr))) // warm-up
             break;
         float ms = 0.0f;
@@ -286,7 +297,7 @@ extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* 
         }
         if (rc)
             break;
-        if (ms_per_shape)
+        if (ms_per_shape && i < WG_BATCH_NSHAPES)
             ms_per_shape[i] = ms;
         if (i == 0 || ms < best_ms) {
             best_ms = ms;

@@ -25,8 +25,10 @@ struct wg_tile_split {
     uint32_t start[WG_NXCD + 1];
 };
 
-// Deterministic block-wide sum (xor butterfly inside each wave, then the
-// four wave sums in wave order).  All threads receive the result.
+// Deterministic block-wide sum of a block of WPB wavefronts (xor butterfly
+// inside each wave, then the wave sums in wave order).  All threads receive
+// the result.
+template <int WPB = WG_WPB>
 static __device__ __forceinline__ double block_sum_det(double v, double* s_red)
 {
     v = wg_wave_sum(v);
@@ -36,7 +38,7 @@ static __device__ __forceinline__ double block_sum_det(double v, double* s_red)
         s_red[wave] = v;
     __syncthreads();
     double r = 0.0;
-    for (int i = 0; i < WG_WPB; ++i)
+    for (int i = 0; i < WPB; ++i)
         r += s_red[i];
     return r;
 }
@@ -56,15 +58,23 @@ static __device__ __forceinline__ double load_contrib(const T* __restrict__ p, u
 // once per iteration and should not displace contributions from L2), and
 // wavefronts never wait for each other.
 //
-// HUB: the contributions of vertices [0, nhub) (nhub <= WG_PR_HUB_BYTES /
-// sizeof(T)) are copied to LDS first and gathered from there.  After the
-// degree relabelling these are the highest-degree vertices, the sources of a
-// large share of the edges (rmat-s24: 11% for the first 2048, 15% for the
-// first 4096), whose gathers then skip the L2.  16 KB keeps four workgroups
-// per CU; 32 and 64 KB measured slower (fewer resident waves).
+// HUB: the contributions of vertices [0, nhub) (nhub <= wg_pr_hub_max<T,
+// WPB>()) are copied to LDS first and gathered from there.  After the degree
+// relabelling these are the highest-degree vertices, the sources of a large
+// share of the edges (rmat-s24: 11.6% for the first 2048, 15% for the first
+// 4096, 21% for the first 8192), whose gathers then skip the L2.  The cache
+// is 16 KB per 4 wavefronts: 16 KB for the 4-wave workgroups (four per CU;
+// 32 and 64 KB there measured slower, fewer resident waves), 64 KB for a
+// 16-wave workgroup, one per CU (WPB = 16).
 #define WG_PR_HUB_BYTES 16384u
-template <typename T, int MODE, int IPT = WG_IPT, bool HUB = false>
-__global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+#define WG_PR_BIG_WPB 16 // wavefronts of the one-per-CU workgroup (shape blocks_per_cu 1)
+template <typename T, int WPB>
+__host__ __device__ constexpr uint32_t wg_pr_hub_max()
+{
+    return WG_PR_HUB_BYTES * (WPB / WG_WPB) / (uint32_t)sizeof(T);
+}
+template <typename T, int MODE, int IPT = WG_IPT, bool HUB = false, int WPB = WG_WPB>
+__global__ __launch_bounds__(WPB * WG_WAVE) void k_pr_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                        const T* __restrict__ cin, T* __restrict__ cout,
                                                        double* __restrict__ acc_out, const uint32_t* __restrict__ od,
                                                        const uint32_t* __restrict__ tile_v,
@@ -75,14 +85,14 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
 {
     constexpr int NOFF = (WG_TILE_VERTS + 1 + WG_WAVE - 1) / WG_WAVE;
     constexpr int NVTX = WG_TILE_VERTS / WG_WAVE;
-    __shared__ uint32_t s_off_all[WG_WPB][WG_TILE_VERTS + 1];
-    __shared__ double s_acc_all[WG_WPB][WG_TILE_VERTS];
-    __shared__ double s_fp_all[WG_WPB][WG_WAVE];
-    __shared__ double s_lp_all[WG_WPB][WG_WAVE];
-    __shared__ double s_red[WG_WPB];
-    __shared__ T s_hub[HUB ? WG_PR_HUB_BYTES / sizeof(T) : 1];
+    __shared__ uint32_t s_off_all[WPB][WG_TILE_VERTS + 1];
+    __shared__ double s_acc_all[WPB][WG_TILE_VERTS];
+    __shared__ double s_fp_all[WPB][WG_WAVE];
+    __shared__ double s_lp_all[WPB][WG_WAVE];
+    __shared__ double s_red[WPB];
+    __shared__ T s_hub[HUB ? wg_pr_hub_max<T, WPB>() : 1];
     if (HUB) {
-        for (uint32_t i = threadIdx.x; i < nhub; i += WG_BLOCK)
+        for (uint32_t i = threadIdx.x; i < nhub; i += WPB * WG_WAVE)
             s_hub[i] = cin[i];
         __syncthreads();
     }
@@ -105,12 +115,12 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
     }
     double s_part = 0.0;
 
-    uint32_t t = blockIdx.x * WG_WPB + wave, tend = ntiles, stride = gridDim.x * WG_WPB;
+    uint32_t t = blockIdx.x * WPB + wave, tend = ntiles, stride = gridDim.x * WPB;
     if (split.on) { // gridDim.x is a multiple of WG_NXCD
         const uint32_t x = blockIdx.x % WG_NXCD;
-        t = split.start[x] + (blockIdx.x / WG_NXCD) * WG_WPB + wave;
+        t = split.start[x] + (blockIdx.x / WG_NXCD) * WPB + wave;
         tend = split.start[x + 1];
-        stride = (gridDim.x / WG_NXCD) * WG_WPB;
+        stride = (gridDim.x / WG_NXCD) * WPB;
     }
     for (; t < tend; t += stride) {
         const uint32_t tv = tile_v[t];
@@ -245,7 +255,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_pr_tile(const uint32_t* __restrict
         }
         wg_wave_sync(); // the next tile reuses this wavefront's LDS
     }
-    s_part = block_sum_det(s_part, s_red);
+    s_part = block_sum_det<WPB>(s_part, s_red);
     if (threadIdx.x == 0)
         blk_part[blockIdx.x] = s_part;
 }
