@@ -68,10 +68,11 @@ static __device__ __forceinline__ double load_contrib(const T* __restrict__ p, u
 // 16-wave workgroup, one per CU (WPB = 16).
 #define WG_PR_HUB_BYTES 16384u
 #define WG_PR_BIG_WPB 16 // wavefronts of the one-per-CU workgroup (shape blocks_per_cu 1)
+#define WG_PR_BIG_HUB_BYTES 90112u // what the 16-wave workgroup's 160 KB of LDS leaves
 template <typename T, int WPB>
 __host__ __device__ constexpr uint32_t wg_pr_hub_max()
 {
-    return WG_PR_HUB_BYTES * (WPB / WG_WPB) / (uint32_t)sizeof(T);
+    return (WPB == WG_PR_BIG_WPB ? WG_PR_BIG_HUB_BYTES : WG_PR_HUB_BYTES * (WPB / WG_WPB)) / (uint32_t)sizeof(T);
 }
 template <typename T, int MODE, int IPT = WG_IPT, bool HUB = false, int WPB = WG_WPB>
 __global__ __launch_bounds__(WPB * WG_WAVE) void k_pr_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
