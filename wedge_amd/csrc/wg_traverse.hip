@@ -364,6 +364,7 @@ struct PolCC {
     typedef uint32_t T;
     static constexpr uint32_t INF = 0xffffffffu;
     static constexpr bool HUB = true; // dense pulls read the first labels from LDS
+    static constexpr int DENSE_IPT = WG_IPT; // 512-edge tiles (11.2 ms against 11.6 with 256 on rmat-s24 CC)
     __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
         (void)w;
@@ -382,6 +383,7 @@ struct PolSF { // FP64 distances as bit patterns
     typedef unsigned long long T;
     static constexpr unsigned long long INF = 0x7FF0000000000000ull;
     static constexpr bool HUB = false;
+    static constexpr int DENSE_IPT = 4; // 256-edge tiles (rmat-s24 SSSP 37.5 -> 32.1 ms)
     __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
         (void)ovf;
@@ -401,6 +403,7 @@ struct PolSU { // 32-bit integer distances
     typedef uint32_t T;
     static constexpr uint32_t INF = 0xffffffffu;
     static constexpr bool HUB = false;
+    static constexpr int DENSE_IPT = 4;
     __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
     {
         if (du == INF)
@@ -459,7 +462,7 @@ __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint
 // order-independent, so pieces of a destination spread over lanes or tiles
 // meet through atomics (the reference's idempotent merge,
 // source/phases.c:54-65).
-template <class P>
+template <class P, int IPT = WG_IPT>
 __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                               const void* __restrict__ w, typename P::T* __restrict__ prop,
                               const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
@@ -482,10 +485,10 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
         const uint32_t tv = tile_v[t];
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
         const uint32_t v0 = tv & ~WG_HEAVY_BIT;
-        const uint32_t a = e0 + (uint32_t)lane * WG_IPT;
-        T g[WG_IPT];
+        const uint32_t a = e0 + (uint32_t)lane * IPT;
+        T g[IPT];
 #pragma unroll
-        for (int j = 0; j < WG_IPT; ++j) {
+        for (int j = 0; j < IPT; ++j) {
             const uint32_t e = a + j;
             if constexpr (P::HUB) {
                 g[j] = P::INF;
@@ -500,7 +503,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
         if (tv & WG_HEAVY_BIT) {
             T m = P::INF;
 #pragma unroll
-            for (int j = 0; j < WG_IPT; ++j)
+            for (int j = 0; j < IPT; ++j)
                 m = g[j] < m ? g[j] : m;
             m = wg_wave_min(m);
             if (lane == 0 && m != P::INF)
@@ -530,7 +533,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
             uint32_t dend = s_off[d + 1];
             T run = P::INF;
 #pragma unroll
-            for (int j = 0; j < WG_IPT; ++j) {
+            for (int j = 0; j < IPT; ++j) {
                 const uint32_t e = a + j;
                 if (e < e1) {
                     run = g[j] < run ? g[j] : run;
@@ -570,7 +573,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
     add_tally(act, ctr);
 }
 
-template <class P>
+template <class P, int IPT = WG_IPT>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                         const void* __restrict__ w, typename P::T* __restrict__ prop,
                                                         const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
@@ -582,7 +585,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
         return;
     __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
     stage_hub<P>(prop, nhub, s_hub);
-    min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
+    min_tile_body<P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
 }
 
 // Frontier-mode pull over the marked edge blocks.
@@ -949,7 +952,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
         bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
 
-template <class P>
+template <class P, int IPT = WG_IPT>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ dst, const void* __restrict__ w,
                                                          typename P::T* __restrict__ prop, const uint32_t* __restrict__ od,
@@ -964,7 +967,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restri
     if (ctr->mode == WG_MODE_DENSE) {
         __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
         stage_hub<P>(prop, nhub, s_hub);
-        min_tile_body<P>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
+        min_tile_body<P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
     } else
         min_blocks_body<P>(ctr, ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
 }
@@ -1567,6 +1570,9 @@ static int init_common(wg_ctx* ctx, int app)
         return rc;
     ctx->app = app;
     ctx->iter = 0;
+    // the 256-edge tiles of the dense SSSP pulls (built once per graph)
+    if (app == WG_APP_SSSP && (rc = wg_build_pr_tilings(ctx)))
+        return rc;
     WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->level, 0, sizeof(unsigned int), ctx->stream));
     WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->done, 0, sizeof(unsigned int), ctx->stream));
     return WG_OK;
@@ -1856,15 +1862,40 @@ static uint32_t min_hub_count(const wg_ctx* ctx)
     return (P::HUB && ctx->perm) ? (uint32_t)(ctx->V < cap ? ctx->V : cap) : 0u;
 }
 
+// Tiles of the dense pulls of policy P: 256 edges (4 per lane) if
+// P::DENSE_IPT asks for them and the finer tilings exist (built by the SSSP
+// init), else the default 512.
+struct min_tiling {
+    const uint32_t *tile_v, *tile_e;
+    uint32_t ntiles;
+    int ipt;
+};
+
+template <class P>
+static min_tiling dense_min_tiling(const wg_ctx* ctx)
+{
+    if (P::DENSE_IPT == 4 && ctx->pr_alt_built) {
+        const wg_tiling& t = ctx->pr_alt[0];
+        return min_tiling{t.tile_v, t.tile_e, t.ntiles, 4};
+    }
+    return min_tiling{ctx->tile_v, ctx->tile_e, ctx->ntiles, WG_IPT};
+}
+
 template <class P>
 static int min_pull(wg_ctx* ctx, int mode, void* prop)
 {
     int rc;
     if (mode == WG_MODE_DENSE) {
         int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
-        hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->w,
-                           (typename P::T*)prop, ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, ctx->bm_next,
-                           ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
+        const min_tiling mt = dense_min_tiling<P>(ctx);
+        if (mt.ipt == 4)
+            hipLaunchKernelGGL((k_min_tile<P, 4>), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
+                               ctx->w, (typename P::T*)prop, ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, ctx->bm_next,
+                               ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
+        else
+            hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
+                               ctx->w, (typename P::T*)prop, ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, ctx->bm_next,
+                               ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
@@ -2007,10 +2038,18 @@ static int enqueue_iteration(wg_ctx* ctx)
                            ctx->bfs_heavy_e, ctx->nbfs_heavy);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
-#define WG_MIN_LEVEL(P, T)                                                                                            \
-    hipLaunchKernelGGL(k_min_level<P>, dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, ctx->w, \
-                       (T*)ctx->prop[0], ctx->od, ctx->tile_v, ctx->tile_e, ctx->ntiles, (uint32_t)ctx->E, ctx->bm_next,   \
-                       ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx))
+#define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
+    hipLaunchKernelGGL((k_min_level<P, IPT>), dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, \
+                       ctx->w, (T*)ctx->prop[0], ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, (uint32_t)ctx->E,            \
+                       ctx->bm_next, ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx))
+#define WG_MIN_LEVEL(P, T)                          \
+    do {                                            \
+        const min_tiling mt = dense_min_tiling<P>(ctx); \
+        if (mt.ipt == 4)                            \
+            WG_MIN_LEVEL_IPT(P, T, 4);              \
+        else                                        \
+            WG_MIN_LEVEL_IPT(P, T, WG_IPT);         \
+    } while (0)
         if (ctx->app == WG_APP_CC)
             WG_MIN_LEVEL(PolCC, uint32_t);
         else if (ctx->wkind == WG_WEIGHT_F64)
@@ -2018,6 +2057,7 @@ static int enqueue_iteration(wg_ctx* ctx)
         else
             WG_MIN_LEVEL(PolSU, uint32_t);
 #undef WG_MIN_LEVEL
+#undef WG_MIN_LEVEL_IPT
     }
     const uint32_t run_wgs = ctx->run_wgs[ctx->app & 7];
     const bool run = run_wgs && ctx->run_q && (ctx->app == WG_APP_BFS || ctx->run_spare);
