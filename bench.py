@@ -15,7 +15,9 @@ The same run reports:
             threshold rule and under the learned schedule (the packaged
             model), on this graph and on a 4096x6144 road grid (integer
             weights, seed 3), and CC on the symmetrised graph: GTEPS,
-            learned/plain speedup, identical outputs;
+            learned/plain speedup, identical outputs, and the violations the
+            on-device certificate (wg_validate) finds in each result (0 =
+            a valid BFS tree / shortest-path fixpoint / minimum labels);
   batched   64-source BFS and SSSP and 32-seed personalised PageRank on this
             graph against one source at a time, and 256 uneven R-MAT graphs
             (scale U{10..16}, edge factor U{8..32}, seed 4) as one batch
@@ -106,6 +108,8 @@ def schedule_pair(ctx, g, root, reps, weights):
                 "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
                 "ms": round(st["running_ms"], 3), "iterations": st["iterations"],
                 "frontier_iterations": st["frontier_iterations"], "fused_levels": ctx.fused_level_count(),
+                # on-device certificate of the result (BFS tree / tight shortest-path edges)
+                "violations": ctx.validate(),
             }
         if app == "bfs":  # the same runs with one launch sequence per level (fused levels off)
             ctx.set_fused_levels(0)
@@ -296,7 +300,8 @@ def large_graph(damping, steps=5, iterations=16, reps=3):
         out[f"bfs_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
                                "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 2),
                                "algorithmic_bytes": alg,
-                               "roofline_frac": round(alg / (st["running_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                               "roofline_frac": round(alg / (st["running_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                               "violations": ctx.validate()}
         log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
     out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
     out["bfs_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
@@ -348,7 +353,8 @@ def cc_pair(scale, edge_factor, seed, reps=2):
         st = best[1]
         out[f"cc_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
                               "frontier_iterations": st["frontier_iterations"],
-                              "bedges_per_s": round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2)}
+                              "bedges_per_s": round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2),
+                              "violations": ctx.validate()}  # on-device certificate of the labels
     out["cc_learned_vs_plain"] = round(out["cc_plain"]["ms"] / out["cc_learned"]["ms"], 3)
     out["cc_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
     ctx.close()
