@@ -96,16 +96,31 @@ __device__ __forceinline__ void stage_cur(const uint64_t* __restrict__ cur, uint
     __syncthreads();
 }
 
-// Dense bottom-up level: one wave per 64 consecutive vertices.  A lane scans
-// up to WG_BFS_SEQ in-edges of its own vertex; vertices with more edges are
+// first[v]: the smallest in-neighbour of v (the first of its ascending
+// in-list), UINT32_MAX if v has no in-edges or more than WG_BFS_HEAVY.
+__global__ void k_first_src(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src, uint64_t V,
+                            uint32_t* __restrict__ first)
+{
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = off[v], d = off[v + 1] - s;
+        first[v] = (d == 0u || d > WG_BFS_HEAVY) ? 0xffffffffu : src[s];
+    }
+}
+
+// Dense bottom-up level: one wave per 64 consecutive vertices.  A lane first
+// tests its vertex's smallest in-neighbour (first[], read coalesced): after
+// the degree relabelling that is usually a hub, and most vertices of a dense
+// level stop there without reading their in-list.  Otherwise the lane scans
+// the rest of up to WG_BFS_SEQ in-edges; vertices with more edges are
 // finished by the whole wave, 64 edges per step, stopping at the first hit.
 // Vertices with more than WG_BFS_HEAVY in-edges are left to k_bfs_heavy.
 // s_cur: the frontier's first words in LDS (stage_cur).
 __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
-                               const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
-                               const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
-                               uint64_t* __restrict__ packed, uint32_t V, const uint32_t* __restrict__ perm,
-                               wg_dev_counters* __restrict__ ctr, const uint64_t* s_cur)
+                               const uint32_t* __restrict__ first, const uint32_t* __restrict__ od,
+                               const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
+                               uint64_t* __restrict__ next, uint64_t* __restrict__ packed, uint32_t V,
+                               const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
+                               const uint64_t* s_cur)
 {
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
@@ -118,19 +133,23 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
             continue;
         const uint32_t v = (uint32_t)(w * 64 + lane);
         bool todo = (v < V) && !((vis >> lane) & 1ull);
-        uint32_t s = 0, e = 0;
-        if (todo) {
-            s = off[v];
-            e = off[v + 1];
-        }
-        const uint32_t deg = e - s;
-        todo = todo && deg > 0 && deg <= WG_BFS_HEAVY;
+        uint32_t f = 0xffffffffu;
+        if (todo)
+            f = first[v];
+        todo = todo && f != 0xffffffffu; // has 1..WG_BFS_HEAVY in-edges
         if (wg_ballot(todo) == 0ull)
             continue;
         int64_t parent = -1;
-        if (todo) {
-            const uint32_t lim = (deg < WG_BFS_SEQ) ? e : s + WG_BFS_SEQ;
-            for (uint32_t p = s; p < lim; ++p) {
+        if (todo && test_cur(cur, s_cur, f))
+            parent = f;
+        // the rest of the in-list, if the smallest in-neighbour is not on the frontier
+        const bool more = todo && parent < 0;
+        uint32_t s = 0, e = 0;
+        if (more) {
+            s = off[v];
+            e = off[v + 1];
+            const uint32_t lim = (e - s < WG_BFS_SEQ) ? e : s + WG_BFS_SEQ;
+            for (uint32_t p = s + 1; p < lim; ++p) {
                 const uint32_t u = src[p];
                 if (test_cur(cur, s_cur, u)) {
                     parent = u;
@@ -138,7 +157,8 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
                 }
             }
         }
-        unsigned long long need = wg_ballot(todo && parent < 0 && deg > WG_BFS_SEQ);
+        const uint32_t deg = e - s;
+        unsigned long long need = wg_ballot(more && parent < 0 && deg > WG_BFS_SEQ);
         while (need) {
             const int l = __ffsll((long long)need) - 1;
             need &= need - 1ull;
@@ -178,6 +198,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
 }
 
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
+                                                         const uint32_t* __restrict__ first,
                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
                                                          const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
                                                          uint64_t* __restrict__ packed, uint32_t V,
@@ -188,7 +209,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
         return;
     __shared__ uint64_t s_cur[WG_BFS_CUR_LDS];
     stage_cur(cur, V, s_cur);
-    bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr, s_cur);
+    bfs_dense_body(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
 }
 
 // Dense level for chunks of WG_BFS_HEAVY in-edges of high in-degree vertices,
@@ -927,8 +948,9 @@ __global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp, wg_r
 // control block selected (dense or frontier, one kernel), the decision for
 // the next level, the frontier build for it, and the advance (with the hub
 // out-lists the build deferred).  Every launch costs a dispatch, gated off or
-// not (consecutive kernels start about 8 us apart on this device), which
-// dominates levels with small frontiers.  The decision stays a kernel of its
+// not (back-to-back launches start about 3 us apart on this device), and each
+// kernel its own chain of memory round trips, which dominates levels with
+// small frontiers.  The decision stays a kernel of its
 // own: taking it in the last workgroup of the pull needs device-scope fences
 // in every wave, which cost more than the launch.
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
@@ -938,14 +960,14 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
                                                          uint32_t V, uint32_t E, uint32_t* __restrict__ wedge,
                                                          const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
                                                          const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
-                                                         uint32_t nheavy)
+                                                         uint32_t nheavy, const uint32_t* __restrict__ first)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
     if (ctr->mode == WG_MODE_DENSE) {
         __shared__ uint64_t s_cur[WG_BFS_CUR_LDS];
         stage_cur(cur, V, s_cur);
-        bfs_dense_body(off, src, od, cur, visited, next, packed, V, perm, ctr, s_cur);
+        bfs_dense_body(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
         // and the high in-degree destinations the dense body leaves out
         bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
     } else
@@ -1573,6 +1595,14 @@ static int init_common(wg_ctx* ctx, int app)
     // the 256-edge tiles of the dense SSSP pulls (built once per graph)
     if (app == WG_APP_SSSP && (rc = wg_build_pr_tilings(ctx)))
         return rc;
+    // the smallest in-neighbours the dense BFS levels test first (once per graph)
+    if (app == WG_APP_BFS && !ctx->first_src) {
+        WG_HIP(ctx, hipMalloc((void**)&ctx->first_src, (ctx->V ? ctx->V : 1) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_first_src, dim3(wg_grid(ctx, ctx->V, 256, 8)), dim3(256), 0, ctx->stream, ctx->off, ctx->src,
+                           ctx->V, ctx->first_src);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+    }
     WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->level, 0, sizeof(unsigned int), ctx->stream));
     WG_HIP(ctx, hipMemsetAsync(&ctx->ctr->done, 0, sizeof(unsigned int), ctx->stream));
     return WG_OK;
@@ -1824,8 +1854,8 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
     if (mode == WG_MODE_DENSE) {
         int slot = wg_prof_begin(ctx, WG_K_BFS_DENSE);
         hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
-                           ctx->stream, ctx->off, ctx->src, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
-                           (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_ALWAYS);
+                           ctx->stream, ctx->off, ctx->src, ctx->first_src, ctx->od, ctx->bm_cur, ctx->bm_visited,
+                           ctx->bm_next, packed, (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
             return rc;
         wg_prof_end(ctx, WG_K_BFS_DENSE, slot);
@@ -2035,7 +2065,7 @@ static int enqueue_iteration(wg_ctx* ctx)
         hipLaunchKernelGGL(k_bfs_level, dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
-                           ctx->bfs_heavy_e, ctx->nbfs_heavy);
+                           ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
