@@ -313,7 +313,8 @@ int wg_validate_values(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t nro
 int wg_set_members(wg_ctx* ctx, uint32_t n, const uint64_t* first /* n + 1 entries */);
 
 /* A launch shape of the batched PageRank: edges per lane of a tile (a tile
- * holds up to 64 * edges_per_lane edges; 4, 8 or 16), workgroups per compute
+ * holds up to 64 * edges_per_lane edges; 4, 8 or 16, and for the single-graph
+ * PageRank also 2), workgroups per compute
  * unit of the tile kernel (1..16; for the single-graph PageRank, 1 means one
  * workgroup of 16 wavefronts per CU with a 64 KB LDS contribution cache,
  * the others workgroups of 4 wavefronts), and xcd_split: 1 = each of the 8 XCDs
@@ -353,12 +354,16 @@ int wg_read_batch_ranks(wg_ctx* ctx, double* host_ranks);
  * order). */
 int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape);
 int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape);
+/* The single-graph PageRank autotuner's candidate number i
+ * (0 <= i < WG_PR_NSHAPES): the batch's WG_BATCH_NSHAPES shapes, then
+ * {4|8|16, 1, 0} (one 16-wavefront workgroup per CU), then 128-edge tiles
+ * {2, 1, 0}, {2, 4, 0}, {2, 16, 0} (2 edges per lane). */
+#define WG_PR_NSHAPES (WG_BATCH_NSHAPES + 6)
+int wg_pr_shape_at(uint32_t i, wg_batch_shape* out);
 /* After wg_init_pr: times `iterations` fused iterations per candidate (best of
  * two after a warm-up), keeps the fastest shape, and initialises PageRank
- * again with the same damping and precision.  The candidates are the batch's
- * WG_BATCH_NSHAPES shapes and {4|8|16, 1, 0} (one 16-wavefront workgroup per
- * CU).  ms_per_shape (optional, WG_BATCH_NSHAPES entries) receives the time
- * per iteration of each of the batch's shapes. */
+ * again with the same damping and precision.  ms_per_shape (optional,
+ * WG_PR_NSHAPES entries) receives the time per iteration of each candidate. */
 int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* best, float* ms_per_shape);
 
 /* BFS result: packed[v] = (depth << 32) | parent, UINT64_MAX if unreached;

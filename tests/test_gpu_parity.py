@@ -116,7 +116,8 @@ def test_pr_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 0), (4, 16, 1), (16, 2, 0), (16, 8, 1), (8, 2, 1), (4, 1, 0), (16, 1, 1)])
+@pytest.mark.parametrize("shape", [(4, 4, 0), (4, 16, 1), (16, 2, 0), (16, 8, 1), (8, 2, 1), (4, 1, 0), (16, 1, 1),
+                                   (2, 1, 0), (2, 4, 1)])
 def test_pr_tile_shapes_match_oracle(ctx, shape):
     """Every PageRank tile shape (edges per lane, workgroups per CU, XCD split;
     1 workgroup per CU: 16 wavefronts with a 64 KB contribution cache) on a
@@ -145,7 +146,8 @@ def test_pr_autotune_resets_the_state(ctx):
     ctx.load(g)
     ctx.init_pr(0.85)
     best, times = ctx.pr_autotune(2)
-    assert best in times and times[best] == min(times.values()) and ctx.pr_shape() == best
+    assert len(times) == W.PR_NSHAPES and best in times
+    assert times[best] == min(times.values()) and ctx.pr_shape() == best
     print("PR autotune:", best, {k: round(v, 4) for k, v in times.items()})
     for _ in range(12):
         ctx.pr_iteration()

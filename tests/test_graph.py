@@ -167,3 +167,12 @@ def test_batch_shapes_are_listed():
     assert {s[0] for s in shapes} == {4, 8, 16}
     with pytest.raises(W.WedgeError):
         W.batch_shape_at(W.BATCH_NSHAPES)
+
+
+def test_pr_shapes_extend_the_batch_shapes():
+    shapes = [W.pr_shape_at(i) for i in range(W.PR_NSHAPES)]
+    assert len(set(shapes)) == W.PR_NSHAPES
+    assert shapes[:W.BATCH_NSHAPES] == [W.batch_shape_at(i) for i in range(W.BATCH_NSHAPES)]
+    assert {s[0] for s in shapes} == {2, 4, 8, 16}
+    with pytest.raises(W.WedgeError):
+        W.pr_shape_at(W.PR_NSHAPES)

@@ -136,6 +136,7 @@ GPU_SYMBOLS = [
     ("wg_read_batch_ranks", C.c_int, [_p, _f64p]),
     ("wg_set_pr_shape", C.c_int, [_p, _p]),
     ("wg_get_pr_shape", C.c_int, [_p, _p]),
+    ("wg_pr_shape_at", C.c_int, [C.c_uint32, _p]),
     ("wg_pr_autotune", C.c_int, [_p, C.c_uint32, _p, _f32p]),
     ("wg_edge_pull_pr", C.c_int, [_p, _f64p]),
     ("wg_vertex_pr", C.c_int, [_p]),
@@ -180,6 +181,7 @@ class BatchShape(C.Structure):
 
 
 BATCH_NSHAPES = 24
+PR_NSHAPES = BATCH_NSHAPES + 6
 
 
 class RunOpts(C.Structure):
@@ -422,6 +424,12 @@ def batch_shape_at(i: int) -> tuple[int, int]:
     return s.as_tuple()
 
 
+def pr_shape_at(i: int) -> tuple[int, int, int]:
+    s = BatchShape()
+    _check(_lib_gpu().wg_pr_shape_at(i, C.byref(s)), "wg_pr_shape_at")
+    return s.as_tuple()
+
+
 def weight_f64(src: int, dst: int, seed: int) -> float:
     return float(_lib_host().wg_weight_f64(src, dst, seed))
 
@@ -621,9 +629,9 @@ class Context:
         """After init_pr: picks the fastest PageRank tile shape and re-initialises;
         returns (best shape, {shape: ms per iteration})."""
         best = BatchShape()
-        ms = np.zeros(BATCH_NSHAPES, np.float32)
+        ms = np.zeros(PR_NSHAPES, np.float32)
         self._chk(self._lib.wg_pr_autotune(self._h, iterations, C.byref(best), _ptr(ms, C.c_float)), "wg_pr_autotune")
-        return best.as_tuple(), {batch_shape_at(i): float(ms[i]) for i in range(BATCH_NSHAPES)}
+        return best.as_tuple(), {pr_shape_at(i): float(ms[i]) for i in range(PR_NSHAPES)}
 
     def read_batch_ranks(self) -> np.ndarray:
         out = np.empty(self.V, np.float64)

@@ -604,7 +604,7 @@ void wg_tile_split_points(const std::vector<uint32_t>& te, uint64_t E, uint32_t 
                                          te.begin());
 }
 
-// Tilings with 4 and 16 edges per lane for the PageRank shapes, from the
+// Tilings with 4, 16 and 2 edges per lane for the PageRank shapes, from the
 // device offsets (in device order).
 int wg_build_pr_tilings(wg_ctx* ctx)
 {
@@ -613,8 +613,8 @@ int wg_build_pr_tilings(wg_ctx* ctx)
     std::vector<uint32_t> hoff(ctx->V + 1);
     WG_HIP(ctx, hipMemcpy(hoff.data(), ctx->off, (ctx->V + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
     uint32_t max_tiles = ctx->ntiles;
-    const uint32_t ipts[2] = {4, 16};
-    for (int i = 0; i < 2; ++i) {
+    const uint32_t ipts[3] = {4, 16, 2};
+    for (int i = 0; i < 3; ++i) {
         std::vector<uint32_t> tv, te, hv, ht;
         wg_build_tiles(hoff.data(), ctx->V, WG_WAVE * ipts[i], tv, te, hv, ht);
         wg_tiling& t = ctx->pr_alt[i];

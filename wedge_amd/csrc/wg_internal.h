@@ -200,7 +200,7 @@ struct wg_ctx {
     // PageRank tile shape (wg_set_pr_shape / wg_pr_autotune): edges per lane
     // 4, 8 (the tiles above) or 16, workgroups per CU, XCD split
     int pr_ipt, pr_bpc, pr_xcd;
-    wg_tiling pr_alt[2];      // tilings with 4 and 16 edges per lane (built on first use)
+    wg_tiling pr_alt[3];      // tilings with 4, 16 and 2 edges per lane (built on first use)
     int pr_alt_built;
     uint32_t pr_split8[9];    // XCD split of the default tiles
     uint64_t bm_words;        // 64-bit words per vertex bitmask

@@ -143,7 +143,7 @@ static pr_shape current_shape(const wg_ctx* ctx)
         s.ntiles = ctx->ntiles;
         s.nheavy = ctx->nheavy;
     } else {
-        const wg_tiling& t = ctx->pr_alt[ctx->pr_ipt == 4 ? 0 : 1];
+        const wg_tiling& t = ctx->pr_alt[ctx->pr_ipt == 4 ? 0 : ctx->pr_ipt == 16 ? 1 : 2];
         s.tile_v = t.tile_v;
         s.tile_e = t.tile_e;
         s.heavy_v = t.heavy_v;
@@ -199,6 +199,8 @@ static int pr_launch(wg_ctx* ctx, int mode)
     // one bracket around tile + span: the iteration's device time
     const int slot = wg_prof_begin(ctx, mode == 0 ? WG_K_PR_TILE : WG_K_PR_ACC);
     switch (ctx->pr_ipt * 2 + mode) {
+    case 4: pr_launch_shape<T, 0, 2>(ctx, s, cin, cout, S_in, S_out, N); break;
+    case 5: pr_launch_shape<T, 1, 2>(ctx, s, cin, cout, S_in, S_out, N); break;
     case 8: pr_launch_shape<T, 0, 4>(ctx, s, cin, cout, S_in, S_out, N); break;
     case 9: pr_launch_shape<T, 1, 4>(ctx, s, cin, cout, S_in, S_out, N); break;
     case 16: pr_launch_shape<T, 0, 8>(ctx, s, cin, cout, S_in, S_out, N); break;
@@ -225,7 +227,7 @@ extern "C" int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape)
         return WG_OK;
     }
     const uint32_t ipt = shape->edges_per_lane;
-    if ((ipt != 4 && ipt != 8 && ipt != 16) || shape->blocks_per_cu < 1 || shape->blocks_per_cu > 16 ||
+    if ((ipt != 2 && ipt != 4 && ipt != 8 && ipt != 16) || shape->blocks_per_cu < 1 || shape->blocks_per_cu > 16 ||
         shape->xcd_split > 1)
         return WG_E_INVAL;
     if (ipt != WG_IPT) {
@@ -251,6 +253,19 @@ extern "C" int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape)
     return WG_OK;
 }
 
+extern "C" int wg_pr_shape_at(uint32_t i, wg_batch_shape* out)
+{
+    // one 16-wave workgroup per CU for each tile size, then 128-edge tiles
+    static const wg_batch_shape extra[WG_PR_NSHAPES - WG_BATCH_NSHAPES] = {
+        {4, 1, 0}, {8, 1, 0}, {16, 1, 0}, {2, 1, 0}, {2, 4, 0}, {2, 16, 0}};
+    if (!out || i >= WG_PR_NSHAPES)
+        return WG_E_INVAL;
+    if (i < WG_BATCH_NSHAPES)
+        return wg_batch_shape_at(i, out);
+    *out = extra[i - WG_BATCH_NSHAPES];
+    return WG_OK;
+}
+
 extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* best, float* ms_per_shape)
 {
     if (!ctx || iterations == 0)
@@ -269,13 +284,9 @@ extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* 
     int rc = WG_OK;
     float best_ms = 0.0f;
     wg_batch_shape best_s = {WG_IPT, 8, 0};
-    // the batch's candidates, then one 16-wave workgroup per CU for each tile size
-    for (uint32_t i = 0; i < WG_BATCH_NSHAPES + 3 && rc == WG_OK; ++i) {
+    for (uint32_t i = 0; i < WG_PR_NSHAPES && rc == WG_OK; ++i) {
         wg_batch_shape s;
-        if (i < WG_BATCH_NSHAPES)
-            wg_batch_shape_at(i, &s);
-        else
-            s = wg_batch_shape{4u << (i - WG_BATCH_NSHAPES), 1u, 0u};
+        wg_pr_shape_at(i, &s);
         if ((rc = wg_set_pr_shape(ctx, &s)) || (rc = wg_pr_iteration(ctx, nullptr))) // warm-up
             break;
         float ms = 0.0f;
@@ -297,7 +308,7 @@ extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* 
         }
         if (rc)
             break;
-        if (ms_per_shape && i < WG_BATCH_NSHAPES)
+        if (ms_per_shape)
             ms_per_shape[i] = ms;
         if (i == 0 || ms < best_ms) {
             best_ms = ms;
