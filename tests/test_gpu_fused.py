@@ -281,3 +281,55 @@ def test_hub_overflows_the_queue(ctx):
     assert np.all(W.bfs_parent(p1)[3:] == 2)
     (c0, _, _), (c1, _, _) = _pair_values(ctx, ctx.init_cc, ctx.read_cc, W.SCHED_FRONTIER)
     assert np.array_equal(c0, c1) and np.all(c1 == 0)
+
+
+# ---- first level pushed along the roots' out-edges ---------------------------------
+
+
+@pytest.mark.parametrize("nroots", [1, 3, 64, 65])
+def test_first_level_push_matches_block_pull(ctx, nroots):
+    """wg_converge pushes a frontier-filtered first level along the out-edges of
+    up to 64 roots instead of pulling over marked blocks; the host-driven
+    loop always pulls.  Same depths and parents, both against the oracle."""
+    g = W.rmat(12, 8, seed=21)
+    ctx.load(g)
+    rng = np.random.default_rng(nroots)
+    roots = rng.choice(g.num_vertices, nroots, replace=False).astype(np.uint64)
+    roots[0] = g.max_outdegree_vertex()
+    ctx.init_bfs_roots(roots)
+    it_dev = ctx.converge(W.SCHED_FRONTIER)
+    dev = ctx.read_bfs()
+    # host-driven: mark every root's blocks, then one pull per level
+    ctx.init_bfs_roots(roots)
+    for r in np.unique(roots):
+        ctx.wedge_activate_single_vertex(int(r))
+    levels = 0
+    while True:
+        act = ctx.edge_pull(W.APP_BFS, W.MODE_FRONTIER)
+        levels += 1
+        if act == 0:
+            break
+        ctx.wedge_generate_frontier()
+        ctx.advance_frontier()
+    host = ctx.read_bfs()
+    assert np.array_equal(dev, host)
+    assert it_dev[0] == levels
+    _check_against_oracle(g, roots, dev)
+
+
+def test_converge_after_a_host_pull_does_not_push(ctx):
+    """After a host-driven first level the frontier is no longer the root: the
+    device loop must pull its first level over the marked blocks."""
+    g = W.rmat(12, 8, seed=22)
+    ctx.load(g)
+    root = g.max_outdegree_vertex()
+    ctx.init_bfs(root)
+    ctx.converge(W.SCHED_FRONTIER)
+    want = ctx.read_bfs()
+    ctx.init_bfs(root)
+    ctx.wedge_activate_single_vertex(root)
+    ctx.edge_pull(W.APP_BFS, W.MODE_FRONTIER)
+    ctx.wedge_generate_frontier()
+    ctx.advance_frontier()
+    ctx.converge(W.SCHED_FRONTIER)
+    assert np.array_equal(ctx.read_bfs(), want)

@@ -36,6 +36,7 @@
 #define WG_HEAVY_BIT 0x80000000u              // tile_v flag: heavy tile
 #define WG_BFS_SEQ 16                         // edges a lane scans alone in dense BFS
 #define WG_BFS_HEAVY 2048                     // in-degree above which dense BFS splits a vertex
+#define WG_PUSH_ROOTS_MAX 64u                 // roots up to which wg_converge pushes the first BFS level
 
 // Counts that many workgroups add to are spread over WG_CSLOTS slots, one
 // 256-byte line each: atomics from many workgroups on one address serialise
@@ -220,6 +221,8 @@ struct wg_ctx {
     uint32_t* roots;          // BFS / SSSP roots in device numbering, distinct
     uint32_t nroots;
     unsigned long long root_activity; // summed out-degree of the roots
+    int roots_frontier;       // the BFS frontier is still the roots (no pull or advance since the init)
+    uint32_t push_roots;      // roots the running wg_converge pushes on its first level (0: none)
     struct wg_members* members;      // batch of member graphs (wg_set_members), NULL if none
     // fused small-frontier levels of the device loop (k_bfs_run)
     uint32_t run_wgs[8];      // workgroups of a run, by WG_APP_*, 0 = off

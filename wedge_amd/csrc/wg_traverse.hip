@@ -312,6 +312,46 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
     add_tally(act, ctr);
 }
 
+// The first frontier-filtered level of a traversal whose frontier is still its
+// roots.  The pull over the blocks marked by the roots' out-edges
+// (source/phase_edge_pull_bfs.c over source/wedge.c:63-155) tests exactly the
+// pairs (root r, out-neighbour v) with v unvisited, and gives v depth 1 and the
+// smallest such r as parent.  Going along the roots' out-edges (out-edge
+// positions, then their destinations) reaches the same pairs with the same
+// atomics, without marking blocks or reading the other edges of each block.
+__device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t nroots,
+                                    const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
+                                    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ od,
+                                    const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
+                                    uint64_t* __restrict__ packed, const uint32_t* __restrict__ perm,
+                                    wg_dev_counters* __restrict__ ctr)
+{
+    const uint64_t newdepth = ctr->level + 1u;
+    const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    act_tally act = { 0ull, 0ull, 0ull };
+    for (uint32_t i = 0; i < nroots; ++i) {
+        const uint32_t r = roots[i];
+        const uint64_t val = (newdepth << 32) | (uint64_t)(perm ? perm[r] : r);
+        const uint64_t s = csri_off[r], e = csri_off[r + 1];
+        for (uint64_t p = s + tid; p < e; p += nthreads) {
+            const uint32_t v = dst[csri_pos[p]];
+            if (!test_bit64(visited, v)) {
+                // packed[v] of an unvisited vertex holds ~0 until the first
+                // write of this level: that writer alone sets and counts v
+                const unsigned long long old = atomicMin((unsigned long long*)&packed[v], (unsigned long long)val);
+                if (old == ~0ull) {
+                    atomicOr((unsigned long long*)&next[v >> 6], 1ull << (v & 63));
+                    act.edges += od[v];
+                    act.verts += 1ull;
+                }
+            }
+        }
+    }
+    act.fresh = act.verts; // a BFS activation is always a first visit
+    add_tally(act, ctr);
+}
+
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
@@ -960,7 +1000,10 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
                                                          uint32_t V, uint32_t E, uint32_t* __restrict__ wedge,
                                                          const uint32_t* __restrict__ perm, wg_dev_counters* __restrict__ ctr,
                                                          const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
-                                                         uint32_t nheavy, const uint32_t* __restrict__ first)
+                                                         uint32_t nheavy, const uint32_t* __restrict__ first,
+                                                         const uint32_t* __restrict__ roots, uint32_t npush,
+                                                         const uint32_t* __restrict__ csri_off,
+                                                         const uint32_t* __restrict__ csri_pos)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
@@ -970,7 +1013,9 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
         bfs_dense_body(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
         // and the high in-degree destinations the dense body leaves out
         bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
-    } else
+    } else if (npush && ctr->iters == 0u) // the frontier is the npush roots; no blocks were marked
+        bfs_push_roots_body(roots, npush, csri_off, csri_pos, dst, od, visited, next, packed, perm, ctr);
+    else
         bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
 
@@ -1592,6 +1637,7 @@ static int init_common(wg_ctx* ctx, int app)
         return rc;
     ctx->app = app;
     ctx->iter = 0;
+    ctx->roots_frontier = 0;
     // the 256-edge tiles of the dense SSSP pulls (built once per graph)
     if (app == WG_APP_SSSP && (rc = wg_build_pr_tilings(ctx)))
         return rc;
@@ -1665,6 +1711,7 @@ static int init_roots(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t n)
     if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
         return rc;
     ctx->root_activity = ctx->hctr->activity;
+    ctx->roots_frontier = app == WG_APP_BFS;
     return WG_OK;
 }
 
@@ -1797,6 +1844,7 @@ extern "C" int wg_wedge_generate_frontier(wg_ctx* ctx, int block_log2)
 static int finish_pull(wg_ctx* ctx, uint64_t* edges_activated)
 {
     ctx->iter += 1;
+    ctx->roots_frontier = 0;
     if (!edges_activated)
         return WG_OK;
     int rc = wg_read_counters(ctx);
@@ -1978,6 +2026,7 @@ extern "C" int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices)
 {
     if (!ctx || ctx->app == WG_APP_NONE || ctx->app == WG_APP_PR)
         return WG_E_STATE;
+    ctx->roots_frontier = 0;
     int rc = wg_reset_counters(ctx, RST_FRONTIER);
     if (rc)
         return rc;
@@ -2065,7 +2114,8 @@ static int enqueue_iteration(wg_ctx* ctx)
         hipLaunchKernelGGL(k_bfs_level, dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
-                           ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src);
+                           ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src, ctx->roots, ctx->push_roots,
+                           ctx->csri_off, ctx->csri_pos);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
@@ -2180,13 +2230,21 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
                        ctx->app == WG_APP_CC ? 1 : 0,
                        ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - ctx->nroots), (unsigned int)ctx->app,
                        (unsigned long long)ctx->V, (unsigned long long)ctx->E);
+    // A BFS that starts from its (few) roots pushes its first level along the
+    // roots' out-edges if that level is frontier-filtered (k_bfs_level), so
+    // no blocks are marked for it.
+    ctx->push_roots =
+        (ctx->app == WG_APP_BFS && ctx->roots_frontier && ctx->nroots <= WG_PUSH_ROOTS_MAX) ? ctx->nroots : 0u;
+    ctx->roots_frontier = 0;
     if (ctx->app != WG_APP_CC) { // the roots' blocks, if the first pull is frontier-filtered
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr, RST_HEAVY);
-        hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->bm_cur,
-                           ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list, ctx->heavy_cap,
-                           ctx->ctr, WG_GATE_FRONTIER);
-        hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
-                           ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+        if (!ctx->push_roots) {
+            hipLaunchKernelGGL(k_wedge_build, dim3(wg_wedge_build_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream,
+                               ctx->bm_cur, ctx->bm_words, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->heavy_list,
+                               ctx->heavy_cap, ctx->ctr, WG_GATE_FRONTIER);
+            hipLaunchKernelGGL(k_wedge_heavy, dim3(ctx->num_cu * 2), dim3(WG_BLOCK), 0, ctx->stream, ctx->heavy_list,
+                               ctx->heavy_cap, ctx->csri_off, ctx->csri_pos, 0, ctx->wedge, ctx->ctr, WG_GATE_FRONTIER);
+        }
     }
     if ((rc = wg_check_launch(ctx)))
         return rc;
