@@ -317,8 +317,9 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 // (source/phase_edge_pull_bfs.c over source/wedge.c:63-155) tests exactly the
 // pairs (root r, out-neighbour v) with v unvisited, and gives v depth 1 and the
 // smallest such r as parent.  Going along the roots' out-edges (out-edge
-// positions, then their destinations) reaches the same pairs with the same
-// atomics, without marking blocks or reading the other edges of each block.
+// positions, then their destinations) reaches the same pairs and keeps the
+// same minimum, without marking blocks or reading the other edges of each
+// block.
 __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t nroots,
                                     const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
                                     const uint32_t* __restrict__ dst, const uint32_t* __restrict__ od,
