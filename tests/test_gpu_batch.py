@@ -49,6 +49,29 @@ def test_msbfs_matches_single_source_oracle(ctx, nroots):
             assert reached == sum(int(od[d != np.uint32(2**32 - 1)].sum()) for d in want)
 
 
+def test_msbfs_reuses_buffers_without_stale_depths(ctx):
+    """Consecutive runs on one graph keep their buffers and do not clear the
+    depth matrix: entries of sources that did not reach a vertex must read as
+    unreached even where an earlier run wrote a depth."""
+    g = W.rmat(12, 4, seed=6)  # directed and sparse: roots reach different sets
+    ctx.load(g)
+    orc = oracle.Oracle.of(g, 4)
+    first = _roots(g, 64, seed=11)
+    ctx.msbfs(first)
+    rng = np.random.default_rng(12)
+    od = g.outdegrees()
+    # sources with few out-edges, so that most vertices stay unreached
+    low = np.flatnonzero((od > 0) & (od <= 2))
+    second = rng.choice(low, 5, replace=False).astype(np.uint64)
+    for roots in (second, first[:7]):
+        ctx.msbfs(roots)
+        for i in range(len(roots)):
+            assert np.array_equal(ctx.read_msbfs(i), orc.bfs(int(roots[i]))[1]), f"root #{i}"
+    ctx.init_bfs(int(first[0]))  # another application frees the buffers
+    ctx.msbfs(second)
+    assert np.array_equal(ctx.read_msbfs(4), orc.bfs(int(second[4]))[1])
+
+
 def test_msbfs_heavy_destinations(ctx):
     rng = np.random.default_rng(7)
     V = 6000

@@ -6,8 +6,10 @@
 //   seen[v]   sources that have reached v
 //   front[v]  sources that reached v at the last level (the frontier)
 //   nl[v]     sources reaching v at the running level (filled by the pull)
-//   depth[v][64]  uint16 level at which source s reached v, 0xFFFF if never
-//                 (vertex-major: a vertex's row is written by one store)
+//   depth[v][64]  uint16 level at which source s reached v, valid where
+//                 seen[v] has bit s (vertex-major: a vertex's row is written
+//                 by one store; the entries of unreached sources are never
+//                 written, so the matrix needs no clearing between runs)
 // A dense level is the bottom-up pull of source/phase_edge_pull_bfs.c:109-210
 // for all sources at once: a vertex ORs the frontier masks of its in-edges,
 // restricted to the sources that have not reached it.  A frontier level walks the Wedge frontier
@@ -19,7 +21,6 @@
 #include <algorithm>
 #include <vector>
 
-#define MS_DEPTH_NONE 0xFFFFu
 #define MS_LEVEL_MAX 0xFFFEu
 
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long x)
@@ -272,12 +273,11 @@ __global__ void k_ms_reached(const unsigned long long* __restrict__ seen, const 
     add_count(ne, c, WG_CNT_REACHED_E);
 }
 
-__global__ void k_ms_column(const uint16_t* __restrict__ depth, uint32_t s, uint32_t V, uint32_t* __restrict__ out)
+__global__ void k_ms_column(const uint16_t* __restrict__ depth, const unsigned long long* __restrict__ seen, uint32_t s,
+                            uint32_t V, uint32_t* __restrict__ out)
 {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint16_t d = depth[v * WG_MS_MAX_SOURCES + s];
-        out[v] = d == MS_DEPTH_NONE ? 0xffffffffu : (uint32_t)d;
-    }
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = ((seen[v] >> s) & 1ull) ? (uint32_t)depth[v * WG_MS_MAX_SOURCES + s] : 0xffffffffu;
 }
 
 // ---- host side ----------------------------------------------------------------------
@@ -307,32 +307,39 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    wg_free_app_state(ctx);
     const uint32_t V = (uint32_t)ctx->V, E = (uint32_t)ctx->E;
     void *seen = nullptr, *front = nullptr, *nl = nullptr, *depth = nullptr, *droots = nullptr;
-    if ((rc = ms_alloc(ctx, (size_t)V * 8, &seen)) || (rc = ms_alloc(ctx, (size_t)V * 8, &front)) ||
-        (rc = ms_alloc(ctx, (size_t)V * 8, &nl)) || (rc = ms_alloc(ctx, (size_t)V * WG_MS_MAX_SOURCES * 2, &depth)) ||
-        (rc = ms_alloc(ctx, (size_t)nroots * 4, &droots))) {
-        (void)hipFree(seen);
-        (void)hipFree(front);
-        (void)hipFree(nl);
-        (void)hipFree(depth);
-        (void)hipFree(droots);
-        return rc;
+    if (ctx->app == WG_APP_MSBFS) { // the last run's buffers (sized by V, which a new graph frees)
+        depth = ctx->prop[0];
+        seen = ctx->ms[0];
+        front = ctx->ms[1];
+        nl = ctx->ms[2];
+    } else {
+        wg_free_app_state(ctx);
+        if ((rc = ms_alloc(ctx, (size_t)V * 8, &seen)) || (rc = ms_alloc(ctx, (size_t)V * 8, &front)) ||
+            (rc = ms_alloc(ctx, (size_t)V * 8, &nl)) ||
+            (rc = ms_alloc(ctx, (size_t)V * WG_MS_MAX_SOURCES * 2, &depth))) {
+            (void)hipFree(seen);
+            (void)hipFree(front);
+            (void)hipFree(nl);
+            (void)hipFree(depth);
+            return rc;
+        }
+        ctx->prop[0] = depth; // freed with the application state
+        ctx->ms[0] = seen;
+        ctx->ms[1] = front;
+        ctx->ms[2] = nl;
+        ctx->app = WG_APP_MSBFS;
     }
-    ctx->prop[0] = depth;  // freed with the application state
-    ctx->ms[0] = seen;
-    ctx->ms[1] = front;
-    ctx->ms[2] = nl;
+    if ((rc = ms_alloc(ctx, (size_t)nroots * 4, &droots)))
+        return rc;
     ctx->ms_n = nroots;
-    ctx->app = WG_APP_MSBFS;
     unsigned long long *dseen = (unsigned long long*)seen, *dfront = (unsigned long long*)front,
                        *dnl = (unsigned long long*)nl;
     const unsigned long long live = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
     WG_HIP(ctx, hipMemcpyAsync(droots, hroots.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, ctx->stream));
     if ((rc = wg_clear_u64(ctx, (uint64_t*)seen, V)) || (rc = wg_clear_u64(ctx, (uint64_t*)front, V)) ||
         (rc = wg_clear_u64(ctx, (uint64_t*)nl, V)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
-        (rc = wg_fill_u32(ctx, (uint32_t*)depth, (uint64_t)V * (WG_MS_MAX_SOURCES / 2), 0xffffffffu)) ||
         (rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u)))
         goto out;
     if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
@@ -413,7 +420,8 @@ extern "C" int wg_read_msbfs(wg_ctx* ctx, uint32_t source, uint32_t* host_depths
     uint32_t* tmp = nullptr;
     WG_HIP(ctx, hipMalloc((void**)&tmp, ctx->V * sizeof(uint32_t)));
     hipLaunchKernelGGL(k_ms_column, dim3(wg_grid(ctx, ctx->V, 1024, 8)), dim3(256), 0, ctx->stream,
-                       (const uint16_t*)ctx->prop[0], source, (uint32_t)ctx->V, tmp);
+                       (const uint16_t*)ctx->prop[0], (const unsigned long long*)ctx->ms[0], source, (uint32_t)ctx->V,
+                       tmp);
     int rc = wg_check_launch(ctx);
     if (!rc)
         rc = wg_read_vertex_array(ctx, tmp, host_depths, sizeof(uint32_t));
