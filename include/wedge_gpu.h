@@ -251,8 +251,8 @@ int wg_fused_level_count(wg_ctx* ctx, uint64_t* levels);
 #define WG_MS_MAX_SOURCES 64
 /* levels: BFS levels run; edges_reached: sum over sources of the out-degrees
  * of the vertices each reached (both optional).  WG_E_OVERFLOW if a depth
- * would exceed 65534.  Its device buffers (8V bytes of depth rows per source
- * lane, 64 in all, and three 8V-byte masks) stay allocated until another
+ * would exceed 65534.  Its device buffers (128V bytes of 16-bit depth rows,
+ * 64 per vertex, and three 8V-byte masks) stay allocated until another
  * application is initialised or the graph changes, so that further calls on
  * the same graph do not allocate them again. */
 int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_params* params, uint64_t* levels,
