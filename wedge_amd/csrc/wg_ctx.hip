@@ -448,6 +448,7 @@ static void free_graph(wg_ctx* ctx)
     free_dev((void**)&ctx->heavy_list);
     free_dev((void**)&ctx->run_spare);
     free_dev((void**)&ctx->first_src);
+    free_dev((void**)&ctx->csr_dst);
     free_dev((void**)&ctx->perm);
     free_dev((void**)&ctx->inv);
     ctx->V = ctx->E = 0;

@@ -177,6 +177,8 @@ struct wg_ctx {
     uint32_t nbfs_heavy;      // dense BFS chunks of vertices with in-degree > WG_BFS_HEAVY
     uint32_t* first_src;      // dense BFS: smallest in-neighbour of each vertex, UINT32_MAX if it has
                               // none or more than WG_BFS_HEAVY (built by the first BFS init)
+    uint32_t* csr_dst;        // BFS: destination of out-edge position p, dst[csri_pos[p]] (E entries,
+                              // built by the first BFS init)
     uint32_t* bfs_heavy_v;
     uint32_t* bfs_heavy_e;    // chunk start edge (chunk end = min(start + WG_BFS_HEAVY, off[v+1]))
     uint32_t n_out_pos;       // vertices with out-degree > 0

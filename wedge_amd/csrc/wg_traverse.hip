@@ -107,6 +107,15 @@ __global__ void k_first_src(const uint32_t* __restrict__ off, const uint32_t* __
     }
 }
 
+// csr_dst[p] = dst[csri_pos[p]]: the out-edges' destinations in out-edge
+// order, so that walking a vertex's out-edges takes one load per edge.
+__global__ void k_csr_dst(const uint32_t* __restrict__ csri_pos, const uint32_t* __restrict__ dst, uint64_t E,
+                          uint32_t* __restrict__ out)
+{
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < E; p += (uint64_t)gridDim.x * blockDim.x)
+        out[p] = dst[csri_pos[p]];
+}
+
 // Dense bottom-up level: one wave per 64 consecutive vertices.  A lane first
 // tests its vertex's smallest in-neighbour (first[], read coalesced): after
 // the degree relabelling that is usually a hub, and most vertices of a dense
@@ -316,13 +325,13 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 // roots.  The pull over the blocks marked by the roots' out-edges
 // (source/phase_edge_pull_bfs.c over source/wedge.c:63-155) tests exactly the
 // pairs (root r, out-neighbour v) with v unvisited, and gives v depth 1 and the
-// smallest such r as parent.  Going along the roots' out-edges (out-edge
-// positions, then their destinations) reaches the same pairs and keeps the
+// smallest such r as parent.  Going along the roots' out-edges (their
+// destinations in out-edge order, csr_dst) reaches the same pairs and keeps the
 // same minimum, without marking blocks or reading the other edges of each
 // block.
 __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t nroots,
-                                    const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csri_pos,
-                                    const uint32_t* __restrict__ dst, const uint32_t* __restrict__ od,
+                                    const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csr_dst,
+                                    const uint32_t* __restrict__ od,
                                     const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
                                     uint64_t* __restrict__ packed, const uint32_t* __restrict__ perm,
                                     wg_dev_counters* __restrict__ ctr)
@@ -336,7 +345,7 @@ __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t
         const uint64_t val = (newdepth << 32) | (uint64_t)(perm ? perm[r] : r);
         const uint64_t s = csri_off[r], e = csri_off[r + 1];
         for (uint64_t p = s + tid; p < e; p += nthreads) {
-            const uint32_t v = dst[csri_pos[p]];
+            const uint32_t v = csr_dst[p];
             if (!test_bit64(visited, v)) {
                 // packed[v] of an unvisited vertex holds ~0 until the first
                 // write of this level: that writer alone sets and counts v
@@ -1004,7 +1013,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
                                                          uint32_t nheavy, const uint32_t* __restrict__ first,
                                                          const uint32_t* __restrict__ roots, uint32_t npush,
                                                          const uint32_t* __restrict__ csri_off,
-                                                         const uint32_t* __restrict__ csri_pos)
+                                                         const uint32_t* __restrict__ csr_dst)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
@@ -1015,7 +1024,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
         // and the high in-degree destinations the dense body leaves out
         bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
     } else if (npush && ctr->iters == 0u) // the frontier is the npush roots; no blocks were marked
-        bfs_push_roots_body(roots, npush, csri_off, csri_pos, dst, od, visited, next, packed, perm, ctr);
+        bfs_push_roots_body(roots, npush, csri_off, csr_dst, od, visited, next, packed, perm, ctr);
     else
         bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
@@ -1421,8 +1430,7 @@ __device__ __forceinline__ uint32_t run_queue_len(const wg_run_ctl* rc, unsigned
 }
 
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict__ csri_off,
-                                                       const uint32_t* __restrict__ csri_pos,
-                                                       const uint32_t* __restrict__ dst, uint64_t* __restrict__ next,
+                                                       const uint32_t* __restrict__ csr_dst, uint64_t* __restrict__ next,
                                                        uint64_t* __restrict__ visited, uint64_t* __restrict__ packed,
                                                        uint64_t nwords, const uint32_t* __restrict__ perm,
                                                        uint4* __restrict__ qa, uint4* __restrict__ qb,
@@ -1467,7 +1475,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
 #pragma unroll
                 for (int j = 0; j < (int)WG_RUN_CH; ++j)
                     if (en.y + (uint32_t)j < en.z)
-                        vv[j] = dst[csri_pos[en.y + (uint32_t)j]];
+                        vv[j] = csr_dst[en.y + (uint32_t)j];
 #pragma unroll
                 for (int j = 0; j < (int)WG_RUN_CH; ++j)
                     if (en.y + (uint32_t)j < en.z)
@@ -1647,6 +1655,16 @@ static int init_common(wg_ctx* ctx, int app)
         WG_HIP(ctx, hipMalloc((void**)&ctx->first_src, (ctx->V ? ctx->V : 1) * sizeof(uint32_t)));
         hipLaunchKernelGGL(k_first_src, dim3(wg_grid(ctx, ctx->V, 256, 8)), dim3(256), 0, ctx->stream, ctx->off, ctx->src,
                            ctx->V, ctx->first_src);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+    }
+    // the out-edges' destinations in out-edge order, for the first level's
+    // push and the fused levels (once per graph)
+    if (app == WG_APP_BFS && !ctx->csr_dst) {
+        WG_HIP(ctx, hipMalloc((void**)&ctx->csr_dst, (ctx->E ? ctx->E : 1) * sizeof(uint32_t)));
+        if (ctx->E)
+            hipLaunchKernelGGL(k_csr_dst, dim3(wg_grid(ctx, ctx->E, 1024, 8)), dim3(256), 0, ctx->stream, ctx->csri_pos,
+                               ctx->dst, ctx->E, ctx->csr_dst);
         if ((rc = wg_check_launch(ctx)))
             return rc;
     }
@@ -2116,7 +2134,7 @@ static int enqueue_iteration(wg_ctx* ctx)
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
                            ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src, ctx->roots, ctx->push_roots,
-                           ctx->csri_off, ctx->csri_pos);
+                           ctx->csri_off, ctx->csr_dst);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
@@ -2147,7 +2165,7 @@ static int enqueue_iteration(wg_ctx* ctx)
         const dim3 rg(run_wgs), rb(WG_BLOCK);
         uint4* qb = ctx->run_q + WG_RUN_QCAP;
         if (ctx->app == WG_APP_BFS) {
-            hipLaunchKernelGGL(k_bfs_run, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->bm_next,
+            hipLaunchKernelGGL(k_bfs_run, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csr_dst, ctx->bm_next,
                                ctx->bm_visited, packed, ctx->bm_words, ctx->perm, ctx->run_q, qb, ctx->run_ctl, ctx->ctr,
                                mlp);
         } else {
