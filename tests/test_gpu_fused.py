@@ -286,12 +286,12 @@ def test_hub_overflows_the_queue(ctx):
 # ---- first level pushed along the roots' out-edges ---------------------------------
 
 
-@pytest.mark.parametrize("nroots", [1, 3, 64, 65])
+@pytest.mark.parametrize("nroots", [1, 3, 64, 300, 4096, 4200])
 def test_first_level_push_matches_block_pull(ctx, nroots):
     """wg_converge pushes a frontier-filtered first level along the out-edges of
-    up to 64 roots instead of pulling over marked blocks; the host-driven
+    up to 4096 roots instead of pulling over marked blocks; the host-driven
     loop always pulls.  Same depths and parents, both against the oracle."""
-    g = W.rmat(12, 8, seed=21)
+    g = W.rmat(13, 8, seed=21)
     ctx.load(g)
     rng = np.random.default_rng(nroots)
     roots = rng.choice(g.num_vertices, nroots, replace=False).astype(np.uint64)

@@ -406,6 +406,7 @@ int wg_free_app_state(wg_ctx* ctx)
     ctx->ms_n = 0;
     free_dev((void**)&ctx->acc);
     free_dev((void**)&ctx->roots);
+    free_dev((void**)&ctx->root_off);
     ctx->nroots = 0;
     ctx->app = WG_APP_NONE;
     return WG_OK;

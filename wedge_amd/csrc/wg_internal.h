@@ -36,7 +36,7 @@
 #define WG_HEAVY_BIT 0x80000000u              // tile_v flag: heavy tile
 #define WG_BFS_SEQ 16                         // edges a lane scans alone in dense BFS
 #define WG_BFS_HEAVY 2048                     // in-degree above which dense BFS splits a vertex
-#define WG_PUSH_ROOTS_MAX 64u                 // roots up to which wg_converge pushes the first BFS level
+#define WG_PUSH_ROOTS_MAX 4096u               // roots up to which wg_converge pushes the first BFS level
 
 // Counts that many workgroups add to are spread over WG_CSLOTS slots, one
 // 256-byte line each: atomics from many workgroups on one address serialise
@@ -221,6 +221,8 @@ struct wg_ctx {
     wg_dev_counters* hctr;    // pinned host mirror
     uint32_t root_dev;        // BFS / SSSP root in device numbering (the first root)
     uint32_t* roots;          // BFS / SSSP roots in device numbering, distinct
+    uint32_t* root_off;       // BFS, at most WG_PUSH_ROOTS_MAX roots: exclusive prefix of the roots'
+                              // out-degrees (nroots + 1 entries), for the first level's push
     uint32_t nroots;
     unsigned long long root_activity; // summed out-degree of the roots
     int roots_frontier;       // the BFS frontier is still the roots (no pull or advance since the init)

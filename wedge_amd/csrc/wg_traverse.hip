@@ -328,8 +328,11 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 // smallest such r as parent.  Going along the roots' out-edges (their
 // destinations in out-edge order, csr_dst) reaches the same pairs and keeps the
 // same minimum, without marking blocks or reading the other edges of each
-// block.
+// block.  The out-edges of all roots are numbered one after the other
+// (root_off: the prefix of the roots' out-degrees) and spread over the grid;
+// each thread finds the root of its edge by binary search.
 __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t nroots,
+                                    const uint32_t* __restrict__ root_off,
                                     const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csr_dst,
                                     const uint32_t* __restrict__ od,
                                     const uint64_t* __restrict__ visited, uint64_t* __restrict__ next,
@@ -339,27 +342,67 @@ __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t
     const uint64_t newdepth = ctr->level + 1u;
     const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = root_off[nroots];
     act_tally act = { 0ull, 0ull, 0ull };
-    for (uint32_t i = 0; i < nroots; ++i) {
-        const uint32_t r = roots[i];
+    for (uint64_t g = tid; g < total; g += nthreads) {
+        // the last root whose out-edges start at or before g (roots without
+        // out-edges share their successor's start and are passed over)
+        uint32_t lo = 0, hi = nroots - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (root_off[mid] <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t r = roots[lo];
         const uint64_t val = (newdepth << 32) | (uint64_t)(perm ? perm[r] : r);
-        const uint64_t s = csri_off[r], e = csri_off[r + 1];
-        for (uint64_t p = s + tid; p < e; p += nthreads) {
-            const uint32_t v = csr_dst[p];
-            if (!test_bit64(visited, v)) {
-                // packed[v] of an unvisited vertex holds ~0 until the first
-                // write of this level: that writer alone sets and counts v
-                const unsigned long long old = atomicMin((unsigned long long*)&packed[v], (unsigned long long)val);
-                if (old == ~0ull) {
-                    atomicOr((unsigned long long*)&next[v >> 6], 1ull << (v & 63));
-                    act.edges += od[v];
-                    act.verts += 1ull;
-                }
+        const uint32_t v = csr_dst[csri_off[r] + (uint32_t)(g - root_off[lo])];
+        if (!test_bit64(visited, v)) {
+            // packed[v] of an unvisited vertex holds ~0 until the first
+            // write of this level: that writer alone sets and counts v
+            const unsigned long long old = atomicMin((unsigned long long*)&packed[v], (unsigned long long)val);
+            if (old == ~0ull) {
+                atomicOr((unsigned long long*)&next[v >> 6], 1ull << (v & 63));
+                act.edges += od[v];
+                act.verts += 1ull;
             }
         }
     }
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+}
+
+// root_off[i] = out-degree sum of roots[0..i), root_off[n] = the total; one
+// workgroup of WG_BLOCK threads, each summing a contiguous run of roots.
+__global__ __launch_bounds__(WG_BLOCK) void k_root_offsets(const uint32_t* __restrict__ roots, uint32_t n,
+                                                            const uint32_t* __restrict__ csri_off,
+                                                            uint32_t* __restrict__ root_off)
+{
+    __shared__ uint32_t s_sum[WG_BLOCK];
+    const uint32_t per = (n + WG_BLOCK - 1) / WG_BLOCK;
+    const uint32_t b = threadIdx.x * per < n ? threadIdx.x * per : n;
+    const uint32_t e = b + per < n ? b + per : n;
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i)
+        sum += csri_off[roots[i] + 1] - csri_off[roots[i]];
+    s_sum[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int t = 0; t < WG_BLOCK; ++t) {
+            const uint32_t x = s_sum[t];
+            s_sum[t] = acc;
+            acc += x;
+        }
+        root_off[n] = acc;
+    }
+    __syncthreads();
+    uint32_t acc = s_sum[threadIdx.x];
+    for (uint32_t i = b; i < e; ++i) {
+        root_off[i] = acc;
+        acc += csri_off[roots[i] + 1] - csri_off[roots[i]];
+    }
 }
 
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
@@ -1012,6 +1055,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
                                                          const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
                                                          uint32_t nheavy, const uint32_t* __restrict__ first,
                                                          const uint32_t* __restrict__ roots, uint32_t npush,
+                                                         const uint32_t* __restrict__ root_off,
                                                          const uint32_t* __restrict__ csri_off,
                                                          const uint32_t* __restrict__ csr_dst)
 {
@@ -1024,7 +1068,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
         // and the high in-degree destinations the dense body leaves out
         bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
     } else if (npush && ctr->iters == 0u) // the frontier is the npush roots; no blocks were marked
-        bfs_push_roots_body(roots, npush, csri_off, csr_dst, od, visited, next, packed, perm, ctr);
+        bfs_push_roots_body(roots, npush, root_off, csri_off, csr_dst, od, visited, next, packed, perm, ctr);
     else
         bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
@@ -1731,6 +1775,13 @@ static int init_roots(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t n)
         return rc;
     ctx->root_activity = ctx->hctr->activity;
     ctx->roots_frontier = app == WG_APP_BFS;
+    if (app == WG_APP_BFS && ctx->nroots <= WG_PUSH_ROOTS_MAX) { // for the first level's push
+        WG_HIP(ctx, hipMalloc((void**)&ctx->root_off, (ctx->nroots + 1) * sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_root_offsets, dim3(1), dim3(WG_BLOCK), 0, ctx->stream, ctx->roots, ctx->nroots,
+                           ctx->csri_off, ctx->root_off);
+        if ((rc = wg_check_launch(ctx)))
+            return rc;
+    }
     return WG_OK;
 }
 
@@ -2134,7 +2185,7 @@ static int enqueue_iteration(wg_ctx* ctx)
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
                            ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src, ctx->roots, ctx->push_roots,
-                           ctx->csri_off, ctx->csr_dst);
+                           ctx->root_off, ctx->csri_off, ctx->csr_dst);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
@@ -2249,11 +2300,12 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
                        ctx->app == WG_APP_CC ? 1 : 0,
                        ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - ctx->nroots), (unsigned int)ctx->app,
                        (unsigned long long)ctx->V, (unsigned long long)ctx->E);
-    // A BFS that starts from its (few) roots pushes its first level along the
+    // A BFS that starts from its roots (at most WG_PUSH_ROOTS_MAX, whose
+    // out-edge offsets the init computed) pushes its first level along the
     // roots' out-edges if that level is frontier-filtered (k_bfs_level), so
     // no blocks are marked for it.
     ctx->push_roots =
-        (ctx->app == WG_APP_BFS && ctx->roots_frontier && ctx->nroots <= WG_PUSH_ROOTS_MAX) ? ctx->nroots : 0u;
+        (ctx->app == WG_APP_BFS && ctx->roots_frontier && ctx->root_off) ? ctx->nroots : 0u;
     ctx->roots_frontier = 0;
     if (ctx->app != WG_APP_CC) { // the roots' blocks, if the first pull is frontier-filtered
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, ctx->stream, ctx->ctr, RST_HEAVY);
