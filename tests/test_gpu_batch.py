@@ -127,6 +127,27 @@ def test_ppr_matches_oracle(ctx, nseeds):
         print(f"PPR {name} {nseeds} seeds: max L1 {max(np.abs(ctx.read_ppr(i) - want[i]).sum() for i in range(nseeds)):.2e}")
 
 
+def test_ppr_reuses_buffers_across_calls(ctx):
+    """Calls on one graph keep their buffers: a call with fewer seeds and other
+    iteration counts after a 32-seed call must still match the oracle, also
+    after the PageRank tile shape was changed in between."""
+    g = W.rmat(12, 16, seed=5)
+    ctx.load(g)
+    orc = oracle.Oracle.of(g, 4)
+    ctx.ppr(_roots(g, 32, seed=21), 9, 0.85)
+    for seeds, iters in ((_roots(g, 3, seed=22), 12), (_roots(g, 7, seed=23), 5)):
+        ctx.ppr(seeds, iters, 0.85)
+        want = orc.ppr(seeds, iters, 0.85)
+        for i in range(len(seeds)):
+            assert np.abs(ctx.read_ppr(i) - want[i]).sum() <= PPR_TOL, f"{len(seeds)} seeds, seed #{i}"
+    ctx.set_pr_shape((16, 2, 1))
+    seeds = _roots(g, 5, seed=24)
+    ctx.ppr(seeds, 6, 0.85)
+    want = orc.ppr(seeds, 6, 0.85)
+    assert all(np.abs(ctx.read_ppr(i) - want[i]).sum() <= PPR_TOL for i in range(len(seeds)))
+    ctx.set_pr_shape(None)
+
+
 def test_ppr_is_reproducible_and_checks_arguments(ctx):
     g = W.rmat(12, 16, seed=2)
     ctx.load(g)

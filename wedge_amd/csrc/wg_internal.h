@@ -191,6 +191,7 @@ struct wg_ctx {
     void* prop[2];            // PR contributions (double buffer) / BFS packed / CC labels / SSSP dist
     void* ms[3];              // batched sources: per-vertex masks or rows (freed with the app state)
     uint32_t ms_n;            // batched sources: number of roots / seeds
+    size_t ms_aux_bytes;      // personalised PageRank: size of ms[2] (its reduction partials)
     int pr_cur;               // PR: index of the current contribution buffer
     int pr_parity;            // PR: index of S[] holding the current iteration's sum
     int pr_fp32;

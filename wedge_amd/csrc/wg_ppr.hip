@@ -236,7 +236,6 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
             return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    wg_free_app_state(ctx);
     const uint64_t V = ctx->V;
     const uint32_t grid = ctx->pr_grid, sgrid = ctx->span_grid;
     const uint32_t nw = grid * WG_WPB, ns = sgrid * WG_WPB;
@@ -244,22 +243,34 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
     // aux: seeds (32 u32) | D[2][32] | tile partials [ntiles][32] | wave partials | span partials |
     //      reduction block partials [PPR_RED_BLOCKS][32] | reduction ticket (u32)
     const size_t aux_doubles = 2 * PPR_LANES + ((size_t)ctx->ntiles + nw + ns + PPR_RED_BLOCKS) * PPR_LANES;
-    if (hipMalloc(&C0, V * PPR_LANES * sizeof(float)) != hipSuccess ||
-        hipMalloc(&C1, V * PPR_LANES * sizeof(float)) != hipSuccess ||
-        hipMalloc(&R, V * PPR_LANES * sizeof(double)) != hipSuccess ||
-        hipMalloc(&aux, PPR_LANES * sizeof(uint32_t) + aux_doubles * sizeof(double) + sizeof(unsigned int)) != hipSuccess) {
-        (void)hipFree(C0);
-        (void)hipFree(C1);
-        (void)hipFree(R);
-        (void)hipFree(aux);
-        return WG_E_NOMEM;
+    const size_t aux_bytes = PPR_LANES * sizeof(uint32_t) + aux_doubles * sizeof(double) + sizeof(unsigned int);
+    if (ctx->app == WG_APP_PPR && ctx->ms_aux_bytes == aux_bytes) {
+        // the last call's buffers (sized by V, which a new graph frees): the
+        // init below rewrites C0, every iteration the other matrix, the last
+        // one R, and the partials are rewritten before they are read
+        C0 = ctx->ms[0];
+        C1 = ctx->ms[1];
+        aux = ctx->ms[2];
+        R = ctx->prop[0];
+    } else {
+        wg_free_app_state(ctx);
+        if (hipMalloc(&C0, V * PPR_LANES * sizeof(float)) != hipSuccess ||
+            hipMalloc(&C1, V * PPR_LANES * sizeof(float)) != hipSuccess ||
+            hipMalloc(&R, V * PPR_LANES * sizeof(double)) != hipSuccess || hipMalloc(&aux, aux_bytes) != hipSuccess) {
+            (void)hipFree(C0);
+            (void)hipFree(C1);
+            (void)hipFree(R);
+            (void)hipFree(aux);
+            return WG_E_NOMEM;
+        }
+        ctx->ms[0] = C0;
+        ctx->ms[1] = C1;
+        ctx->ms[2] = aux;
+        ctx->prop[0] = R;
+        ctx->ms_aux_bytes = aux_bytes;
+        ctx->app = WG_APP_PPR;
     }
-    ctx->ms[0] = C0;
-    ctx->ms[1] = C1;
-    ctx->ms[2] = aux;
-    ctx->prop[0] = R;
     ctx->ms_n = nseeds;
-    ctx->app = WG_APP_PPR;
     uint32_t* dseeds = (uint32_t*)aux;
     double* D = (double*)((char*)aux + PPR_LANES * sizeof(uint32_t));
     double* part = D + 2 * PPR_LANES;
