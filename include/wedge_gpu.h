@@ -274,7 +274,10 @@ int wg_read_mssssp(wg_ctx* ctx, uint32_t source, void* host_dist);
  * and its dangling mass returns to s; starting from all mass on s:
  *   r(v) = d * (sum over in-edges u->v of r(u)/od(u) + [v = s] * D) + [v = s] * (1 - d)
  * with D the mass on vertices without out-edges.  Contributions are stored in
- * FP32 (a 128-byte row per vertex), sums and ranks are FP64, in a fixed order. */
+ * FP32 (a 128-byte row per vertex), sums and ranks are FP64, in a fixed order.
+ * The device buffers (2 x 128V bytes of contributions, 256V bytes of ranks,
+ * partials) stay allocated until another application is initialised or the
+ * graph changes, so that further calls on the same graph reuse them. */
 #define WG_PPR_MAX_SEEDS 32
 int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping);
 /* Ranks (V entries, FP64) for seed number `seed`. */
