@@ -107,6 +107,33 @@ def check_bfs_tree(g, root, depth, parent):
     return bad
 
 
+def min_frontier_parent_vec(g, depth, chunk=1 << 26):
+    """min_frontier_parent for large graphs: per destination, the smallest
+    in-neighbour one level up, with segment minima over the destination-grouped
+    in-lists (np.minimum.reduceat) taken in chunks of whole destinations."""
+    V = g.num_vertices
+    off = g.offsets.astype(np.int64)
+    src = g.sources
+    out = np.full(V, U32_MAX, np.uint32)
+    lvl = depth.astype(np.int64)
+    lvl[depth == U32_MAX] = -2  # unreached: no level matches
+    v0 = 0
+    while v0 < V:
+        # whole destinations whose in-edges fit the chunk (at least one)
+        v1 = int(np.searchsorted(off, off[v0] + chunk, side="right")) - 1
+        v1 = min(max(v1, v0 + 1), V)
+        a, b = int(off[v0]), int(off[v1])
+        if b > a:
+            s = src[a:b]
+            dd = np.repeat(lvl[v0:v1], np.diff(off[v0:v1 + 1]))
+            ok = (lvl[s] + 1 == dd) & (dd >= 1)
+            cand = np.where(ok, s, U32_MAX).astype(np.uint32)
+            nz = np.nonzero(off[v0 + 1:v1 + 1] > off[v0:v1])[0]
+            out[v0 + nz] = np.minimum.reduceat(cand, off[v0 + nz] - a)
+        v0 = v1
+    return out
+
+
 def min_frontier_parent(g, depth):
     """Smallest in-neighbour one level up, for every reached non-root vertex
     (U32_MAX elsewhere): the parent the HIP path defines."""

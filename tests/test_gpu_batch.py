@@ -105,7 +105,7 @@ def test_msbfs_errors(ctx):
 
 # ---- personalised PageRank ---------------------------------------------------------------
 
-PPR_TOL = 1e-5  # L1 per seed: FP32-stored contributions (expected ~1e-7), FP64 sums
+PPR_TOL = 2e-7  # L1 per seed: FP32-stored contributions (about 10x the measured error), FP64 sums
 
 
 @pytest.mark.parametrize("nseeds", [1, 7, 32])

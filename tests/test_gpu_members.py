@@ -16,7 +16,7 @@ from helpers import U32_MAX, check_bfs_tree, min_frontier_parent
 pytestmark = pytest.mark.gpu
 
 PR_TOL = 1e-10  # L1 per member, FP64 contributions
-PR_TOL_FP32 = 1e-5  # L1 per member, FP32-stored contributions
+PR_TOL_FP32 = 2e-7  # L1 per member, FP32-stored contributions (about 10x the measured error)
 
 
 def _members(seed, n, weights=None, symmetrize=False):

@@ -3,7 +3,7 @@
 Bars: BFS depths, CC labels and SSSP distances bit-exact; BFS parents form a
 valid shortest-path tree and equal the smallest in-neighbour one level up (the
 HIP path's definition, identical for every schedule); PageRank within 1e-10 L1
-of the FP64 oracle (FP32-stored contributions: within 1e-5, error reported).
+of the FP64 oracle (FP32-stored contributions: within 2e-7 L1, error reported).
 """
 import numpy as np
 import pytest
@@ -16,7 +16,7 @@ from helpers import U32_MAX, check_bfs_tree, min_frontier_parent
 pytestmark = pytest.mark.gpu
 
 PR_TOL = 1e-10  # L1, FP64 contributions
-PR_TOL_FP32 = 1e-5  # L1, FP32-stored contributions (expected ~1e-8..1e-7)
+PR_TOL_FP32 = 2e-7  # L1, FP32-stored contributions (about 10x the measured error)
 
 
 @pytest.fixture(scope="module", params=[True, False], ids=["relabel", "native-order"])
