@@ -302,6 +302,17 @@ int wg_validate(wg_ctx* ctx, uint64_t* violations);
  * its roots (BFS / SSSP). */
 int wg_validate_values(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t nroots, const void* values,
                        uint64_t* violations);
+/* wg_validate without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Its bitmaps and
+ * counter live in a fixed device array of 2 * WG_NOALLOC_MAX_VERTICES / 8 +
+ * 256 bytes (32 MiB) that each GPU the library runs on holds until the
+ * process ends.  Graphs with more than WG_NOALLOC_MAX_VERTICES vertices are
+ * rejected with WG_E_RANGE.  Every call on one GPU shares that GPU's array,
+ * and each call clears what it uses first: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller. */
+#define WG_NOALLOC_MAX_VERTICES (1ull << 27)
+int wg_validate_noalloc(wg_ctx* ctx, uint64_t* violations);
 
 /* ---- batches of independent graphs (no reference counterpart) ------------
  * A batch of small graphs is loaded as one graph, their disjoint union:

@@ -40,6 +40,7 @@ WEIGHT_NONE, WEIGHT_F64, WEIGHT_U32 = 0, 1, 2
 APP_BFS, APP_CC, APP_PR, APP_SSSP = 1, 2, 3, 4
 APP_MSBFS, APP_MSSSSP, APP_PPR = 5, 6, 7
 MS_MAX_SOURCES = 64
+NOALLOC_MAX_VERTICES = 1 << 27  # WG_NOALLOC_MAX_VERTICES
 MODE_DENSE, MODE_FRONTIER = 0, 1
 SCHED_PLAIN, SCHED_LEARNED, SCHED_DENSE, SCHED_FRONTIER = 0, 1, 2, 3
 DEFAULT_BLOCK_LOG2 = 4
@@ -130,6 +131,7 @@ GPU_SYMBOLS = [
     ("wg_set_members", C.c_int, [_p, C.c_uint32, _u64p]),
     ("wg_validate", C.c_int, [_p, _u64p]),
     ("wg_validate_values", C.c_int, [_p, C.c_int, _u64p, C.c_uint32, _p, _u64p]),
+    ("wg_validate_noalloc", C.c_int, [_p, _u64p]),
     ("wg_batch_shape_at", C.c_int, [C.c_uint32, _p]),
     ("wg_batch_pr", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, C.c_int]),
     ("wg_batch_autotune", C.c_int, [_p, C.c_uint32, C.c_double, C.c_int, _p, _f32p]),

@@ -16,6 +16,7 @@
 // A violation is one failing edge or vertex; the count is returned.
 #include "wg_frontier.h"
 
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -182,25 +183,18 @@ __global__ void k_chk_cc_vertices(const uint32_t* __restrict__ label, uint64_t V
 
 // ---- host side ------------------------------------------------------------------
 
-// Runs the checks on device-order values (vals) with device-id roots.
-static int run_checks(wg_ctx* ctx, int app, const void* vals, const uint32_t* droots, uint32_t nroots,
-                      uint64_t* violations)
+// Runs the checks on device-order values (vals) with device-id roots, in the
+// buffers ok and rootbm (bm_words words each) and the counter bad, which it
+// clears first.
+static int run_checks_in(wg_ctx* ctx, int app, const void* vals, const uint32_t* droots, uint32_t nroots,
+                         uint64_t* ok, uint64_t* rootbm, unsigned long long* bad, uint64_t* violations)
 {
     const uint64_t V = ctx->V, E = ctx->E, words = ctx->bm_words;
-    uint64_t* ok = nullptr;
-    uint64_t* rootbm = nullptr;
-    unsigned long long* bad = nullptr;
     int rc = WG_OK;
-    if (hipMalloc((void**)&ok, words * 8) != hipSuccess || hipMalloc((void**)&rootbm, words * 8) != hipSuccess ||
-        hipMalloc((void**)&bad, sizeof(unsigned long long)) != hipSuccess) {
-        (void)hipFree(ok);
-        (void)hipFree(rootbm);
-        return WG_E_NOMEM;
-    }
     unsigned long long hbad = 0;
     if ((rc = wg_clear_u64(ctx, ok, words)) || (rc = wg_clear_u64(ctx, rootbm, words)) ||
         (rc = wg_clear_u64(ctx, (uint64_t*)bad, 1)))
-        goto out;
+        return rc;
     if (nroots)
         hipLaunchKernelGGL(k_chk_roots, dim3((nroots + 255) / 256), dim3(256), 0, ctx->stream, droots, nroots, rootbm);
     {
@@ -236,15 +230,32 @@ static int run_checks(wg_ctx* ctx, int app, const void* vals, const uint32_t* dr
         }
     }
     if ((rc = wg_check_launch(ctx)))
-        goto out;
+        return rc;
     if (hipMemcpyAsync(&hbad, bad, sizeof(hbad), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
         ctx->last_hip_error = (int)hipGetLastError();
-        rc = WG_E_HIP;
-        goto out;
+        return WG_E_HIP;
     }
     *violations = hbad;
-out:
+    return WG_OK;
+}
+
+// run_checks_in with buffers allocated for the call.
+static int run_checks(wg_ctx* ctx, int app, const void* vals, const uint32_t* droots, uint32_t nroots,
+                      uint64_t* violations)
+{
+    const uint64_t words = ctx->bm_words;
+    uint64_t* ok = nullptr;
+    uint64_t* rootbm = nullptr;
+    unsigned long long* bad = nullptr;
+    if (hipMalloc((void**)&ok, words * 8) != hipSuccess || hipMalloc((void**)&rootbm, words * 8) != hipSuccess ||
+        hipMalloc((void**)&bad, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(ok);
+        (void)hipFree(rootbm);
+        return WG_E_NOMEM;
+    }
+    const int rc = run_checks_in(ctx, app, vals, droots, nroots, ok, rootbm, bad, violations);
+    (void)hipStreamSynchronize(ctx->stream); // the kernels are done with the buffers before they are freed
     (void)hipFree(ok);
     (void)hipFree(rootbm);
     (void)hipFree(bad);
@@ -259,6 +270,61 @@ extern "C" int wg_validate(wg_ctx* ctx, uint64_t* violations)
         return WG_E_STATE;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     return run_checks(ctx, ctx->app, ctx->prop[0], ctx->roots, ctx->app == WG_APP_CC ? 0u : ctx->nroots, violations);
+}
+
+// ---- allocation-free validation -----------------------------------------------------
+//
+// wg_validate_noalloc keeps its two bitmaps and its counter in `scratch`, a
+// device array of fixed size that every GPU holds from the library's load until
+// the process ends.  Host code never passes the array's name to a kernel: the
+// name is a host-side stand-in, and each device has its own copy at its own
+// address, which hipGetSymbolAddress returns with that device current.
+#define WG_NOALLOC_WORDS (WG_NOALLOC_MAX_VERTICES / 64)
+#define WG_NOALLOC_SCRATCH_WORDS (2 * WG_NOALLOC_WORDS + 32) // ok, roots, counter line
+static __device__ __align__(256) uint64_t scratch[WG_NOALLOC_SCRATCH_WORDS];
+
+#define WG_NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static uint64_t* g_scratch_ptr[WG_NOALLOC_MAX_DEVICES];
+
+// Device address of this device's scratch, looked up once per device (the
+// device must be current).
+static int scratch_on_device(wg_ctx* ctx, uint64_t** out)
+{
+    if (ctx->device < 0 || ctx->device >= WG_NOALLOC_MAX_DEVICES)
+        return WG_E_RANGE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->last_hip_error = (int)e;
+            return WG_E_HIP;
+        }
+        g_scratch_ptr[ctx->device] = (uint64_t*)p;
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return WG_OK;
+}
+
+extern "C" int wg_validate_noalloc(wg_ctx* ctx, uint64_t* violations)
+{
+    if (!ctx || !violations)
+        return WG_E_INVAL;
+    if (ctx->app != WG_APP_BFS && ctx->app != WG_APP_CC && ctx->app != WG_APP_SSSP)
+        return WG_E_STATE;
+    if (ctx->V > WG_NOALLOC_MAX_VERTICES || ctx->bm_words > WG_NOALLOC_WORDS)
+        return WG_E_RANGE;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    uint64_t* base = nullptr;
+    int rc = scratch_on_device(ctx, &base);
+    if (rc)
+        return rc;
+    uint64_t* ok = base;
+    uint64_t* rootbm = base + WG_NOALLOC_WORDS;
+    unsigned long long* bad = (unsigned long long*)(base + 2 * WG_NOALLOC_WORDS);
+    return run_checks_in(ctx, ctx->app, ctx->prop[0], ctx->roots, ctx->app == WG_APP_CC ? 0u : ctx->nroots, ok, rootbm,
+                         bad, violations);
 }
 
 extern "C" int wg_validate_values(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t nroots, const void* values,
