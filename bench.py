@@ -1,36 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X pull engine.
 
-Default workload: R-MAT scale 24, edge factor 16, seed 2 (a/b/c/d =
-.57/.19/.19/.05, self-loops and duplicates removed, grouped by destination),
-FP64 edge weights attached for SSSP.  One step = one PageRank iteration
-(edge pull + vertex phase, FP64 contributions, damping 0.85) over the whole
-graph with the inputs resident in HBM; `value` is E / (time per iteration) in
-GTEPS, and `roofline` prices the PageRank kernels' algorithmic bytes
-(4E + 24V per iteration) per launch, timed with HIP events on the engine's
-stream, against the 8 TB/s HBM peak.
+Headline (one step = one PageRank iteration, edge pull + vertex phase, FP64
+contributions, damping 0.85, over the whole graph with the inputs resident in
+HBM): R-MAT scale 27, edge factor 16, seed 5 (a/b/c/d = .57/.19/.19/.05,
+self-loops and duplicates removed, grouped by destination; 134M vertices,
+2.1 billion edges).  `value` is E / (time per iteration) in GTEPS;
+`roofline` prices the iteration's algorithmic bytes (4E + 24V + 4: sources,
+offsets, out-degrees, contributions read once and written once) per launch of
+its kernels, timed with HIP events on the engine's stream, against the 8 TB/s
+HBM peak; `cpu_baseline` is the oracle (a C restatement of Wedge's pull
+engine) on the same graph, all host cores, a bounded number of iterations.
 
-The same run reports:
-  apps      BFS and SSSP from the max-out-degree root under the plain
-            threshold rule and under the learned schedule (the packaged
-            model), on this graph and on a 4096x6144 road grid (integer
-            weights, seed 3), and CC on the symmetrised graph: GTEPS,
-            learned/plain speedup, identical outputs, and the violations the
-            on-device certificate (wg_validate) finds in each result (0 =
-            a valid BFS tree / shortest-path fixpoint / minimum labels);
-  batched   64-source BFS and SSSP and 32-seed personalised PageRank on this
-            graph against one source at a time, and 256 uneven R-MAT graphs
-            (scale U{10..16}, edge factor U{8..32}, seed 4) as one batch
-            (PageRank in one captured HIP graph with the autotuned tile shape,
-            BFS from every member in one traversal) against graph by graph;
-  large     R-MAT scale 27, edge factor 16, seed 5 (2.1 billion edges):
-            PageRank with FP64 and FP32-stored contributions and BFS (plain
-            and learned), each with its HBM roofline fraction, and the L1
-            distance of the FP32 ranks from the FP64 ranks;
-  cpu_baseline  the oracle (a C restatement of Wedge's pull engine) timed on
-            a bounded PageRank sample on the host cores, and its BFS from the
-            same root; its ranks also give the L1 error of the HIP path's FP64
-            and FP32 ranks.
+The same run reports, each with the oracle's time on the host beside it:
+  large        the scale-27 graph: PageRank with FP32-stored contributions
+               (error against the FP64 ranks), BFS from the max-out-degree
+               vertex under the plain rule and the learned schedule with the
+               algorithmic bytes counted by the kernels;
+  pagerank_s20 R-MAT scale 20, edge factor 16, seed 1: 16 iterations at
+               d = 0.85 and at the reference's 0.875;
+  apps         R-MAT scale 24, seed 2: BFS and FP64 SSSP (plain and learned),
+               CC on the symmetrised graph; the 4096 x 6144 road grid (seed 3):
+               BFS and integer SSSP, plain and learned;
+  batched      64-source BFS and SSSP and 32-seed personalised PageRank on the
+               scale-24 graph, and 256 uneven R-MAT graphs as one batch.
 
 Prints one JSON line.
 """
@@ -52,16 +45,14 @@ import wedge_amd as W  # noqa: E402
 METRIC = "GTEPS (PR iter, BFS) + HBM roofline %, 1 MI355X; learned vs plain schedule"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
-CONFIGS = {
-    "s24": dict(scale=24, edge_factor=16, seed=2),
-    "s20": dict(scale=20, edge_factor=16, seed=1),
-    "s22": dict(scale=22, edge_factor=16, seed=2),
-    "s27": dict(scale=27, edge_factor=16, seed=5),
-}
-
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads_used() -> int:
+    """Host threads the oracle may use: the CPUs this process may run on."""
+    return max(1, len(os.sched_getaffinity(0)))
 
 
 def pr_algorithmic_bytes(V: int, E: int, fp32: bool) -> int:
@@ -71,11 +62,25 @@ def pr_algorithmic_bytes(V: int, E: int, fp32: bool) -> int:
     return 4 * E + 4 * (V + 1) + 4 * V + 2 * c * V
 
 
-def measured_traffic(config: str, fp32: bool):
-    """HBM bytes per PageRank iteration from the committed rocprofv3 counter
-    summary of the same workload (FETCH_SIZE and WRITE_SIZE passes, scaled by
-    a copy kernel of known size), or None when no summary covers it."""
-    name = f"k_pr_tile_rmat_{config}_{'fp32_' if fp32 else ''}hbm_traffic.md"
+def traversal_bytes(app: str, V: int, st: dict, work: tuple[int, int], wbytes: int = 0) -> int:
+    """Bytes a traversal must move at minimum, from the work the kernels
+    counted: per edge read, its 4-byte source (plus its weight for SSSP); per
+    destination state examined, that state (BFS depth/parent word 8 B, SSSP
+    distance 8 B FP64 or 4 B integer, CC label 4 B); per iteration one read of
+    the current frontier bitmap and one write of the next (2 * V/8); BFS also
+    writes 8 B per newly reached vertex."""
+    edges, dests = work
+    state = {"bfs": 8, "cc": 4}.get(app, 8 if wbytes == 8 else 4)
+    b = (4 + wbytes) * edges + state * dests + st["iterations"] * 2 * ((V + 7) // 8)
+    if app == "bfs":
+        b += 8 * st["vertices_reached"]
+    return b
+
+
+def measured_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of known size),
+    or None when no summary covers the workload."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -85,43 +90,98 @@ def measured_traffic(config: str, fp32: bool):
     return None
 
 
+def shape_name(shape) -> str:
+    return "binned" if shape[2] == W.PR_BINNED else ("tiles-xcd" if shape[2] else "tiles")
+
+
+def pr_timed(ctx, damping, fp32, steps, warmup):
+    """PageRank with the autotuned shape: wall ms per iteration over `steps`
+    iterations and the iteration's kernel ms per launch (HIP events on the
+    engine's stream)."""
+    ctx.init_pr(damping, fp32)
+    shape, times = ctx.pr_autotune(2)
+    for _ in range(warmup):
+        ctx.pr_iteration()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx._lib.wg_pr_iteration(ctx.handle, None)
+    ctx.sync()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    kern = {}
+    for k in ("pr_tile", "pr_span", "pb_push", "pb_pull"):
+        n, ms = ctx.profile_get(k)
+        if n:
+            kern[k] = ms / n
+    ctx.profile(False)
+    return wall, kern, shape, times
+
+
+def roofline_entry(alg: int, kern: dict, traffic):
+    ms = sum(kern.values())
+    achieved = alg / (ms * 1e-3) / 1e9
+    names = {"pr_tile": "k_pr_tile", "pr_span": "k_pr_span", "pb_push": "k_pb_push", "pb_pull": "k_pb_pull"}
+    return {
+        "bound": "hbm",
+        "kernel": " + ".join(names[k] for k in kern) + " (one PageRank iteration)",
+        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "algorithmic_bytes_per_launch": alg, "avg_launch_ms": round(ms, 5),
+        "kernel_ms": {names[k]: round(v, 5) for k, v in kern.items()},
+    }
+
+
 def run_best(ctx, app, root, reps, **kw):
     best = None
     for _ in range(reps):
         res, st = ctx.run_bfs(root, **kw) if app == "bfs" else ctx.run_sssp(root, **kw)
         if best is None or st["running_ms"] < best[1]["running_ms"]:
-            best = (res, st)
+            best = (res, st, ctx.work_counts())
     return best
 
 
-def schedule_pair(ctx, g, root, reps, weights):
+def traversal_entry(app, V, st, work, wbytes=0, violations=None):
+    b = traversal_bytes(app, V, st, work, wbytes)
+    e = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
+         "frontier_iterations": st["frontier_iterations"],
+         "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
+         "edges_read": work[0], "dests_examined": work[1], "algorithmic_bytes": b,
+         "roofline_frac": round(b / (st["running_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if violations is not None:
+        e["violations"] = violations
+    return e
+
+
+def schedule_pair(ctx, g, root, reps, wbytes, orc=None):
     """BFS and SSSP under the plain rule and the learned schedule: GTEPS
-    (edges whose source was reached / run time), iterations, and whether the
-    outputs are identical."""
+    (edges whose source was reached / run time), iterations, algorithmic bytes
+    and roofline fraction from the kernels' work counts, on-device certificate
+    violations, identical outputs, and the oracle's time on the host."""
     out = {"vertices": g.num_vertices, "edges": g.num_edges, "root": root}
     for app in ("bfs", "sssp"):
         res = {}
         for sched in ("plain", "learned"):
-            r, st = run_best(ctx, app, root, reps, schedule=sched)
+            r, st, work = run_best(ctx, app, root, reps, schedule=sched)
             res[sched] = (r, st)
-            out[f"{app}_{sched}"] = {
-                "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
-                "ms": round(st["running_ms"], 3), "iterations": st["iterations"],
-                "frontier_iterations": st["frontier_iterations"], "fused_levels": ctx.fused_level_count(),
-                # on-device certificate of the result (BFS tree / tight shortest-path edges)
-                "violations": ctx.validate(),
-            }
-        if app == "bfs":  # the same runs with one launch sequence per level (fused levels off)
-            ctx.set_fused_levels(0)
-            for sched in ("plain", "learned"):
-                r, st = run_best(ctx, app, root, reps, schedule=sched)
-                out[f"bfs_{sched}"]["ms_level_per_launch"] = round(st["running_ms"], 3)
-                assert np.array_equal(r, res[sched][0])
-            ctx.set_fused_levels(32)
+            out[f"{app}_{sched}"] = traversal_entry(app, g.num_vertices, st, work, 0 if app == "bfs" else wbytes,
+                                                    ctx.validate())
+            out[f"{app}_{sched}"]["fused_levels"] = ctx.fused_level_count()
         out[f"{app}_learned_vs_plain"] = round(res["plain"][1]["running_ms"] / res["learned"][1]["running_ms"], 3)
         out[f"{app}_identical"] = bool(np.array_equal(res["plain"][0], res["learned"][0]))
-        log(f"{app} ({weights} weights for SSSP): plain {out[f'{app}_plain']}, learned {out[f'{app}_learned']}, "
-            f"x{out[f'{app}_learned_vs_plain']}, identical {out[f'{app}_identical']}")
+        if orc is not None:
+            t0 = time.time()
+            if app == "bfs":
+                _, _, ost = orc.bfs(root)
+            else:
+                _, ost = orc.sssp(root)
+            out[f"{app}_cpu_oracle"] = {"ms": round(ost["running_ms"], 2), "iterations": int(ost["iterations"]),
+                                        "threads": orc.threads, "wall_s": round(time.time() - t0, 2)}
+            out[f"{app}_gpu_vs_cpu"] = round(ost["running_ms"] / res["learned"][1]["running_ms"], 1)
+        log(f"{app}: plain {out[f'{app}_plain']}, learned {out[f'{app}_learned']}, "
+            f"x{out[f'{app}_learned_vs_plain']}, identical {out[f'{app}_identical']}, "
+            f"cpu {out.get(f'{app}_cpu_oracle')}")
     return out
 
 
@@ -231,84 +291,6 @@ def batched_graphs(damping, iterations=16, reps=5):
     return out
 
 
-def bfs_algorithmic_bytes(V: int, st: dict) -> int:
-    """Bytes a BFS must move at minimum, Graph500-style: the source index of
-    every edge leaving a reached vertex (4 B), the depth/parent word of every
-    reached vertex (8 B), and per level one read of the current frontier
-    bitmap and one write of the next (2 * V/8 B)."""
-    return 4 * st["edges_reached"] + 8 * st["vertices_reached"] + st["iterations"] * 2 * ((V + 7) // 8)
-
-
-def pr_timed(ctx, damping, fp32, steps, warmup=2):
-    """Wall ms per fused PageRank iteration and the tile + span kernel ms per
-    launch (HIP events on the engine's stream), with the autotuned tile shape."""
-    ctx.init_pr(damping, fp32)
-    ctx.pr_autotune(2)
-    for _ in range(warmup):
-        ctx.pr_iteration()
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx._lib.wg_pr_iteration(ctx.handle, None)
-    ctx.sync()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
-    n_tile, ms_tile = ctx.profile_get("pr_tile")
-    n_span, ms_span = ctx.profile_get("pr_span")
-    ctx.profile(False)
-    return wall, ms_tile / max(n_tile, 1) + ms_span / max(n_span, 1)
-
-
-def large_graph(damping, steps=5, iterations=16, reps=3):
-    """R-MAT scale 27, edge factor 16, seed 5 (2.1 billion edges resident in
-    HBM): PageRank with FP64 and with FP32-stored contributions (time per
-    iteration, roofline of the tile + span kernels, L1 distance of the FP32
-    ranks from the FP64 ranks after 16 iterations), and BFS from the
-    max-out-degree vertex under the plain and the learned schedule."""
-    t0 = time.time()
-    g = W.rmat(27, 16, seed=5)
-    V, E = g.num_vertices, g.num_edges
-    root = g.max_outdegree_vertex()
-    log(f"rmat s27 ef16 seed 5: V={V} E={E} ({time.time() - t0:.1f} s)")
-    ctx = W.Context(0)
-    ctx.load(g)
-    out = {"workload": "rmat-s27-ef16-seed5", "vertices": V, "edges": E, "root": root}
-    ranks = {}
-    for fp32 in (False, True):
-        wall, kern = pr_timed(ctx, damping, fp32, steps)
-        alg = pr_algorithmic_bytes(V, E, fp32)
-        key = "pr_fp32" if fp32 else "pr_fp64"
-        out[key] = {"ms_per_iteration": round(wall, 3), "gteps": round(E / (wall * 1e-3) / 1e9, 2),
-                    "tile_shape": list(ctx.pr_shape()),
-                    "kernel_ms": round(kern, 3), "algorithmic_bytes": alg,
-                    "roofline_frac": round(alg / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        ctx.init_pr(damping, fp32)
-        for _ in range(iterations):
-            ctx._lib.wg_pr_iteration(ctx.handle, None)
-        ranks[fp32] = ctx.read_ranks()
-        log(f"s27 {key}: {out[key]}")
-    out["pr_fp32_l1_vs_fp64"] = float(np.abs(ranks[True] - ranks[False]).sum())
-    out["pr_iterations_compared"] = iterations
-    del ranks
-    W.mlp_load()
-    res = {}
-    for sched in ("plain", "learned"):
-        r, st = run_best(ctx, "bfs", root, reps, schedule=sched)
-        res[sched] = r
-        alg = bfs_algorithmic_bytes(V, st)
-        out[f"bfs_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
-                               "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 2),
-                               "algorithmic_bytes": alg,
-                               "roofline_frac": round(alg / (st["running_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                               "violations": ctx.validate()}
-        log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
-    out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
-    out["bfs_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
-    ctx.close()
-    return out
-
-
 def host_cpu_info():
     """CPU model and logical / physical core counts of the host."""
     model, phys = "unknown", set()
@@ -331,10 +313,19 @@ def host_cpu_info():
             "usable": len(os.sched_getaffinity(0))}
 
 
-def cc_pair(scale, edge_factor, seed, reps=2):
+def oracle_of(g):
+    import oracle  # test infrastructure: the checker and the CPU baseline only
+
+    t0 = time.time()
+    o = oracle.Oracle.of(g, threads=host_threads_used())
+    return o, time.time() - t0
+
+
+def cc_pair(scale, edge_factor, seed, reps=2, cpu=True):
     """CC on the symmetrised R-MAT (the reference's CC input) under the plain
     rule and the learned schedule: run time, iterations, reference-style
-    E * iterations / time, identical labels."""
+    E * iterations / time, algorithmic bytes, identical labels, and the
+    oracle's time."""
     t0 = time.time()
     g = W.rmat(scale, edge_factor, seed=seed, symmetrize=True)
     log(f"rmat s{scale} ef{edge_factor} seed {seed} symmetrised: V={g.num_vertices} E={g.num_edges} "
@@ -348,62 +339,135 @@ def cc_pair(scale, edge_factor, seed, reps=2):
         for _ in range(reps):
             lab, st = ctx.run_cc(schedule=sched)
             if best is None or st["running_ms"] < best[1]["running_ms"]:
-                best = (lab, st)
+                best = (lab, st, ctx.work_counts())
         res[sched] = best[0]
         st = best[1]
-        out[f"cc_{sched}"] = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
-                              "frontier_iterations": st["frontier_iterations"],
-                              "bedges_per_s": round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2),
-                              "violations": ctx.validate()}  # on-device certificate of the labels
+        e = traversal_entry("cc", g.num_vertices, st, best[2], violations=ctx.validate())
+        e["bedges_per_s"] = round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2)
+        out[f"cc_{sched}"] = e
     out["cc_learned_vs_plain"] = round(out["cc_plain"]["ms"] / out["cc_learned"]["ms"], 3)
     out["cc_identical"] = bool(np.array_equal(res["plain"], res["learned"]))
     ctx.close()
+    if cpu:
+        o, build_s = oracle_of(g)
+        _, ost = o.cc()
+        out["cc_cpu_oracle"] = {"ms": round(ost["running_ms"], 2), "iterations": int(ost["iterations"]),
+                                "threads": o.threads, "layout_s": round(build_s, 1)}
+        out["cc_gpu_vs_cpu"] = round(ost["running_ms"] / out["cc_plain"]["ms"], 1)
+        o.close()
     log(f"cc: {out}")
     return out
 
 
-def cpu_baseline(g, damping, budget_s=20.0, root=None):
-    """The oracle's PageRank on the same graph, all host threads, a bounded
-    number of iterations (time per iteration -> GTEPS)."""
-    import oracle
-
+def large_graph(args):
+    """The headline: R-MAT scale 27 resident in HBM.  PageRank FP64 (timed
+    steps, roofline), FP32-stored contributions, BFS plain and learned with
+    kernel-counted bytes, and the oracle's PageRank and BFS on the host."""
     t0 = time.time()
-    orc = oracle.Oracle.of(g, threads=0)
-    build_s = time.time() - t0
-    # one iteration to size the sample, then as many as fit the budget (max 8)
-    _, st = orc.pr(1, damping)
-    per_iter = max(st["running_ms"], 1e-3) / 1e3
-    iters = int(max(1, min(8, (budget_s - build_s) / per_iter)))
-    ref, st = orc.pr(iters, damping)
-    ms = st["running_ms"] / iters
-    threads = orc.threads
-    bfs = None
-    if root is not None:  # the oracle's BFS from the bench's root, same threads
-        _, _, bst = orc.bfs(root)
-        bfs = {"ms": round(bst["running_ms"], 2), "iterations": int(bst["iterations"])}
-    orc.close()
-    # the oracle's ranks also check the HIP path's ranks after the same iterations
-    errs = {}
+    g = W.rmat(27, 16, seed=5)
+    V, E = g.num_vertices, g.num_edges
+    root = g.max_outdegree_vertex()
+    log(f"rmat s27 ef16 seed 5: V={V} E={E} ({time.time() - t0:.1f} s, {W.host_threads()} host threads)")
     ctx = W.Context(0)
+    t0 = time.time()
     ctx.load(g)
+    log(f"uploaded + device layout built in {time.time() - t0:.1f} s")
+    out = {"workload": "rmat-s27-ef16-seed5", "vertices": V, "edges": E, "root": root}
+    res = {}
     for fp32 in (False, True):
-        ctx.init_pr(damping, fp32)
-        for _ in range(iters):
-            ctx._lib.wg_pr_iteration(ctx.handle, None)
-        errs["fp32" if fp32 else "fp64"] = float(np.abs(ctx.read_ranks() - ref).sum())
+        wall, kern, shape, times = pr_timed(ctx, args.damping, fp32, args.steps if not fp32 else 5, args.warmup)
+        alg = pr_algorithmic_bytes(V, E, fp32)
+        traffic = measured_traffic(f"pr_rmat_s27_{'fp32_' if fp32 else ''}hbm_traffic.md")
+        key = "pr_fp32" if fp32 else "pr_fp64"
+        res[key] = {"ms_per_iteration": round(wall, 4), "gteps": round(E / (wall * 1e-3) / 1e9, 3),
+                    "shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1], "kind": shape_name(shape)},
+                    "roofline": roofline_entry(alg, kern, traffic),
+                    "autotune_ms": {f"{s[0]}x{s[1]}x{s[2]}": round(v, 3) for s, v in times.items()}}
+        if shape[2] == W.PR_BINNED:
+            ctx.init_pr(args.damping, fp32)
+            res[key]["binned_layout"] = ctx.pr_bins_info()
+        log(f"s27 {key}: {res[key]}")
+    out.update(res)
+    ranks = {}
+    for fp32 in (False, True):
+        ranks[fp32], _ = ctx.run_pr(iterations=16, damping=args.damping, contrib_fp32=fp32)
+    out["pr_fp32_l1_vs_fp64"] = float(np.abs(ranks[True] - ranks[False]).sum())
+    out["pr_iterations_compared"] = 16
+    del ranks
+    W.mlp_load()
+    bfs = {}
+    for sched in ("plain", "learned"):
+        r, st, work = run_best(ctx, "bfs", root, 3, schedule=sched)
+        bfs[sched] = r
+        out[f"bfs_{sched}"] = traversal_entry("bfs", V, st, work, violations=ctx.validate())
+        log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
+    out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
+    out["bfs_identical"] = bool(np.array_equal(bfs["plain"], bfs["learned"]))
+    del bfs
     ctx.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(g, args.damping, root, args.cpu_budget)
+        out["bfs_gpu_vs_cpu"] = round(cpu["bfs"]["ms"] / out["bfs_learned"]["ms"], 1)
+    return g, out, cpu
+
+
+def cpu_baseline(g, damping, root, budget_s):
+    """The oracle's PageRank on the same graph with every host CPU this
+    process may use, as many iterations as fit the budget (at least 1, at
+    most 8): time per iteration -> GTEPS; and the oracle's BFS from the same
+    root.  The oracle's layout build is not timed (the reference times only
+    the iterations, source/main.c:153-160)."""
+    o, build_s = oracle_of(g)
+    _, st = o.pr(1, damping)
+    per_iter = max(st["running_ms"], 1e-3) / 1e3
+    iters = int(max(1, min(8, budget_s / per_iter)))
+    _, st = o.pr(iters, damping)
+    ms = st["running_ms"] / iters
+    _, _, bst = o.bfs(root)
+    threads = o.threads
+    o.close()
     return {
         "value": round(g.num_edges / (ms * 1e-3) / 1e9, 4),
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle PageRank, {iters} iteration(s) on the same graph, {threads} threads, "
-                  f"{ms:.1f} ms/iteration (layout build {build_s:.1f} s not timed)",
+        "sample": f"oracle PageRank on the same graph: {iters} iteration(s) on {threads} threads, "
+                  f"{ms:.1f} ms per iteration (layout build {build_s:.1f} s not timed)",
         "host": host_cpu_info(),
-        "bfs": bfs,
-        "pr_l1_hip_vs_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
-        "pr_iterations_compared": iters,
+        "bfs": {"ms": round(bst["running_ms"], 2), "iterations": int(bst["iterations"])},
     }
+
+
+def pagerank_s20(damping_values=(0.85, 0.875), iterations=16, cpu=True):
+    """R-MAT scale 20, edge factor 16, seed 1: 16 PageRank iterations per
+    damping, GPU (autotuned shape) against the oracle, with the L1 error."""
+    g = W.rmat(20, 16, seed=1)
+    out = {"workload": "rmat-s20-ef16-seed1", "vertices": g.num_vertices, "edges": g.num_edges,
+           "iterations": iterations}
+    o = oracle_of(g)[0] if cpu else None
+    with W.Context(0) as ctx:
+        ctx.load(g)
+        for d in damping_values:
+            ctx.init_pr(d)
+            shape, _ = ctx.pr_autotune(2)
+            ms, _ = best_ms(lambda: ([ctx._lib.wg_pr_iteration(ctx.handle, None) for _ in range(iterations)],
+                                     ctx.sync()), 1)
+            ranks, _ = ctx.run_pr(iterations=iterations, damping=d)
+            e = {"gpu_ms": round(ms, 3), "gpu_ms_per_iteration": round(ms / iterations, 4),
+                 "gteps": round(iterations * g.num_edges / (ms * 1e-3) / 1e9, 2),
+                 "shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1], "kind": shape_name(shape)}}
+            if o is not None:
+                ref, st = o.pr(iterations, d)
+                e.update({"cpu_ms": round(st["running_ms"], 2), "cpu_threads": o.threads,
+                          "cpu_gteps": round(iterations * g.num_edges / (st["running_ms"] * 1e-3) / 1e9, 3),
+                          "gpu_vs_cpu": round(st["running_ms"] / ms, 1),
+                          "l1_vs_oracle": float(f"{float(np.abs(ranks - ref).sum()):.3e}")})
+            out[f"d{d}"] = e
+            log(f"PR s20 d={d}: {e}")
+    if o is not None:
+        o.close()
+    return out
 
 
 def main():
@@ -411,16 +475,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="s24", choices=sorted(CONFIGS))
     ap.add_argument("--damping", type=float, default=0.85)
-    ap.add_argument("--fp32", action="store_true", help="FP32-stored PageRank contributions")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-apps", action="store_true", help="skip BFS/SSSP")
-    ap.add_argument("--bfs-reps", type=int, default=3)
-    ap.add_argument("--no-grid", action="store_true", help="skip the 4096x6144 road grid (plain vs learned)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle PageRank on the headline graph")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle timings")
+    ap.add_argument("--no-apps", action="store_true", help="skip the scale-24 and road-grid traversals")
+    ap.add_argument("--no-grid", action="store_true", help="skip the 4096x6144 road grid")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sources and graphs")
-    ap.add_argument("--no-batch-graphs", action="store_true", help="skip the 256-graph batch")
-    ap.add_argument("--no-s27", action="store_true", help="skip the 2.1-billion-edge R-MAT (scale 27)")
+    ap.add_argument("--no-s20", action="store_true", help="skip the scale-20 PageRank")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -428,111 +489,70 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    cfg = CONFIGS[args.config]
-    t0 = time.time()
-    g = W.rmat(cfg["scale"], cfg["edge_factor"], seed=cfg["seed"], weights=None if args.no_apps else "f64")
-    V, E = g.num_vertices, g.num_edges
-    log(f"graph rmat s{cfg['scale']} ef{cfg['edge_factor']} seed {cfg['seed']}: V={V} E={E} "
-        f"({time.time() - t0:.1f} s, {W.host_threads()} host threads)")
-
-    ctx = W.Context(0)
-    t0 = time.time()
-    ctx.load(g)
-    log(f"uploaded + device layout built in {time.time() - t0:.1f} s")
-
-    # ---- PageRank: the timed steps -------------------------------------------------
-    ctx.init_pr(args.damping, args.fp32)
-    shape, _ = ctx.pr_autotune(2)  # tile shape picked before the timed region; PageRank re-initialised
-    log(f"PR tile shape (edges/lane, WG/CU, XCD split): {shape}")
-    for _ in range(args.warmup):
-        ctx.pr_iteration()
-    ctx.sync()
-    ctx.profile(True)
-    ctx.profile_reset()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx._lib.wg_pr_iteration(ctx.handle, None)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    n_tile, ms_tile = ctx.profile_get("pr_tile")
-    n_span, ms_span = ctx.profile_get("pr_span")
-    ctx.profile(False)
-    ms_per_step = wall * 1e3 / args.steps
-    gteps = E / (ms_per_step * 1e-3) / 1e9
-    tile_ms = ms_tile / max(n_tile, 1)
-    span_ms = ms_span / max(n_span, 1)
-    alg = pr_algorithmic_bytes(V, E, args.fp32)
-    kern_ms = tile_ms + span_ms
-    achieved = alg / (kern_ms * 1e-3) / 1e9
-    log(f"PR: {ms_per_step:.3f} ms/iter wall, tile {tile_ms:.3f} ms + span {span_ms:.3f} ms, "
-        f"{gteps:.1f} GTEPS, {achieved:.0f} GB/s algorithmic")
-    ranks = ctx.read_ranks()
-    rank_sum = float(ranks.sum())
-
+    copy_gbs = None
+    with W.Context(0) as c:
+        copy_gbs = round(c.measure_copy(4 << 30), 1)
+    g27, large, cpu = large_graph(args)
+    del g27
+    pr = large["pr_fp64"]
     out = {
         "metric": METRIC,
-        "value": round(gteps, 3),
+        "value": pr["gteps"],
         "unit": "GTEPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": pr["ms_per_iteration"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-stored/f64-accum" if args.fp32 else "f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"PageRank iteration (FP{'32' if args.fp32 else '64'} contributions, d={args.damping}) "
-                        f"on rmat-s{cfg['scale']}-ef{cfg['edge_factor']}-seed{cfg['seed']}",
-            "scale": cfg["scale"], "edge_factor": cfg["edge_factor"], "seed": cfg["seed"],
-            "vertices": V, "edges": E, "parallelism": "single GPU",
-            "pr_tile_shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1], "xcd_split": shape[2]},
+            "workload": f"PageRank iteration (FP64 contributions, d={args.damping}) on rmat-s27-ef16-seed5",
+            "scale": 27, "edge_factor": 16, "seed": 5,
+            "vertices": large["vertices"], "edges": large["edges"], "parallelism": "single GPU",
+            "pr_shape": pr["shape"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_pr_tile + k_pr_span (one PageRank iteration)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.config, args.fp32),
-            "algorithmic_bytes_per_launch": alg,
-            "avg_launch_ms": round(kern_ms, 5),
-            "measured_copy_GBs": round(ctx.measure_copy(4 << 30), 1),
-        },
-        "pr_rank_sum": rank_sum,
+        "roofline": dict(pr["roofline"], measured_copy_GBs=copy_gbs),
     }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    out["large"] = large
+    if not args.no_s20:
+        out["pagerank_s20"] = pagerank_s20(cpu=not args.no_cpu)
 
-    # ---- BFS / SSSP on the same graph, plain and learned schedules --------------------
     if not args.no_apps:
         W.mlp_load()
+        g = W.rmat(24, 16, seed=2, weights="f64")
+        log(f"rmat s24 seed 2: V={g.num_vertices} E={g.num_edges}")
+        ctx = W.Context(0)
+        ctx.load(g)
+        orc = None if args.no_cpu else oracle_of(g)[0]
         root = g.max_outdegree_vertex()
-        out["apps"] = {"rmat": schedule_pair(ctx, g, root, args.bfs_reps, "f64")}
-        out["apps"]["rmat_symmetrised_cc"] = None  # filled after the road grid (own context)
+        out["apps"] = {"rmat_s24": schedule_pair(ctx, g, root, 3, 8, orc)}
+        if orc is not None:
+            orc.close()
         if not args.no_batch:
-            out["batched"] = batched_sources(ctx, g, ms_per_step, args.damping)
+            ctx.init_pr(args.damping)
+            ctx.pr_autotune(2)
+            ms, _ = best_ms(lambda: (ctx.pr_iteration(), ctx.sync()), 5)
+            out["batched"] = batched_sources(ctx, g, ms, args.damping)
+        ctx.close()
+        del g
+        out["apps"]["rmat_s24_symmetrised_cc"] = cc_pair(24, 16, 2, cpu=not args.no_cpu)
         if not args.no_grid:
-            t0 = time.time()
             road = W.grid(4096, 6144, seed=3)
+            log(f"road grid 4096x6144 seed 3: V={road.num_vertices} E={road.num_edges}")
+            ctx = W.Context(0)
             ctx.load(road)
-            log(f"road grid 4096x6144 seed 3: V={road.num_vertices} E={road.num_edges} ({time.time() - t0:.1f} s)")
-            out["apps"]["road_grid_4096x6144"] = schedule_pair(ctx, road, road.max_outdegree_vertex(), 2, "u32")
-    ctx.close()
-    if not args.no_apps:
-        if cfg["scale"] <= 24:  # the symmetrised scale-27 graph would pass 2^32 edges
-            out["apps"]["rmat_symmetrised_cc"] = cc_pair(cfg["scale"], cfg["edge_factor"], cfg["seed"])
-        else:
-            del out["apps"]["rmat_symmetrised_cc"]
-    if not args.no_batch and not args.no_batch_graphs:
+            orc = None if args.no_cpu else oracle_of(road)[0]
+            out["apps"]["road_grid_4096x6144"] = schedule_pair(ctx, road, road.max_outdegree_vertex(), 2, 4, orc)
+            if orc is not None:
+                orc.close()
+            ctx.close()
+    if not args.no_batch:
         out.setdefault("batched", {})["graphs_256"] = batched_graphs(args.damping)
-    if not args.no_s27:
-        out["large"] = large_graph(args.damping)
-
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, args.damping, root=None if args.no_apps else g.max_outdegree_vertex())
-        log(f"CPU baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
 

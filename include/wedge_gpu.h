@@ -76,7 +76,9 @@ typedef struct wg_ctx wg_ctx;
 #define WG_K_PR_ACC        10  /* stand-alone PageRank edge pull (accumulators) */
 #define WG_K_LOOP          11  /* whole device-resident iteration loop (wg_converge) */
 #define WG_K_BATCH_PR      12  /* whole batched PageRank run (wg_batch_pr) */
-#define WG_K_COUNT         13
+#define WG_K_PB_PUSH       13  /* binned PageRank: window partial sums */
+#define WG_K_PB_PULL       14  /* binned PageRank: bin accumulation + vertex phase */
+#define WG_K_COUNT         15
 
 /* ---- context -------------------------------------------------------------- */
 
@@ -99,6 +101,11 @@ int wg_sync(wg_ctx* ctx);
 #define WG_RELABEL_OFF 0
 #define WG_RELABEL_ON 1
 #define WG_RELABEL_AUTO 2
+#define WG_OPT_PR_BIN_SLICE 2  /* binned PageRank pull (WG_PR_BINNED): window-destination
+                                  sums one pull workgroup adds, 1..2^24 (default
+                                  262144); a bin with more is split over several
+                                  workgroups whose accumulators are then added.
+                                  Takes effect at the next layout build */
 int wg_set_option(wg_ctx* ctx, int option, int64_t value);
 
 /* Uploads a destination-grouped CSC and builds every derived device structure:
@@ -355,10 +362,11 @@ int wg_batch_shape_at(uint32_t i, wg_batch_shape* out);
  * waits for it. */
 int wg_batch_pr(wg_ctx* ctx, uint32_t iterations, double damping, int contrib_fp32, const wg_batch_shape* shape,
                 int use_graph);
-/* Times every candidate shape (one graph launch of `iterations` iterations,
+/* Times every candidate shape (plain launches of `iterations` iterations,
  * best of three after a warm-up), keeps the fastest for shape == NULL and
  * returns it; ms_per_shape (WG_BATCH_NSHAPES entries, optional) receives each
- * candidate's time. */
+ * candidate's time.  The next wg_batch_pr with use_graph = 1 captures the
+ * chosen shape. */
 int wg_batch_autotune(wg_ctx* ctx, uint32_t iterations, double damping, int contrib_fp32, wg_batch_shape* best,
                       float* ms_per_shape);
 /* Ranks of the last wg_batch_pr (V entries, FP64; each member sums to ~1). */
@@ -367,16 +375,34 @@ int wg_read_batch_ranks(wg_ctx* ctx, double* host_ranks);
 /* The tile shape of the single-graph PageRank (wg_pr_iteration,
  * wg_edge_pull_pr): the same candidates as the batch.  NULL restores the
  * default {8, 8, 0}.  The shape stays until the next graph load; results do
- * not depend on it beyond FP64 summation order (every shape sums in a fixed
- * order). */
+ * not depend on it beyond FP64 summation order (every tile shape sums in a
+ * fixed order).
+ * xcd_split == WG_PR_BINNED selects the binned pull instead of edge tiles
+ * (only with edges_per_lane 8): sources in windows whose contributions fill
+ * one workgroup's LDS, destinations in bins whose FP64 accumulators do, one
+ * streaming pass that sums each window's edges per destination and one that
+ * adds those sums per bin and runs the vertex phase.  blocks_per_cu 1: 128 KiB
+ * windows and bins (one 1024-thread workgroup per CU); 2: 64 KiB.  Its layout
+ * is built on the first iteration after a change of graph, shape or
+ * precision (about 2 bytes per edge plus 10 per window-destination pair, and
+ * one FP64 sum per pair while iterating).  Its per-bin sums use LDS atomics,
+ * so results are reproducible to rounding, not bit for bit. */
+#define WG_PR_BINNED 2
 int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape);
 int wg_get_pr_shape(const wg_ctx* ctx, wg_batch_shape* shape);
 /* The single-graph PageRank autotuner's candidate number i
  * (0 <= i < WG_PR_NSHAPES): the batch's WG_BATCH_NSHAPES shapes, then
  * {4|8|16, 1, 0} (one 16-wavefront workgroup per CU), then 128-edge tiles
- * {2, 1, 0}, {2, 4, 0}, {2, 16, 0} (2 edges per lane). */
-#define WG_PR_NSHAPES (WG_BATCH_NSHAPES + 6)
+ * {2, 1, 0}, {2, 4, 0}, {2, 16, 0} (2 edges per lane), then the binned pull
+ * {8, 1, WG_PR_BINNED}, {8, 2, WG_PR_BINNED}. */
+#define WG_PR_NSHAPES (WG_BATCH_NSHAPES + 8)
 int wg_pr_shape_at(uint32_t i, wg_batch_shape* out);
+/* Size of the binned pull's layout for the current graph, precision and
+ * shape (built if needed; WG_E_STATE unless PageRank is initialised with a
+ * binned shape): window-destination pairs, code entries (edges plus padding),
+ * push workgroups, pull workgroups. */
+int wg_pr_bins_info(wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* push_groups,
+                    uint32_t* pull_groups);
 /* After wg_init_pr: times `iterations` fused iterations per candidate (best of
  * two after a warm-up), keeps the fastest shape, and initialises PageRank
  * again with the same damping and precision.  ms_per_shape (optional,
@@ -395,6 +421,14 @@ int wg_read_sssp(wg_ctx* ctx, void* host_dist);
 /* Activity counters of the current state: vertices reached (BFS: visited;
  * SSSP: finite distance) and the sum of their out-degrees. */
 int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges);
+/* Work of the BFS / CC / SSSP pulls since the application's init (no
+ * reference counterpart; the measure of a traversal's minimum traffic):
+ * edges_read = edges whose source a pull read (an in-list entry, a marked
+ * block's edge, a pushed out-edge), dests_examined = destination states a
+ * pull examined (a destination group of a dense pull, a run of one
+ * destination's edges in a marked block, the destination of a pushed
+ * out-edge). */
+int wg_work_counts(wg_ctx* ctx, uint64_t* edges_read, uint64_t* dests_examined);
 
 /* ---- learned schedule --------------------------------------------------------
  * No reference counterpart: replaces the fixed threshold rule of

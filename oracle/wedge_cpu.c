@@ -1126,6 +1126,65 @@ int wcpu_sssp(wcpu* w, uint64_t root, uint32_t threshold_pct, void* dist, wcpu_s
 
 /* ---- personalised PageRank ------------------------------------------------------ */
 
+/* One seed, serially: the same operations in the same order whatever thread
+ * runs it, so results do not depend on the thread count. */
+static void ppr_seed(const wcpu* w, uint64_t s, uint32_t iterations, double damping, double* c, double* acc,
+                     double* out)
+{
+    const uint64_t V = w->V;
+    memset(c, 0, V * sizeof(double));
+    c[s] = w->outdeg[s] ? 1.0 / (double)w->outdeg[s] : 0.0;
+    double D = w->outdeg[s] ? 0.0 : 1.0;
+    for (uint32_t t = 0; t < iterations; ++t) {
+        memset(acc, 0, V * sizeof(double));
+        for (uint64_t k = 0; k < w->nvec; ++k) {
+            const uint64_t* l = vec_at(w, k);
+            const uint64_t dst = vec_dst(l);
+            for (int q = 0; q < 4; ++q)
+                if (l[q] & LANE_VALID)
+                    acc[dst] += c[l[q] & LANE_SRC];
+        }
+        double Dn = 0.0;
+        for (uint64_t v = 0; v < V; ++v) {
+            const double a = (v == s) ? acc[v] + D : acc[v];
+            const double r = (v == s) ? a * damping + (1.0 - damping) : a * damping;
+            c[v] = w->outdeg[v] ? r / (double)w->outdeg[v] : 0.0;
+            if (!w->outdeg[v])
+                Dn += r;
+            if (t + 1 == iterations)
+                out[v] = r;
+        }
+        D = Dn;
+    }
+}
+
+typedef struct {
+    const wcpu* w;
+    const uint64_t* seeds;
+    uint32_t nseeds, iterations, tid, nthreads;
+    double damping;
+    double* ranks;
+    int err;
+} ppr_arg;
+
+/* Seeds tid, tid + nthreads, ... with this thread's own buffers. */
+static void* ppr_worker(void* p)
+{
+    ppr_arg* a = (ppr_arg*)p;
+    const uint64_t V = a->w->V;
+    double* c = (double*)malloc(V * sizeof(double));
+    double* acc = (double*)malloc(V * sizeof(double));
+    if (!c || !acc) {
+        a->err = 1;
+    } else {
+        for (uint32_t i = a->tid; i < a->nseeds; i += a->nthreads)
+            ppr_seed(a->w, a->seeds[i], a->iterations, a->damping, c, acc, a->ranks + (uint64_t)i * V);
+    }
+    free(c);
+    free(acc);
+    return NULL;
+}
+
 int wcpu_ppr(wcpu* w, const uint64_t* seeds, uint32_t nseeds, uint32_t iterations, double damping, double* ranks)
 {
     if (!w || !seeds || !ranks || nseeds == 0 || iterations == 0)
@@ -1134,41 +1193,37 @@ int wcpu_ppr(wcpu* w, const uint64_t* seeds, uint32_t nseeds, uint32_t iteration
     for (uint32_t i = 0; i < nseeds; ++i)
         if (seeds[i] >= V)
             return -1;
-    double* c = (double*)calloc(V, sizeof(double));
-    double* acc = (double*)calloc(V, sizeof(double));
-    if (!c || !acc) {
-        free(c);
-        free(acc);
+    /* seeds are independent: one per thread at a time */
+    uint32_t T = (uint32_t)(w->nthreads > 0 ? w->nthreads : 1);
+    if (T > nseeds)
+        T = nseeds;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * T);
+    ppr_arg* args = (ppr_arg*)malloc(sizeof(ppr_arg) * T);
+    if (!th || !args) {
+        free(th);
+        free(args);
         return -1;
     }
-    for (uint32_t i = 0; i < nseeds; ++i) {
-        const uint64_t s = seeds[i];
-        memset(c, 0, V * sizeof(double));
-        c[s] = w->outdeg[s] ? 1.0 / (double)w->outdeg[s] : 0.0;
-        double D = w->outdeg[s] ? 0.0 : 1.0;
-        for (uint32_t t = 0; t < iterations; ++t) {
-            memset(acc, 0, V * sizeof(double));
-            for (uint64_t k = 0; k < w->nvec; ++k) {
-                const uint64_t* l = vec_at(w, k);
-                const uint64_t dst = vec_dst(l);
-                for (int q = 0; q < 4; ++q)
-                    if (l[q] & LANE_VALID)
-                        acc[dst] += c[l[q] & LANE_SRC];
-            }
-            double Dn = 0.0;
-            for (uint64_t v = 0; v < V; ++v) {
-                const double a = (v == s) ? acc[v] + D : acc[v];
-                const double r = (v == s) ? a * damping + (1.0 - damping) : a * damping;
-                c[v] = w->outdeg[v] ? r / (double)w->outdeg[v] : 0.0;
-                if (!w->outdeg[v])
-                    Dn += r;
-                if (t + 1 == iterations)
-                    ranks[(uint64_t)i * V + v] = r;
-            }
-            D = Dn;
-        }
+    uint32_t started = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        args[t] = (ppr_arg){ w, seeds, nseeds, iterations, t, T, damping, ranks, 0 };
+        if (t > 0 && pthread_create(&th[t], NULL, ppr_worker, &args[t]) != 0)
+            break;
+        started = t + 1;
     }
-    free(c);
-    free(acc);
-    return 0;
+    int err = 0;
+    if (started < T) { /* a thread could not start: run every seed here instead */
+        for (uint32_t t = 1; t < started; ++t)
+            pthread_join(th[t], NULL);
+        args[0].nthreads = 1;
+        started = 1;
+    }
+    ppr_worker(&args[0]);
+    for (uint32_t t = 1; t < started; ++t)
+        pthread_join(th[t], NULL);
+    for (uint32_t t = 0; t < started; ++t)
+        err |= args[t].err;
+    free(th);
+    free(args);
+    return err ? -1 : 0;
 }

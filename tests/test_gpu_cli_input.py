@@ -90,9 +90,12 @@ def test_cli_three_word_files_sssp(tmp_path, sched):
     got = column(out, float)
     got[got == -1.0] = np.inf
     assert np.array_equal(got, ref)  # %.17g round-trips every double
-    # the same 3-word file read by a non-SSSP application ignores the weights
-    run("-a", "bfs", "-i", ff, "-r", str(root), "-o", out)
-    _bfs_check(gf, out, root)
+    # the reference reads the weight word only for SSSP (source/application.c:171)
+    # and checks the file size against that (source/graphdata.c:963-979): a
+    # 3-word file given to BFS is refused as malformed
+    r = subprocess.run([W.CLI_PATH, "-a", "bfs", "-i", ff, "-r", str(root)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and "malformed" in r.stdout
 
 
 def test_cli_input_errors(tmp_path):

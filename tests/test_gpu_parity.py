@@ -153,7 +153,10 @@ def test_pr_autotune_resets_the_state(ctx):
         ctx.pr_iteration()
     tuned = ctx.read_ranks()
     again, _ = ctx.run_pr(iterations=12, damping=0.85)  # fresh run, same shape
-    assert np.array_equal(tuned, again)
+    if best[2] == W.PR_BINNED:  # its per-bin sums are reproducible to rounding only
+        assert float(np.abs(tuned - again).sum()) <= 1e-13
+    else:
+        assert np.array_equal(tuned, again)
     ref, _ = oracle.Oracle.of(g).pr(12, 0.85)
     assert float(np.abs(tuned - ref).sum()) <= PR_TOL
     with pytest.raises(W.WedgeError):

@@ -50,10 +50,11 @@ MAX_BLOCK_LOG2 = 10
 KERNELS = {
     "pr_tile": 0, "pr_span": 1, "pr_vertex": 2, "min_tile": 3, "min_blocks": 4,
     "bfs_dense": 5, "bfs_heavy": 6, "bfs_blocks": 7, "wedge_build": 8,
-    "advance": 9, "pr_acc": 10, "loop": 11, "batch_pr": 12,
+    "advance": 9, "pr_acc": 10, "loop": 11, "batch_pr": 12, "pb_push": 13, "pb_pull": 14,
 }
 
 OPT_RELABEL = 1
+OPT_PR_BIN_SLICE = 2
 RELABEL_OFF, RELABEL_ON, RELABEL_AUTO = 0, 1, 2
 
 UNREACHED_U64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -129,6 +130,7 @@ GPU_SYMBOLS = [
     ("wg_init_bfs_roots", C.c_int, [_p, _u64p, C.c_uint32]),
     ("wg_init_sssp_roots", C.c_int, [_p, _u64p, C.c_uint32]),
     ("wg_set_members", C.c_int, [_p, C.c_uint32, _u64p]),
+    ("wg_work_counts", C.c_int, [_p, _u64p, _u64p]),
     ("wg_validate", C.c_int, [_p, _u64p]),
     ("wg_validate_values", C.c_int, [_p, C.c_int, _u64p, C.c_uint32, _p, _u64p]),
     ("wg_validate_noalloc", C.c_int, [_p, _u64p]),
@@ -140,6 +142,7 @@ GPU_SYMBOLS = [
     ("wg_get_pr_shape", C.c_int, [_p, _p]),
     ("wg_pr_shape_at", C.c_int, [C.c_uint32, _p]),
     ("wg_pr_autotune", C.c_int, [_p, C.c_uint32, _p, _f32p]),
+    ("wg_pr_bins_info", C.c_int, [_p, _u64p, _u64p, _u32p, _u32p]),
     ("wg_edge_pull_pr", C.c_int, [_p, _f64p]),
     ("wg_vertex_pr", C.c_int, [_p]),
     ("wg_pr_iteration", C.c_int, [_p, _f64p]),
@@ -183,7 +186,8 @@ class BatchShape(C.Structure):
 
 
 BATCH_NSHAPES = 24
-PR_NSHAPES = BATCH_NSHAPES + 6
+PR_NSHAPES = BATCH_NSHAPES + 8
+PR_BINNED = 2  # xcd_split value of the binned PageRank pull
 
 
 class RunOpts(C.Structure):
@@ -573,6 +577,12 @@ class Context:
         r = np.ascontiguousarray(roots, np.uint64)
         self._chk(self._lib.wg_init_sssp_roots(self._h, _ptr(r, C.c_uint64), r.size), "wg_init_sssp_roots")
 
+    def work_counts(self) -> tuple[int, int]:
+        """(edges read, destination states examined) by the pulls since the init."""
+        e, d = C.c_uint64(), C.c_uint64()
+        self._chk(self._lib.wg_work_counts(self._h, C.byref(e), C.byref(d)), "wg_work_counts")
+        return int(e.value), int(d.value)
+
     def validate(self) -> int:
         """Violations of the current BFS / CC / SSSP result (0 = correct)."""
         n = C.c_uint64()
@@ -634,6 +644,15 @@ class Context:
         ms = np.zeros(PR_NSHAPES, np.float32)
         self._chk(self._lib.wg_pr_autotune(self._h, iterations, C.byref(best), _ptr(ms, C.c_float)), "wg_pr_autotune")
         return best.as_tuple(), {pr_shape_at(i): float(ms[i]) for i in range(PR_NSHAPES)}
+
+    def pr_bins_info(self) -> dict:
+        """Size of the binned PageRank layout (after init_pr with a binned shape)."""
+        pairs, q = C.c_uint64(), C.c_uint64()
+        push, pull = C.c_uint32(), C.c_uint32()
+        self._chk(self._lib.wg_pr_bins_info(self._h, C.byref(pairs), C.byref(q), C.byref(push), C.byref(pull)),
+                  "wg_pr_bins_info")
+        return {"pairs": int(pairs.value), "code_entries": int(q.value), "push_groups": int(push.value),
+                "pull_groups": int(pull.value)}
 
     def read_batch_ranks(self) -> np.ndarray:
         out = np.empty(self.V, np.float64)

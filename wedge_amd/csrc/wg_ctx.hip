@@ -278,6 +278,15 @@ extern "C" int wg_set_option(wg_ctx* ctx, int option, int64_t value)
             return WG_E_INVAL;
         ctx->relabel = (int)value;
         return WG_OK;
+    case WG_OPT_PR_BIN_SLICE:
+        if (value < 1 || value > (1 << 24))
+            return WG_E_INVAL;
+        ctx->pb_slice = (uint32_t)value;
+        if (ctx->pb) { // rebuilt with the new size on the next binned iteration
+            WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            wg_free_pb(ctx);
+        }
+        return WG_OK;
     default:
         return WG_E_INVAL;
     }
@@ -415,6 +424,7 @@ int wg_free_app_state(wg_ctx* ctx)
 static void free_graph(wg_ctx* ctx)
 {
     wg_free_members(ctx);
+    wg_free_pb(ctx);
     wg_free_app_state(ctx);
     free_dev((void**)&ctx->off);
     free_dev((void**)&ctx->src);

@@ -68,6 +68,19 @@ __device__ __forceinline__ void add_count(unsigned long long a, wg_dev_counters*
         atomicAdd(&c->slots[blockIdx.x & (WG_CSLOTS - 1)].v[f], a);
 }
 
+// Work of a pull (edges whose source it read, destination states it
+// examined), counted per thread.
+struct work_tally {
+    unsigned long long edges;
+    unsigned long long dests;
+};
+
+__device__ __forceinline__ void add_work(const work_tally& t, wg_dev_counters* __restrict__ c)
+{
+    add_count(t.edges, c, WG_CNT_EDGES_READ);
+    add_count(t.dests, c, WG_CNT_DESTS);
+}
+
 __device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* __restrict__ c)
 {
     add_count(t.edges, c, WG_CNT_ACTIVITY);
@@ -88,8 +101,12 @@ __device__ __forceinline__ unsigned long long* count_field(wg_dev_counters* c, i
         return &c->frontier_verts;
     case WG_CNT_REACHED_V:
         return &c->reached_verts;
-    default:
+    case WG_CNT_REACHED_E:
         return &c->reached_edges;
+    case WG_CNT_EDGES_READ:
+        return &c->edges_read;
+    default:
+        return &c->dests_examined;
     }
 }
 
@@ -119,6 +136,7 @@ __device__ inline void fold_counts(wg_dev_counters* c, unsigned long long* tot =
 #define RST_FRONTIER 4u
 #define RST_OVERFLOW 8u
 #define RST_HEAVY 16u
+#define RST_WORK 32u
 
 // Lane -> (marked block, edge) mapping shared by the frontier pulls.
 struct block_walk {

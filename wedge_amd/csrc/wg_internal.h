@@ -44,7 +44,8 @@
 // served in parallel.  A fold (one wavefront) adds the slots into the scalar
 // fields below and zeroes them.
 #define WG_CSLOTS 32
-enum { WG_CNT_ACTIVITY = 0, WG_CNT_ACTIVATED, WG_CNT_FRESH, WG_CNT_FRONTIER, WG_CNT_REACHED_V, WG_CNT_REACHED_E, WG_CNT_N };
+enum { WG_CNT_ACTIVITY = 0, WG_CNT_ACTIVATED, WG_CNT_FRESH, WG_CNT_FRONTIER, WG_CNT_REACHED_V, WG_CNT_REACHED_E,
+       WG_CNT_EDGES_READ, WG_CNT_DESTS, WG_CNT_N };
 struct __attribute__((aligned(256))) wg_count_slot {
     unsigned long long v[WG_CNT_N];
     unsigned long long pad[32 - WG_CNT_N];
@@ -65,6 +66,9 @@ struct wg_dev_counters {
     double total_rank;                  // PR: sum of accumulators of the last iteration
     unsigned long long reached_verts;
     unsigned long long reached_edges;
+    // work of the BFS / CC / SSSP pulls since their init (wg_work_counts):
+    unsigned long long edges_read;      // edges whose source the pulls read
+    unsigned long long dests_examined;  // destination states the pulls examined
     // control block of the device-resident iteration loop (wg_converge)
     unsigned int mode;                  // pull mode of the next iteration
     int blog2;                          // its Wedge granularity
@@ -207,6 +211,8 @@ struct wg_ctx {
     wg_tiling pr_alt[3];      // tilings with 4, 16 and 2 edges per lane (built on first use)
     int pr_alt_built;
     uint32_t pr_split8[9];    // XCD split of the default tiles
+    struct wg_pb* pb;         // layout of the binned PageRank pull (wg_pb.hip), built on first use
+    uint32_t pb_slice;        // option: pairs per pull workgroup of that layout (0 = default)
     uint64_t bm_words;        // 64-bit words per vertex bitmask
     uint64_t* bm_cur;         // BFS: current frontier
     uint64_t* bm_next;        // activated by the running pull
@@ -347,3 +353,10 @@ void wg_drop_graph(wg_ctx* ctx);
 void wg_tile_split_points(const std::vector<uint32_t>& te, uint64_t E, uint32_t start[9]);
 // builds the 4- and 16-edges-per-lane PageRank tilings once per graph
 int wg_build_pr_tilings(wg_ctx* ctx);
+// binned PageRank pull (wg_pb.hip): one iteration (mode 0 fused, 1 edge phase
+// only), building the layout for the current precision and shape first if needed
+int wg_pb_iteration(wg_ctx* ctx, int mode, const void* cin, void* cout, double* S_in, double* S_out);
+int wg_pb_prepare(wg_ctx* ctx);
+int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* tasks, uint32_t* slices);
+// releases the binned layout (graph change)
+void wg_free_pb(wg_ctx* ctx);

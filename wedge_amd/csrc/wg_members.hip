@@ -512,16 +512,26 @@ extern "C" int wg_batch_autotune(wg_ctx* ctx, uint32_t iterations, double dampin
     int rc = WG_OK;
     float best_ms = 0.0f;
     wg_batch_shape best_s = m->best;
+    // Candidates are timed with plain launches: the captured graph runs the same
+    // kernels (it saved nothing measurable on 256 graphs), and one capture for
+    // the chosen shape replaces 24 rounds of capture, instantiation and
+    // destruction.
     for (uint32_t i = 0; i < WG_BATCH_NSHAPES && rc == WG_OK; ++i) {
         wg_batch_shape s;
         wg_batch_shape_at(i, &s);
         float ms = 0.0f;
-        if ((rc = launch(ctx, m, s, iterations, damping, fp32, 1))) // capture + warm-up
+        if ((rc = launch(ctx, m, s, iterations, damping, fp32, 0))) // warm-up
             break;
         for (int rep = 0; rep < 3 && rc == WG_OK; ++rep) {
             float t = 0.0f;
-            if (hipEventRecord(ev[0], ctx->stream) != hipSuccess || hipGraphLaunch(m->exec, ctx->stream) != hipSuccess ||
-                hipEventRecord(ev[1], ctx->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+            if (hipEventRecord(ev[0], ctx->stream) != hipSuccess) {
+                ctx->last_hip_error = (int)hipGetLastError();
+                rc = WG_E_HIP;
+                break;
+            }
+            if ((rc = launch(ctx, m, s, iterations, damping, fp32, 0)))
+                break;
+            if (hipEventRecord(ev[1], ctx->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
                 hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess) {
                 ctx->last_hip_error = (int)hipGetLastError();
                 rc = WG_E_HIP;

@@ -140,6 +140,17 @@ static inline WG_HD int wg_mlp_settle(const float* P, int app, int best, float b
     return (plain_y - best_y > P[WG_MLP_OFF_MARGIN + k]) ? best : plain;
 }
 
+/* Below 1/WG_MLP_GATE of the plain rule's threshold, a dense pull (which reads
+ * every edge) is not a candidate, and the schedule takes the plain rule's
+ * frontier pull without evaluating the model.  Small-frontier levels are where
+ * a traversal spends most of its levels (all but a few on road-like graphs),
+ * and there the model's evaluation costs more than it can save. */
+#define WG_MLP_GATE 8ull
+static inline WG_HD int wg_mlp_gated(uint64_t activity, uint64_t threshold)
+{
+    return activity * WG_MLP_GATE <= threshold;
+}
+
 /* CC and SSSP keep the plain rule on graphs with more vertices than the model
  * was trained on (L(V) above the model's bound): their schedule also decides
  * how many pulls follow, which a per-pull model cannot extrapolate.  A BFS

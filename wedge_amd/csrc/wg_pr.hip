@@ -195,6 +195,8 @@ static int pr_launch(wg_ctx* ctx, int mode)
         S_out = &ctx->ctr->total_rank;
         cout = nullptr;
     }
+    if (ctx->pr_xcd == WG_PR_BINNED)
+        return wg_pb_iteration(ctx, mode, cin, cout, S_in, S_out);
     const pr_shape s = current_shape(ctx);
     // one bracket around tile + span: the iteration's device time
     const int slot = wg_prof_begin(ctx, mode == 0 ? WG_K_PR_TILE : WG_K_PR_ACC);
@@ -227,6 +229,14 @@ extern "C" int wg_set_pr_shape(wg_ctx* ctx, const wg_batch_shape* shape)
         return WG_OK;
     }
     const uint32_t ipt = shape->edges_per_lane;
+    if (shape->xcd_split == WG_PR_BINNED) {
+        if (ipt != 8 || shape->blocks_per_cu < 1 || shape->blocks_per_cu > 2)
+            return WG_E_INVAL;
+        ctx->pr_ipt = (int)ipt;
+        ctx->pr_bpc = (int)shape->blocks_per_cu;
+        ctx->pr_xcd = WG_PR_BINNED;
+        return WG_OK;
+    }
     if ((ipt != 2 && ipt != 4 && ipt != 8 && ipt != 16) || shape->blocks_per_cu < 1 || shape->blocks_per_cu > 16 ||
         shape->xcd_split > 1)
         return WG_E_INVAL;
@@ -257,7 +267,7 @@ extern "C" int wg_pr_shape_at(uint32_t i, wg_batch_shape* out)
 {
     // one 16-wave workgroup per CU for each tile size, then 128-edge tiles
     static const wg_batch_shape extra[WG_PR_NSHAPES - WG_BATCH_NSHAPES] = {
-        {4, 1, 0}, {8, 1, 0}, {16, 1, 0}, {2, 1, 0}, {2, 4, 0}, {2, 16, 0}};
+        {4, 1, 0}, {8, 1, 0}, {16, 1, 0}, {2, 1, 0}, {2, 4, 0}, {2, 16, 0}, {8, 1, WG_PR_BINNED}, {8, 2, WG_PR_BINNED}};
     if (!out || i >= WG_PR_NSHAPES)
         return WG_E_INVAL;
     if (i < WG_BATCH_NSHAPES)
@@ -325,6 +335,20 @@ extern "C" int wg_pr_autotune(wg_ctx* ctx, uint32_t iterations, wg_batch_shape* 
     if (best)
         *best = best_s;
     return WG_OK;
+}
+
+extern "C" int wg_pr_bins_info(wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* push_groups,
+                               uint32_t* pull_groups)
+{
+    if (!ctx || !pairs || !code_entries || !push_groups || !pull_groups)
+        return WG_E_INVAL;
+    if (ctx->app != WG_APP_PR || ctx->pr_xcd != WG_PR_BINNED)
+        return WG_E_STATE;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    const int rc = wg_pb_prepare(ctx);
+    if (rc)
+        return rc;
+    return wg_pb_stats(ctx, pairs, code_entries, push_groups, pull_groups);
 }
 
 extern "C" int wg_pr_iteration(wg_ctx* ctx, double* total_rank)

@@ -51,6 +51,10 @@ __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
         }
         if (flags & RST_FRONTIER)
             c->slots[t].v[WG_CNT_FRONTIER] = 0ull;
+        if (flags & RST_WORK) {
+            c->slots[t].v[WG_CNT_EDGES_READ] = 0ull;
+            c->slots[t].v[WG_CNT_DESTS] = 0ull;
+        }
     }
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         if (flags & RST_ACTIVITY) {
@@ -64,6 +68,10 @@ __global__ void k_reset(wg_dev_counters* c, unsigned int flags)
             c->overflow = 0u;
         if (flags & RST_HEAVY)
             c->heavy_len = 0u;
+        if (flags & RST_WORK) {
+            c->edges_read = 0ull;
+            c->dests_examined = 0ull;
+        }
     }
 }
 
@@ -136,6 +144,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
     const uint64_t nwaves = (uint64_t)gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     for (uint64_t w = (uint64_t)blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
         const uint64_t vis = visited[w];
         if (vis == ~0ull)
@@ -149,8 +158,12 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
         if (wg_ballot(todo) == 0ull)
             continue;
         int64_t parent = -1;
-        if (todo && test_cur(cur, s_cur, f))
-            parent = f;
+        if (todo) {
+            work.dests += 1ull;
+            work.edges += 1ull; // the smallest in-neighbour, from first[]
+            if (test_cur(cur, s_cur, f))
+                parent = f;
+        }
         // the rest of the in-list, if the smallest in-neighbour is not on the frontier
         const bool more = todo && parent < 0;
         uint32_t s = 0, e = 0;
@@ -160,6 +173,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
             const uint32_t lim = (e - s < WG_BFS_SEQ) ? e : s + WG_BFS_SEQ;
             for (uint32_t p = s + 1; p < lim; ++p) {
                 const uint32_t u = src[p];
+                work.edges += 1ull;
                 if (test_cur(cur, s_cur, u)) {
                     parent = u;
                     break;
@@ -180,6 +194,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
                 bool hit = false;
                 if (q < le) {
                     u = src[q];
+                    work.edges += 1ull;
                     hit = test_cur(cur, s_cur, u);
                 }
                 const unsigned long long hm = wg_ballot(hit);
@@ -204,6 +219,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
     }
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
@@ -234,10 +250,13 @@ __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* 
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     for (uint32_t c = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; c < n; c += nwaves) {
         const uint32_t v = hv[c];
         if (test_bit64(visited, v))
             continue;
+        if (lane == 0)
+            work.dests += 1ull;
         const uint32_t cs = he[c];
         const uint32_t vend = off[v + 1];
         const uint32_t ce = (cs + WG_BFS_HEAVY < vend) ? cs + WG_BFS_HEAVY : vend;
@@ -254,6 +273,7 @@ __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* 
             bool hit = false;
             if (q < ce) {
                 u = src[q];
+                work.edges += 1ull;
                 hit = test_bit64(cur, u);
             }
             const unsigned long long hm = wg_ballot(hit);
@@ -270,6 +290,7 @@ __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* 
     }
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
@@ -296,6 +317,7 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool first) {
         bool cand = false;
         uint32_t u = 0, v = 0;
@@ -311,6 +333,12 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
         const uint32_t pv = __shfl_up(v, 1);
         const unsigned long long cm = wg_ballot(cand);
         const bool prev_same = !first && ((cm >> (lane - 1)) & 1ull) && pv == v;
+        const unsigned long long okm = wg_ballot(ok);
+        if (ok) { // a destination's state is examined once per run of its edges in a block
+            work.edges += 1ull;
+            if (first || !((okm >> (lane - 1)) & 1ull) || pv != v)
+                work.dests += 1ull;
+        }
         if (cand && !prev_same) {
             const uint32_t pid = perm ? perm[u] : u;
             atomicMin((unsigned long long*)&packed[v], ((uint64_t)newdepth << 32) | (uint64_t)pid);
@@ -319,6 +347,7 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
     });
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 // The first frontier-filtered level of a traversal whose frontier is still its
@@ -344,7 +373,10 @@ __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t total = root_off[nroots];
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     for (uint64_t g = tid; g < total; g += nthreads) {
+        work.edges += 1ull; // one out-edge of a root and its destination's state
+        work.dests += 1ull;
         // the last root whose out-edges start at or before g (roots without
         // out-edges share their successor's start and are passed over)
         uint32_t lo = 0, hi = nroots - 1;
@@ -371,6 +403,7 @@ __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t
     }
     act.fresh = act.verts; // a BFS activation is always a first visit
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 // root_off[i] = out-degree sum of roots[0..i), root_off[n] = the total; one
@@ -594,12 +627,14 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
     uint32_t* s_off = s_off_all[wave];
     T* s_min = s_min_all[wave];
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     const uint32_t stride = gridDim.x * WG_WPB;
     for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
         const uint32_t tv = tile_v[t];
         const uint32_t e0 = tile_e[t], e1 = tile_e[t + 1];
         const uint32_t v0 = tv & ~WG_HEAVY_BIT;
         const uint32_t a = e0 + (uint32_t)lane * IPT;
+        work.edges += (a < e1) ? ((e1 - a < (uint32_t)IPT) ? e1 - a : (uint32_t)IPT) : 0u;
         T g[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
@@ -620,11 +655,16 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
             for (int j = 0; j < IPT; ++j)
                 m = g[j] < m ? g[j] : m;
             m = wg_wave_min(m);
-            if (lane == 0 && m != P::INF)
-                min_apply<P>(prop, next, od, v0, m, act);
+            if (lane == 0) {
+                work.dests += 1ull;
+                if (m != P::INF)
+                    min_apply<P>(prop, next, od, v0, m, act);
+            }
             continue;
         }
         const uint32_t nv = (tile_v[t + 1] & ~WG_HEAVY_BIT) - v0;
+        if (lane == 0)
+            work.dests += nv;
 #pragma unroll
         for (int k = 0; k < NOFF; ++k) {
             const uint32_t i = (uint32_t)lane + (uint32_t)k * WG_WAVE;
@@ -685,6 +725,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
         wg_wave_sync();
     }
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 template <class P, int IPT = WG_IPT>
@@ -712,14 +753,19 @@ __device__ void min_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 {
     typedef typename P::T T;
     act_tally act = { 0ull, 0ull, 0ull };
+    work_tally work = { 0ull, 0ull };
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool) {
         if (ok) {
+            work.edges += 1ull;
             const T m = P::val(prop, w, src[p], p, ovf);
-            if (m != P::INF)
+            if (m != P::INF) {
+                work.dests += 1ull;
                 min_apply<P>(prop, next, od, dst[p], m, act);
+            }
         }
     });
     add_tally(act, ctr);
+    add_work(work, ctr);
 }
 
 template <class P>
@@ -916,7 +962,8 @@ __device__ __forceinline__ void loop_choose_wave(const loop_cfg& g, const float*
         *mode = WG_MODE_FRONTIER;
         return;
     case LOOP_SCHED_LEARNED:
-        if (mlp && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
+        // below the gate: the plain rule's frontier pull (wg_mlp_gated)
+        if (mlp && !wg_mlp_gated(act, g.threshold) && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
             float x[WG_MLP_IN];
             float base[WG_MLP_ACTIONS];
             wg_mlp_features((int)g.app, iteration, g.nv, g.ne, nact, act, unvisited, x);
@@ -1491,6 +1538,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
         return;
     uint4* qc = qa;
     uint4* qn = qb;
+    work_tally work = { 0ull, 0ull };
     for (unsigned int r = 0;; ++r) {
         const uint32_t nc = run_queue_len(rc, r);
         const unsigned long long newdepth = (unsigned long long)(st.level0 + 2u + r);
@@ -1516,6 +1564,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
                 }
                 const uint32_t pid = perm ? perm[u] : u;
                 const unsigned long long val = (newdepth << 32) | (unsigned long long)pid;
+                work.edges += en.z - en.y; // each out-edge and its destination's state
+                work.dests += en.z - en.y;
 #pragma unroll
                 for (int j = 0; j < (int)WG_RUN_CH; ++j)
                     if (en.y + (uint32_t)j < en.z)
@@ -1542,6 +1592,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
             run_append<WG_RUN_CH>(qn, tn, vv, so, eo, nw, k);
         }
         bool ok = true;
+        add_work(work, ctr);
+        work = { 0ull, 0ull };
         // a BFS activation is always a first visit
         if (!run_level_end(rc, ctr, r, act, nact, nact, st, s_mlp, learned, &ok))
             return;
@@ -1579,6 +1631,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
         return;
     uint4* qc = qa;
     uint4* qn = qb;
+    work_tally work = { 0ull, 0ull };
     for (unsigned int r = 0;; ++r) {
         uint64_t* dc = (r & 1u) ? spare : next;
         uint64_t* dn = (r & 1u) ? next : spare;
@@ -1600,6 +1653,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
                 const uint32_t u = en.x;
                 if (en.w)
                     atomicAnd((unsigned long long*)&dc[u >> 6], ~(1ull << (u & 63u)));
+                work.edges += en.z - en.y; // each out-edge and its destination's state
+                work.dests += en.z - en.y;
 #pragma unroll
                 for (int j = 0; j < (int)WG_RUN_CH; ++j)
                     if (en.y + (uint32_t)j < en.z) {
@@ -1636,6 +1691,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
             run_append<WG_RUN_CH>(qn, tn, vv, so, eo, nw, k);
         }
         bool ok = true;
+        add_work(work, ctr);
+        work = { 0ull, 0ull };
         if (!run_level_end(rc, ctr, r, act, nact, fresh, st, s_mlp, learned, &ok)) {
             if (ok && (r & 1u) == 0u) // the last activations are in spare
                 for (uint64_t wd = (uint64_t)blockIdx.x * WG_BLOCK + threadIdx.x; wd < nwords;
@@ -1686,7 +1743,7 @@ static int init_common(wg_ctx* ctx, int app)
     int rc;
     if ((rc = wg_clear_u64(ctx, ctx->bm_cur, ctx->bm_words)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
         (rc = wg_clear_u64(ctx, ctx->bm_visited, ctx->bm_words)) || (rc = clear_wedge_state(ctx)) ||
-        (rc = wg_reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW)))
+        (rc = wg_reset_counters(ctx, RST_ACTIVITY | RST_FRONTIER | RST_OVERFLOW | RST_WORK)))
         return rc;
     ctx->app = app;
     ctx->iter = 0;
@@ -2141,6 +2198,21 @@ extern "C" int wg_read_sssp(wg_ctx* ctx, void* host_dist)
     if (!ctx)
         return WG_E_INVAL;
     return read_prop(ctx, WG_APP_SSSP, host_dist, ctx->wkind == WG_WEIGHT_F64 ? sizeof(uint64_t) : sizeof(uint32_t));
+}
+
+extern "C" int wg_work_counts(wg_ctx* ctx, uint64_t* edges_read, uint64_t* dests_examined)
+{
+    if (!ctx || (ctx->app != WG_APP_BFS && ctx->app != WG_APP_SSSP && ctx->app != WG_APP_CC))
+        return WG_E_STATE;
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    const int rc = wg_read_counters(ctx);
+    if (rc)
+        return rc;
+    if (edges_read)
+        *edges_read = ctx->hctr->edges_read;
+    if (dests_examined)
+        *dests_examined = ctx->hctr->dests_examined;
+    return WG_OK;
 }
 
 extern "C" int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges)

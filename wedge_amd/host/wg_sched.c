@@ -4,6 +4,7 @@
 #include "wg_sched.h"
 
 #include "wg_mlp.h"
+#include "wg_mlp_math.h"
 
 void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_log2, const wg_sched_features* f,
                      int* mode, int* block_log2)
@@ -17,7 +18,9 @@ void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_l
         *mode = WG_MODE_FRONTIER;
         return;
     case WG_SCHED_LEARNED:
-        if (wg_mlp_decide(f, f->frontier_edges > threshold, mode, block_log2) == WG_OK)
+        /* below the gate: the plain rule's frontier pull (wg_mlp_gated) */
+        if (!wg_mlp_gated(f->frontier_edges, threshold) &&
+            wg_mlp_decide(f, f->frontier_edges > threshold, mode, block_log2) == WG_OK)
             return;
         /* no model loaded: the plain heuristic is the fallback */
         /* fall through */
