@@ -94,12 +94,16 @@ def shape_name(shape) -> str:
     return "binned" if shape[2] == W.PR_BINNED else ("tiles-xcd" if shape[2] else "tiles")
 
 
-def pr_timed(ctx, damping, fp32, steps, warmup):
-    """PageRank with the autotuned shape: wall ms per iteration over `steps`
-    iterations and the iteration's kernel ms per launch (HIP events on the
-    engine's stream)."""
+def pr_timed(ctx, damping, fp32, steps, warmup, shape=None):
+    """PageRank with the autotuned shape (or the given one): wall ms per
+    iteration over `steps` iterations and the iteration's kernel ms per launch
+    (HIP events on the engine's stream)."""
     ctx.init_pr(damping, fp32)
-    shape, times = ctx.pr_autotune(2)
+    if shape is None:
+        shape, times = ctx.pr_autotune(2)
+    else:
+        ctx.set_pr_shape(shape)
+        shape, times = ctx.pr_shape(), {}
     for _ in range(warmup):
         ctx.pr_iteration()
     ctx.sync()
@@ -133,13 +137,30 @@ def roofline_entry(alg: int, kern: dict, traffic):
     }
 
 
+def counted_run(ctx, run):
+    """One more run with the kernels that count their work (slower, so never
+    the timed one); the work it reports, and its iteration count."""
+    ctx.set_option(W.OPT_COUNT_WORK, 1)
+    try:
+        _, st = run()
+        e, d = ctx.work_counts()
+    finally:
+        ctx.set_option(W.OPT_COUNT_WORK, 0)
+    return (e, d), st["iterations"]
+
+
 def run_best(ctx, app, root, reps, **kw):
+    """Best of reps timed runs, and the work of the same run counted apart."""
+    run = (lambda: ctx.run_bfs(root, **kw)) if app == "bfs" else (lambda: ctx.run_sssp(root, **kw))
     best = None
     for _ in range(reps):
-        res, st = ctx.run_bfs(root, **kw) if app == "bfs" else ctx.run_sssp(root, **kw)
+        res, st = run()
         if best is None or st["running_ms"] < best[1]["running_ms"]:
-            best = (res, st, ctx.work_counts())
-    return best
+            best = (res, st)
+    work, it = counted_run(ctx, run)
+    if it != best[1]["iterations"]:
+        log(f"{app} {kw}: the counted run took {it} iterations, the timed one {best[1]['iterations']}")
+    return best[0], best[1], work
 
 
 def traversal_entry(app, V, st, work, wbytes=0, violations=None):
@@ -339,10 +360,11 @@ def cc_pair(scale, edge_factor, seed, reps=2, cpu=True):
         for _ in range(reps):
             lab, st = ctx.run_cc(schedule=sched)
             if best is None or st["running_ms"] < best[1]["running_ms"]:
-                best = (lab, st, ctx.work_counts())
+                best = (lab, st)
+        work, _ = counted_run(ctx, lambda: ctx.run_cc(schedule=sched))
         res[sched] = best[0]
         st = best[1]
-        e = traversal_entry("cc", g.num_vertices, st, best[2], violations=ctx.validate())
+        e = traversal_entry("cc", g.num_vertices, st, work, violations=ctx.validate())
         e["bedges_per_s"] = round(g.num_edges * st["iterations"] / (st["running_ms"] * 1e-3) / 1e9, 2)
         out[f"cc_{sched}"] = e
     out["cc_learned_vs_plain"] = round(out["cc_plain"]["ms"] / out["cc_learned"]["ms"], 3)
@@ -374,8 +396,11 @@ def large_graph(args):
     log(f"uploaded + device layout built in {time.time() - t0:.1f} s")
     out = {"workload": "rmat-s27-ef16-seed5", "vertices": V, "edges": E, "root": root}
     res = {}
-    for fp32 in (False, True):
-        wall, kern, shape, times = pr_timed(ctx, args.damping, fp32, args.steps if not fp32 else 5, args.warmup)
+    for fp32 in (False, True) if not args.no_fp32 else (False,):
+        if args.no_pr:
+            break
+        wall, kern, shape, times = pr_timed(ctx, args.damping, fp32, args.steps if not fp32 else 5, args.warmup,
+                                            args.pr_shape)
         alg = pr_algorithmic_bytes(V, E, fp32)
         traffic = measured_traffic(f"pr_rmat_s27_{'fp32_' if fp32 else ''}hbm_traffic.md")
         key = "pr_fp32" if fp32 else "pr_fp64"
@@ -388,27 +413,30 @@ def large_graph(args):
             res[key]["binned_layout"] = ctx.pr_bins_info()
         log(f"s27 {key}: {res[key]}")
     out.update(res)
-    ranks = {}
-    for fp32 in (False, True):
-        ranks[fp32], _ = ctx.run_pr(iterations=16, damping=args.damping, contrib_fp32=fp32)
-    out["pr_fp32_l1_vs_fp64"] = float(np.abs(ranks[True] - ranks[False]).sum())
-    out["pr_iterations_compared"] = 16
-    del ranks
+    if not args.no_fp32 and not args.no_pr:
+        ranks = {}
+        for fp32 in (False, True):
+            ranks[fp32], _ = ctx.run_pr(iterations=16, damping=args.damping, contrib_fp32=fp32)
+        out["pr_fp32_l1_vs_fp64"] = float(np.abs(ranks[True] - ranks[False]).sum())
+        out["pr_iterations_compared"] = 16
+        del ranks
     W.mlp_load()
     bfs = {}
-    for sched in ("plain", "learned"):
+    for sched in ("plain", "learned") if not args.no_bfs else ():
         r, st, work = run_best(ctx, "bfs", root, 3, schedule=sched)
         bfs[sched] = r
         out[f"bfs_{sched}"] = traversal_entry("bfs", V, st, work, violations=ctx.validate())
         log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
-    out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
-    out["bfs_identical"] = bool(np.array_equal(bfs["plain"], bfs["learned"]))
+    if not args.no_bfs:
+        out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
+        out["bfs_identical"] = bool(np.array_equal(bfs["plain"], bfs["learned"]))
     del bfs
     ctx.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, args.damping, root, args.cpu_budget)
-        out["bfs_gpu_vs_cpu"] = round(cpu["bfs"]["ms"] / out["bfs_learned"]["ms"], 1)
+        if not args.no_bfs:
+            out["bfs_gpu_vs_cpu"] = round(cpu["bfs"]["ms"] / out["bfs_learned"]["ms"], 1)
     return g, out, cpu
 
 
@@ -482,7 +510,14 @@ def main():
     ap.add_argument("--no-grid", action="store_true", help="skip the 4096x6144 road grid")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sources and graphs")
     ap.add_argument("--no-s20", action="store_true", help="skip the scale-20 PageRank")
+    ap.add_argument("--pr-shape", default=None,
+                    help="PageRank shape edges_per_lane,blocks_per_cu,kind (kind 0 tiles, 1 tiles with XCD split, "
+                         "2 binned) instead of the autotuner's choice")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-stored PageRank on the headline graph")
+    ap.add_argument("--no-bfs", action="store_true", help="skip BFS on the headline graph")
+    ap.add_argument("--no-pr", action="store_true", help="skip PageRank on the headline graph (profiling BFS alone)")
     args = ap.parse_args()
+    args.pr_shape = tuple(int(x) for x in args.pr_shape.split(",")) if args.pr_shape else None
 
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built",
@@ -494,6 +529,9 @@ def main():
         copy_gbs = round(c.measure_copy(4 << 30), 1)
     g27, large, cpu = large_graph(args)
     del g27
+    if args.no_pr:  # BFS alone (a profiling run): no headline line
+        print(json.dumps({"large": large}), flush=True)
+        return
     pr = large["pr_fp64"]
     out = {
         "metric": METRIC,

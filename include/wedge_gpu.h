@@ -106,6 +106,15 @@ int wg_sync(wg_ctx* ctx);
                                   262144); a bin with more is split over several
                                   workgroups whose accumulators are then added.
                                   Takes effect at the next layout build */
+#define WG_OPT_MS_BINNED 3     /* dense levels of the 64-source BFS (wg_msbfs) on the binned
+                                  layout of WG_PR_BINNED (OR of frontier masks per
+                                  window-destination pair, then per bin): 0 off (edge
+                                  tiles), 1 on, 2 (default) on for graphs of 2^24 edges
+                                  or more */
+#define WG_OPT_COUNT_WORK 4    /* 1: the BFS / CC / SSSP pulls count the work that
+                                  wg_work_counts reports (kernels built with the
+                                  counts, a little slower); 0 (default): they do not.
+                                  May be changed between runs */
 int wg_set_option(wg_ctx* ctx, int option, int64_t value);
 
 /* Uploads a destination-grouped CSC and builds every derived device structure:
@@ -427,7 +436,7 @@ int wg_reached(wg_ctx* ctx, uint64_t* vertices, uint64_t* edges);
  * block's edge, a pushed out-edge), dests_examined = destination states a
  * pull examined (a destination group of a dense pull, a run of one
  * destination's edges in a marked block, the destination of a pushed
- * out-edge). */
+ * out-edge).  WG_E_STATE unless WG_OPT_COUNT_WORK is 1. */
 int wg_work_counts(wg_ctx* ctx, uint64_t* edges_read, uint64_t* dests_examined);
 
 /* ---- learned schedule --------------------------------------------------------

@@ -196,3 +196,25 @@ def test_mssssp_errors(ctx):
     ctx.mssssp([0, 5])
     with pytest.raises(W.WedgeError):
         ctx.read_mssssp(2)
+
+
+@pytest.mark.parametrize("slice_pairs", [262144, 29])
+def test_msbfs_binned_dense_levels_match_oracle(slice_pairs):
+    """Dense levels on the binned layout (WG_OPT_MS_BINNED = 1): frontier masks
+    ORed per (source window, destination) pair, then per destination bin,
+    bins split over many pull workgroups with a tiny WG_OPT_PR_BIN_SLICE."""
+    c = W.Context(0)
+    try:
+        c.set_option(W.OPT_MS_BINNED, 1)
+        c.set_option(W.OPT_PR_BIN_SLICE, slice_pairs)
+        for name, g in _graphs() + [("rmat16", W.rmat(16, 8, seed=5))]:
+            c.load(g)
+            roots = _roots(g, 64, seed=21)
+            orc = oracle.Oracle.of(g, 8)
+            want = [orc.bfs(int(r))[1] for r in roots]
+            for opts in (dict(schedule="dense"), dict(schedule="plain")):
+                c.msbfs(roots, **opts)
+                for i in range(64):
+                    assert np.array_equal(c.read_msbfs(i), want[i]), f"{name} {opts} root #{i}"
+    finally:
+        c.close()

@@ -119,6 +119,9 @@ def test_learned_schedule_keeps_results(ctx, seed):
                                                                          st_h["frontier_iterations"])
                 rows = W.trace_rows()
                 for r in rows:  # the host loop's choices are the model's
+                    if r["frontier_edges"] <= g.num_edges >> W.MLP_GATE_LOG2:  # gated: the plain rule's frontier pull
+                        assert (r["mode"], r["block_log2"]) == (W.MODE_FRONTIER, W.DEFAULT_BLOCK_LOG2)
+                        continue
                     x = W.mlp_features(W.APP_BFS, r["iteration"], g.num_vertices, g.num_edges,
                                        r["frontier_vertices"], r["frontier_edges"], r["unvisited"])
                     plain = 0 if r["frontier_edges"] > g.num_edges * 20 // 100 else 2

@@ -55,6 +55,8 @@ KERNELS = {
 
 OPT_RELABEL = 1
 OPT_PR_BIN_SLICE = 2
+OPT_MS_BINNED = 3
+OPT_COUNT_WORK = 4
 RELABEL_OFF, RELABEL_ON, RELABEL_AUTO = 0, 1, 2
 
 UNREACHED_U64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -102,6 +104,7 @@ _i32p = C.POINTER(C.c_int32)
 
 # learned schedule model shape (include/wedge_gpu.h)
 MLP_IN, MLP_HIDDEN, MLP_ACTIONS = 12, 64, 6
+MLP_GATE_LOG2 = 11  # WG_MLP_GATE_LOG2: at activity <= E >> 11 the learned schedule takes the plain frontier pull
 MLP_PARAMS = 4 * MLP_IN + MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + \
     MLP_HIDDEN * MLP_ACTIONS + MLP_ACTIONS + 4  # + margins for BFS, CC, SSSP and lv_max
 
@@ -578,7 +581,8 @@ class Context:
         self._chk(self._lib.wg_init_sssp_roots(self._h, _ptr(r, C.c_uint64), r.size), "wg_init_sssp_roots")
 
     def work_counts(self) -> tuple[int, int]:
-        """(edges read, destination states examined) by the pulls since the init."""
+        """(edges read, destination states examined) by the pulls since the init
+        (needs set_option(OPT_COUNT_WORK, 1) before the run)."""
         e, d = C.c_uint64(), C.c_uint64()
         self._chk(self._lib.wg_work_counts(self._h, C.byref(e), C.byref(d)), "wg_work_counts")
         return int(e.value), int(d.value)

@@ -353,12 +353,17 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         uint64_t fverts = nroots;                     // vertices in the frontier (at most)
         const unsigned long long threshold = (ctx->E * (unsigned long long)params->threshold_pct) / 100ull;
         const uint32_t cgrid = wg_grid(ctx, ctx->bm_words, WG_WPB, 16);
+        const int opt = ctx->ms_binned ? ctx->ms_binned - 1 : 2;
+        const bool binned = opt == 1 || (opt == 2 && ctx->E >= (1ull << 24));
         uint32_t level = 0;
         for (;;) {
             const bool dense = params->schedule == 2 || (params->schedule != 3 && act > threshold);
             if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
                 goto out;
-            if (dense) {
+            if (dense && binned) {
+                if ((rc = wg_pb_msbfs_level(ctx, dfront, dseen, dnl, live)))
+                    goto out;
+            } else if (dense) {
                 if (fverts < (uint64_t)V / 8)
                     hipLaunchKernelGGL(k_ms_tile<true>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
                                        ctx->src, ctx->bm_next, dfront, dseen, dnl, ctx->tile_v, ctx->tile_e, ctx->ntiles,

@@ -278,6 +278,16 @@ extern "C" int wg_set_option(wg_ctx* ctx, int option, int64_t value)
             return WG_E_INVAL;
         ctx->relabel = (int)value;
         return WG_OK;
+    case WG_OPT_MS_BINNED:
+        if (value < 0 || value > 2)
+            return WG_E_INVAL;
+        ctx->ms_binned = (int)value + 1; // 0 = unset: the default
+        return WG_OK;
+    case WG_OPT_COUNT_WORK:
+        if (value < 0 || value > 1)
+            return WG_E_INVAL;
+        ctx->count_work = (int)value;
+        return WG_OK;
     case WG_OPT_PR_BIN_SLICE:
         if (value < 1 || value > (1 << 24))
             return WG_E_INVAL;

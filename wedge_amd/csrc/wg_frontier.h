@@ -69,16 +69,30 @@ __device__ __forceinline__ void add_count(unsigned long long a, wg_dev_counters*
 }
 
 // Work of a pull (edges whose source it read, destination states it
-// examined), counted per thread.
+// examined), counted per thread in the kernels instantiated with CW = true
+// (WG_OPT_COUNT_WORK); with CW = false the counts compile away.
+template <bool CW>
 struct work_tally {
-    unsigned long long edges;
-    unsigned long long dests;
+    unsigned long long edges = 0ull;
+    unsigned long long dests = 0ull;
 };
 
-__device__ __forceinline__ void add_work(const work_tally& t, wg_dev_counters* __restrict__ c)
+struct work_none {
+    __device__ __forceinline__ work_none& operator+=(unsigned long long) { return *this; }
+};
+
+template <>
+struct work_tally<false> {
+    work_none edges, dests;
+};
+
+template <bool CW>
+__device__ __forceinline__ void add_work(const work_tally<CW>& t, wg_dev_counters* __restrict__ c)
 {
-    add_count(t.edges, c, WG_CNT_EDGES_READ);
-    add_count(t.dests, c, WG_CNT_DESTS);
+    if constexpr (CW) {
+        add_count(t.edges, c, WG_CNT_EDGES_READ);
+        add_count(t.dests, c, WG_CNT_DESTS);
+    }
 }
 
 __device__ __forceinline__ void add_tally(const act_tally& t, wg_dev_counters* __restrict__ c)

@@ -213,6 +213,8 @@ struct wg_ctx {
     uint32_t pr_split8[9];    // XCD split of the default tiles
     struct wg_pb* pb;         // layout of the binned PageRank pull (wg_pb.hip), built on first use
     uint32_t pb_slice;        // option: pairs per pull workgroup of that layout (0 = default)
+    int ms_binned;            // option WG_OPT_MS_BINNED + 1 (0 = unset: the default)
+    int count_work;           // option WG_OPT_COUNT_WORK
     uint64_t bm_words;        // 64-bit words per vertex bitmask
     uint64_t* bm_cur;         // BFS: current frontier
     uint64_t* bm_next;        // activated by the running pull
@@ -358,5 +360,9 @@ int wg_build_pr_tilings(wg_ctx* ctx);
 int wg_pb_iteration(wg_ctx* ctx, int mode, const void* cin, void* cout, double* S_in, double* S_out);
 int wg_pb_prepare(wg_ctx* ctx);
 int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* tasks, uint32_t* slices);
+// a dense level of the 64-source BFS on the binned layout: nl[v] = (OR of the
+// in-neighbours' front & live) & ~seen[v] where nonzero (other entries untouched)
+int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsigned long long* seen,
+                      unsigned long long* nl, unsigned long long live);
 // releases the binned layout (graph change)
 void wg_free_pb(wg_ctx* ctx);

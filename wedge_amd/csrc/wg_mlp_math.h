@@ -140,15 +140,16 @@ static inline WG_HD int wg_mlp_settle(const float* P, int app, int best, float b
     return (plain_y - best_y > P[WG_MLP_OFF_MARGIN + k]) ? best : plain;
 }
 
-/* Below 1/WG_MLP_GATE of the plain rule's threshold, a dense pull (which reads
- * every edge) is not a candidate, and the schedule takes the plain rule's
- * frontier pull without evaluating the model.  Small-frontier levels are where
- * a traversal spends most of its levels (all but a few on road-like graphs),
- * and there the model's evaluation costs more than it can save. */
-#define WG_MLP_GATE 8ull
-static inline WG_HD int wg_mlp_gated(uint64_t activity, uint64_t threshold)
+/* A level whose frontier pull reads at most 1/8 of the edges even at the
+ * coarsest block granularity (activity * 2^8 * 8 <= E) is one where the dense
+ * pull, which reads every edge, cannot win: there the schedule takes the plain
+ * rule's frontier pull without evaluating the model.  Such levels are most of
+ * a traversal's levels on road-like graphs (thousands of them), and there the
+ * model's evaluation costs more than it can save. */
+#define WG_MLP_GATE_LOG2 11
+static inline WG_HD int wg_mlp_gated(uint64_t activity, uint64_t num_edges)
 {
-    return activity * WG_MLP_GATE <= threshold;
+    return activity <= (num_edges >> WG_MLP_GATE_LOG2);
 }
 
 /* CC and SSSP keep the plain rule on graphs with more vertices than the model

@@ -38,19 +38,23 @@
 //
 // Layout (built once per graph and window size by pb_build, device side):
 //   code[q]        the edges by (window, destination, source), each window's run
-//                  padded to a multiple of WG_PB_TILE entries: bits 14..0 =
+//                  padded to a multiple of WG_PB_TILE entries: bits 13..0 =
 //                  source - window start, bit 15 = last edge of its pair (also
-//                  forced on a tile's last entry, so no pair crosses tiles)
-//   tile_pbase[t]  number of pairs before tile t (WG_PB_TILE entries per tile)
+//                  forced on a tile's last entry, so no pair crosses tiles),
+//                  bit 14 = on a pair's last edge: the pair starts a block
+//   blocks         runs of pairs of one (window, bin); the push numbers the
+//                  pairs in window order, the pull reads them in bin order
+//                  (window order within a bin): pair p of block i goes to
+//                  position p + blk_off[i] of part[] (32-bit wrap-around)
+//   tile_pbase[t]  pairs before tile t (WG_PB_TILE entries per tile)
+//   tile_bcnt[t]   blocks started before tile t's first pair
 //   task[k]        push work: {window, first tile, end tile, end of the real
 //                  entries (code index)}, at most WG_PB_TASK_TILES tiles
-//   pdst[p]        pair p's destination - bin start
-//   part[p]        pair p's partial sum (written by the push every iteration)
-//   piece[i]       {first pair, count}: runs of pairs of one (window, bin), at
-//                  most WG_PB_PIECE pairs, grouped by bin in window order
-//   slice[j]       pull work: {bin, first piece, end piece, accumulator slot or
-//                  WG_PB_NOSLOT when the slice is its bin's only one}; a slice
-//                  holds at most WG_PB_SLICE pairs (or one piece)
+//   part[o]        partial sums in bin order (written by the push every
+//                  iteration), pdst[o] their destinations - bin start
+//   slice[j]       pull work: {bin, first position, end position, accumulator
+//                  slot or WG_PB_NOSLOT when the slice is its bin's only one},
+//                  at most WG_PB_SLICE pairs, a contiguous range of part[]
 //   bin_slot[b]    {first slot, slices} of a bin cut into several slices: each
 //                  slice writes its accumulators to its slot, and the last one
 //                  to finish (a per-bin ticket) adds the slots in slice order
@@ -64,23 +68,26 @@
 #define WG_PB_THREADS 1024                      // 16 wavefronts, one workgroup per CU
 #define WG_PB_WAVES (WG_PB_THREADS / WG_WAVE)
 #define WG_PB_TILE 512u                         // code entries per wavefront tile
-#define WG_PB_TASK_TILES 256u                   // tiles per push workgroup (128K edges)
-#define WG_PB_PIECE 4096u                       // pairs per piece (at most)
+#define WG_PB_TASK_TILES 1024u                  // tiles per push workgroup (512K edges)
 #define WG_PB_SLICE (256u * 1024u)              // pairs per pull workgroup (default)
 #define WG_PB_NOSLOT 0xffffffffu
-#define WG_PB_END 0x8000u
+#define WG_PB_END 0x8000u                       // code: last edge of a pair
+#define WG_PB_BSTART 0x4000u                    // code: that pair starts a block
+#define WG_PB_SRC 0x3fffu                       // code: source - window start
+#define WG_PB_MAX_WS_LOG 14
 
 struct wg_pb {
     int ws_log, wd_log;
     uint32_t S, B;
     uint64_t nq, npairs;
-    uint32_t ntiles, ntasks, npieces, nslices, nslots;
+    uint32_t ntiles, ntasks, nblocks, nslices, nslots;
     uint16_t* code;
     uint32_t* tile_pbase;
+    uint32_t* tile_bcnt;
+    uint32_t* blk_off;
     uint4* task;
     uint16_t* pdst;
     double* part;
-    uint2* piece;
     uint4* slice;
     uint2* bin_slot;
     double* slot_acc;
@@ -102,10 +109,11 @@ void pb_release(wg_pb* pb)
 {
     pb_free_dev(pb->code);
     pb_free_dev(pb->tile_pbase);
+    pb_free_dev(pb->tile_bcnt);
+    pb_free_dev(pb->blk_off);
     pb_free_dev(pb->task);
     pb_free_dev(pb->pdst);
     pb_free_dev(pb->part);
-    pb_free_dev(pb->piece);
     pb_free_dev(pb->slice);
     pb_free_dev(pb->bin_slot);
     pb_free_dev(pb->slot_acc);
@@ -241,6 +249,50 @@ __global__ void k_pb_block_keys(const uint32_t* __restrict__ start, uint32_t nb,
         key[i] = pkey[start[i]];
 }
 
+// Marks the last edge of every pair that starts a block.
+__global__ void k_pb_mark_blocks(uint64_t nq, uint16_t* __restrict__ code, const uint32_t* __restrict__ pidx,
+                                 const uint8_t* __restrict__ bflag)
+{
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nq; q += (uint64_t)gridDim.x * blockDim.x) {
+        const uint16_t c = code[q];
+        if ((c & WG_PB_END) && bflag[pidx[q]])
+            code[q] = (uint16_t)(c | WG_PB_BSTART);
+    }
+}
+
+// Number of entries of the ascending array a[0..n) below x.
+__device__ __forceinline__ uint32_t pb_lower_bound(const uint32_t* __restrict__ a, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// tile_bcnt[t]: blocks started before tile t's first pair.
+__global__ void k_pb_tile_blocks(const uint32_t* __restrict__ tile_pbase, uint32_t ntiles,
+                                 const uint32_t* __restrict__ bstart, uint32_t nb, uint32_t* __restrict__ tile_bcnt)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles)
+        tile_bcnt[t] = pb_lower_bound(bstart, nb, tile_pbase[t]);
+}
+
+// Pair destinations into bin order: pair p of block i goes to p + blk_off[i].
+__global__ void k_pb_place(const uint16_t* __restrict__ pdst_w, uint32_t npairs, const uint32_t* __restrict__ bstart,
+                           uint32_t nb, const uint32_t* __restrict__ blk_off, uint16_t* __restrict__ pdst_b)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
+        const uint32_t i = pb_lower_bound(bstart, nb, p + 1u) - 1u; // the last block starting at or before p
+        pdst_b[p + blk_off[i]] = pdst_w[p];
+    }
+}
+
 #define PB_TRY(call)                                                   \
     do {                                                               \
         hipError_t _e = (call);                                        \
@@ -264,7 +316,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     const uint64_t V = ctx->V, E = ctx->E;
     const uint32_t S = (uint32_t)((V + (1ull << ws_log) - 1) >> ws_log);
     const uint32_t B = (uint32_t)((V + (1ull << wd_log) - 1) >> wd_log);
-    if (S > 65536u || B > 65536u || ws_log > 15 || wd_log > 16)
+    if (S > 65536u || B > 65536u || ws_log > WG_PB_MAX_WS_LOG || wd_log > 16)
         return WG_E_RANGE;
     hipStream_t st = ctx->stream;
     pb->ws_log = ws_log;
@@ -334,8 +386,8 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     pb->ntiles = (uint32_t)(nq / WG_PB_TILE);
     pb->ntasks = (uint32_t)task.size();
 
-    // 3. codes, pair numbers, pair destinations and (window, bin) keys
-    pb_tmp twin, pqd, pidx, pkey;
+    // 3. codes, pair numbers (window order), pair destinations and (window, bin) keys
+    pb_tmp twin, pqd, pidx, pkey, pdst_w, bflag, bst, nsel, bk, boff;
     uint32_t npairs = 0;
     int rc;
     if (nq) {
@@ -364,6 +416,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     }
     pb->npairs = npairs;
     PB_TRY(hipMalloc((void**)&pb->tile_pbase, ((uint64_t)pb->ntiles + 1) * sizeof(uint32_t)));
+    PB_TRY(hipMalloc((void**)&pb->tile_bcnt, ((uint64_t)pb->ntiles + 1) * sizeof(uint32_t)));
     PB_TRY(hipMalloc((void**)&pb->pdst, (npairs ? npairs : 1) * sizeof(uint16_t)));
     PB_TRY(hipMalloc((void**)&pb->part, (npairs ? npairs : 1) * sizeof(double)));
     std::vector<uint32_t> bstart, bkey;
@@ -371,24 +424,28 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         hipLaunchKernelGGL(k_pb_tile_base, dim3(pb->ntiles / 256 + 1), dim3(256), 0, st, (const uint32_t*)pidx.p,
                            pb->ntiles, npairs, pb->tile_pbase);
         PB_TRY(hipMalloc(&pkey.p, (npairs ? npairs : 1) * sizeof(uint32_t)));
+        PB_TRY(hipMalloc(&pdst_w.p, (npairs ? npairs : 1) * sizeof(uint16_t)));
         const uint32_t gq = wg_grid(ctx, nq, 1024, 8);
         hipLaunchKernelGGL(k_pb_pairs, dim3(gq), dim3(256), 0, st, nq, pb->code, (const uint32_t*)pidx.p,
                            (const uint32_t*)twin.p, (const uint64_t*)wsd.p, (const uint64_t*)pqd.p,
-                           (const uint32_t*)perm.p, ctx->dst, wd_log, pb->pdst, (uint32_t*)pkey.p);
+                           (const uint32_t*)perm.p, ctx->dst, wd_log, (uint16_t*)pdst_w.p, (uint32_t*)pkey.p);
         PB_TRY(hipGetLastError());
-        (void)hipFree(pidx.p);
-        pidx.p = nullptr;
         (void)hipFree(perm.p);
         perm.p = nullptr;
 
-        // 4. blocks: runs of equal (window, bin) among the pairs
-        pb_tmp bflag, bst, nsel, bk;
+        // 4. blocks: runs of equal (window, bin) among the pairs; their first
+        //    pairs are flagged in the codes
         PB_TRY(hipMalloc(&bflag.p, npairs));
         PB_TRY(hipMalloc(&bst.p, (uint64_t)npairs * 4));
         PB_TRY(hipMalloc(&nsel.p, sizeof(uint64_t)));
         hipLaunchKernelGGL(k_pb_block_flags, dim3(wg_grid(ctx, npairs, 1024, 8)), dim3(256), 0, st,
                            (const uint32_t*)pkey.p, (uint64_t)npairs, (uint8_t*)bflag.p);
         PB_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_pb_mark_blocks, dim3(gq), dim3(256), 0, st, nq, pb->code, (const uint32_t*)pidx.p,
+                           (const uint8_t*)bflag.p);
+        PB_TRY(hipGetLastError());
+        (void)hipFree(pidx.p);
+        pidx.p = nullptr;
         hipcub::CountingInputIterator<uint32_t> pos(0u);
         size_t tb = 0;
         PB_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb, pos, (const uint8_t*)bflag.p, (uint32_t*)bst.p,
@@ -399,9 +456,14 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         uint64_t nb = 0;
         PB_TRY(hipMemcpyAsync(&nb, nsel.p, sizeof(nb), hipMemcpyDeviceToHost, st));
         PB_TRY(hipStreamSynchronize(st));
+        (void)hipFree(tmp.p);
+        tmp.p = nullptr;
         PB_TRY(hipMalloc(&bk.p, (nb ? nb : 1) * 4));
         hipLaunchKernelGGL(k_pb_block_keys, dim3((uint32_t)(nb / 256 + 1)), dim3(256), 0, st, (const uint32_t*)bst.p,
                            (uint32_t)nb, (const uint32_t*)pkey.p, (uint32_t*)bk.p);
+        PB_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_pb_tile_blocks, dim3(pb->ntiles / 256 + 1), dim3(256), 0, st, pb->tile_pbase, pb->ntiles,
+                           (const uint32_t*)bst.p, (uint32_t)nb, pb->tile_bcnt);
         PB_TRY(hipGetLastError());
         bstart.resize(nb);
         bkey.resize(nb);
@@ -412,8 +474,10 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         PB_TRY(hipStreamSynchronize(st));
     }
 
-    // 5. pieces grouped by bin (window order kept), slices and slots (host)
+    // 5. bin order of the blocks (window order kept within a bin), their
+    //    offsets, slices and accumulator slots (host)
     const uint64_t nb = bstart.size();
+    pb->nblocks = (uint32_t)nb;
     std::vector<uint64_t> bin_first(B + 1, 0);
     for (uint64_t i = 0; i < nb; ++i)
         bin_first[(bkey[i] & 0xffffu) + 1]++;
@@ -425,47 +489,51 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         for (uint64_t i = 0; i < nb; ++i) // blocks are in (window, bin) order: stable by bin
             by_bin[fill[bkey[i] & 0xffffu]++] = (uint32_t)i;
     }
+    std::vector<uint32_t> blk_off(nb ? nb : 1, 0u);
+    std::vector<uint32_t> bin_out(B + 1, 0u); // first position of each bin in part[]
     const uint32_t slice_max = ctx->pb_slice ? ctx->pb_slice : WG_PB_SLICE;
-    const uint32_t piece_max = std::min<uint32_t>(WG_PB_PIECE, slice_max);
-    std::vector<uint2> piece;
     std::vector<uint4> slice;
     std::vector<uint2> bin_slot(B, make_uint2(0u, 0u));
     uint32_t nslots = 0;
-    for (uint32_t b = 0; b < B; ++b) {
-        const size_t first_slice = slice.size();
-        uint4 cur = make_uint4(b, (uint32_t)piece.size(), (uint32_t)piece.size(), WG_PB_NOSLOT);
-        uint64_t in_slice = 0;
-        for (uint64_t k = bin_first[b]; k < bin_first[b + 1]; ++k) {
-            const uint32_t i = by_bin[k];
-            const uint32_t p0 = bstart[i];
-            const uint32_t p1 = (i + 1 < nb) ? bstart[i + 1] : npairs;
-            for (uint32_t p = p0; p < p1; p += piece_max) {
-                const uint32_t n = std::min<uint32_t>(piece_max, p1 - p);
-                if (in_slice && in_slice + n > slice_max) {
-                    slice.push_back(cur);
-                    cur.y = cur.z;
-                    in_slice = 0;
-                }
-                piece.push_back(make_uint2(p, n));
-                cur.z = (uint32_t)piece.size();
-                in_slice += n;
+    {
+        uint32_t o = 0;
+        for (uint32_t b = 0; b < B; ++b) {
+            bin_out[b] = o;
+            for (uint64_t k = bin_first[b]; k < bin_first[b + 1]; ++k) {
+                const uint32_t i = by_bin[k];
+                const uint32_t p0 = bstart[i];
+                const uint32_t p1 = (i + 1 < nb) ? bstart[i + 1] : npairs;
+                blk_off[i] = o - p0; // wraps around: p + blk_off[i] = o + (p - p0)
+                o += p1 - p0;
+            }
+            const uint32_t end = o;
+            const size_t first_slice = slice.size();
+            uint32_t q = bin_out[b];
+            do { // every bin has a slice, even without pairs: its vertex phase
+                const uint32_t q1 = (end - q > slice_max) ? q + slice_max : end;
+                slice.push_back(make_uint4(b, q, q1, WG_PB_NOSLOT));
+                q = q1;
+            } while (q < end);
+            const uint32_t nsl = (uint32_t)(slice.size() - first_slice);
+            if (nsl > 1) {
+                bin_slot[b] = make_uint2(nslots, nsl);
+                for (uint32_t j = 0; j < nsl; ++j)
+                    slice[first_slice + j].w = nslots + j;
+                nslots += nsl;
             }
         }
-        slice.push_back(cur); // every bin has a slice, even without pairs: its vertex phase
-        const uint32_t nsl = (uint32_t)(slice.size() - first_slice);
-        if (nsl > 1) {
-            bin_slot[b] = make_uint2(nslots, nsl);
-            for (uint32_t j = 0; j < nsl; ++j)
-                slice[first_slice + j].w = nslots + j;
-            nslots += nsl;
-        }
+        bin_out[B] = o;
     }
-    pb->npieces = (uint32_t)piece.size();
     pb->nslices = (uint32_t)slice.size();
     pb->nslots = nslots;
-    if ((rc = pb_upload(ctx, &pb->task, task)) || (rc = pb_upload(ctx, &pb->piece, piece)) ||
-        (rc = pb_upload(ctx, &pb->slice, slice)) || (rc = pb_upload(ctx, &pb->bin_slot, bin_slot)))
+    if ((rc = pb_upload(ctx, &pb->task, task)) || (rc = pb_upload(ctx, &pb->slice, slice)) ||
+        (rc = pb_upload(ctx, &pb->bin_slot, bin_slot)) || (rc = pb_upload(ctx, &pb->blk_off, blk_off)))
         return rc;
+    if (npairs) // destinations in bin order
+        hipLaunchKernelGGL(k_pb_place, dim3(wg_grid(ctx, npairs, 1024, 8)), dim3(256), 0, st,
+                           (const uint16_t*)pdst_w.p, npairs, (const uint32_t*)bst.p, (uint32_t)nb, pb->blk_off,
+                           pb->pdst);
+    PB_TRY(hipGetLastError());
     PB_TRY(hipMalloc((void**)&pb->slot_acc, ((uint64_t)(nslots ? nslots : 1) << wd_log) * sizeof(double)));
     PB_TRY(hipMalloc((void**)&pb->bin_ticket, B * sizeof(unsigned int)));
     PB_TRY(hipMemsetAsync(pb->bin_ticket, 0, B * sizeof(unsigned int), st));
@@ -474,20 +542,19 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     return WG_OK;
 }
 
-// Window and bin sizes of a binned shape: one 1024-thread workgroup per CU
-// holds 128 KiB of window and of accumulators (blocks_per_cu 1), or two hold
-// 64 KiB each (blocks_per_cu 2).
-static void pb_sizes(int bpc, size_t esz, int* ws_log, int* wd_log)
+// Window and bin sizes of a binned shape: 16K-vertex windows (at most what the
+// 14-bit source field holds) and 16K-vertex bins, 128 KiB of FP64 accumulators,
+// for one 1024-thread workgroup per CU (blocks_per_cu 1); 8K-vertex windows
+// and bins, 64 KiB each, for two (blocks_per_cu 2).  FP32 windows of 16K
+// vertices take 64 KiB.
+static void pb_sizes(int bpc, int* ws_log, int* wd_log)
 {
-    const int kib_log = bpc == 1 ? 17 : 16;
-    *ws_log = kib_log - (esz == 8 ? 3 : 2);
-    *wd_log = kib_log - 3;
+    *ws_log = bpc == 1 ? 14 : 13;
+    *wd_log = bpc == 1 ? 14 : 13;
 }
 
-int wg_pb_prepare(wg_ctx* ctx)
+static int pb_prepare_sizes(wg_ctx* ctx, int ws_log, int wd_log)
 {
-    int ws_log, wd_log;
-    pb_sizes(ctx->pr_bpc, ctx->pr_fp32 ? 4 : 8, &ws_log, &wd_log);
     if (ctx->pb && ctx->pb->ws_log == ws_log && ctx->pb->wd_log == wd_log)
         return WG_OK;
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -497,6 +564,13 @@ int wg_pb_prepare(wg_ctx* ctx)
     if (rc)
         wg_free_pb(ctx);
     return rc;
+}
+
+int wg_pb_prepare(wg_ctx* ctx)
+{
+    int ws_log, wd_log;
+    pb_sizes(ctx->pr_bpc, &ws_log, &wd_log);
+    return pb_prepare_sizes(ctx, ws_log, wd_log);
 }
 
 // ---- iteration -------------------------------------------------------------------
@@ -516,44 +590,157 @@ static __device__ __forceinline__ double block_sum_pb(double v, double* s_red)
     return r;
 }
 
-template <typename T, int IPT>
-struct pb_codes;
-
-template <typename T>
-struct pb_codes<T, 8> {
-    static __device__ __forceinline__ void load(const uint16_t* p, uint32_t* c)
-    {
-        const uint4 r = *reinterpret_cast<const uint4*>(p);
-        c[0] = r.x & 0xffffu, c[1] = r.x >> 16, c[2] = r.y & 0xffffu, c[3] = r.y >> 16;
-        c[4] = r.z & 0xffffu, c[5] = r.z >> 16, c[6] = r.w & 0xffffu, c[7] = r.w >> 16;
-    }
-};
-
-template <typename T>
-struct pb_codes<T, 4> {
-    static __device__ __forceinline__ void load(const uint16_t* p, uint32_t* c)
-    {
-        const uint2 r = *reinterpret_cast<const uint2*>(p);
-        c[0] = r.x & 0xffffu, c[1] = r.x >> 16, c[2] = r.y & 0xffffu, c[3] = r.y >> 16;
-    }
-};
-
-// Push: one workgroup per task (a window's run of tiles), one wavefront per
-// tile.  Lane l takes the tile's entries [l * IPT, (l + 1) * IPT): it sums its
-// entries in order, and each entry flagged as a pair's last emits that pair's
-// partial; the sum still open at the lane's end flows into the next lanes by
-// a segmented inclusive scan (a lane with an end starts a new segment).
-template <typename T, int IPT, int WS>
-__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __restrict__ code, const T* __restrict__ cin,
-                                                           const uint4* __restrict__ task, uint32_t ntasks,
-                                                           const uint32_t* __restrict__ tile_pbase,
-                                                           double* __restrict__ part, uint32_t V)
+// Codes of the lane's WG_PB_IPT entries of a tile (one 16-byte load).
+#define WG_PB_IPT 8
+__device__ __forceinline__ uint4 pb_load_codes(const uint16_t* __restrict__ code, uint32_t t, int lane)
 {
-    static_assert(WG_WAVE * IPT == WG_PB_TILE, "a tile is one wavefront's entries");
-    __shared__ T s_win[WS];
+    return *reinterpret_cast<const uint4*>(code + (uint64_t)t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT);
+}
+
+// What the push combines: PageRank sums FP64 contributions, the 64-source BFS
+// ORs 64-bit frontier masks.
+struct pb_sum {
+    typedef double V;
+    static __device__ __forceinline__ V zero() { return 0.0; }
+    static __device__ __forceinline__ V op(V a, V b) { return a + b; }
+    static __device__ __forceinline__ V shfl_up(V x, int d) { return __shfl_up(x, d, WG_WAVE); }
+};
+struct pb_or {
+    typedef unsigned long long V;
+    static __device__ __forceinline__ V zero() { return 0ull; }
+    static __device__ __forceinline__ V op(V a, V b) { return a | b; }
+    static __device__ __forceinline__ V shfl_up(V x, int d)
+    {
+        return (unsigned long long)__shfl_up((long long)x, d, WG_WAVE);
+    }
+};
+
+// A tile's inputs, loaded one round ahead: the lane's codes, the tile's first
+// pair number and blocks started before it, and the offsets of the 64 blocks
+// from the one holding the tile's first pair (lane l: block bcnt - 1 + l).
+struct pb_tile_in {
+    uint4 codes;
+    uint32_t pbase, bcnt, boff;
+};
+
+__device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ code,
+                                                   const uint32_t* __restrict__ tile_pbase,
+                                                   const uint32_t* __restrict__ tile_bcnt,
+                                                   const uint32_t* __restrict__ blk_off, uint32_t nblocks, uint32_t t,
+                                                   int lane)
+{
+    pb_tile_in in;
+    in.codes = pb_load_codes(code, t, lane);
+    in.pbase = tile_pbase[t];
+    in.bcnt = tile_bcnt[t];
+    const uint32_t b = in.bcnt + (uint32_t)lane - 1u; // wraps to 2^32 - 1 for lane 0 of the first tile
+    in.boff = b < nblocks ? blk_off[b] : 0u;
+    return in;
+}
+
+// Push body: one workgroup per task (a window's run of tiles), one wavefront
+// per tile, each tile's inputs loaded two rounds ahead.  s_win holds the
+// window (filled by the caller, converted by LOAD on read).  Lane l takes the
+// tile's entries [8l, 8l + 8): it combines its entries in order, and each
+// entry flagged as a pair's last emits that pair's value; the value still open
+// at the lane's end flows into the next lanes by a segmented inclusive scan (a
+// lane with an end starts a new segment).  A value goes to its position in bin
+// order, p + blk_off[block of p].
+template <class OP, typename W, class LOAD>
+__device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, const uint4 tk,
+                                              const uint32_t* __restrict__ tile_pbase,
+                                              const uint32_t* __restrict__ tile_bcnt,
+                                              const uint32_t* __restrict__ blk_off, uint32_t nblocks,
+                                              typename OP::V* __restrict__ part, LOAD load)
+{
+    typedef typename OP::V V;
     const int lane = wg_lane();
     const int wave = threadIdx.x / WG_WAVE;
     const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t t = tk.y + (uint32_t)wave;
+    pb_tile_in n1 = {}, n2 = {};
+    if (t < tk.z)
+        n1 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t, lane);
+    if (t + WG_PB_WAVES < tk.z)
+        n2 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t + WG_PB_WAVES, lane);
+    for (; t < tk.z; t += WG_PB_WAVES) {
+        const pb_tile_in in = n1;
+        n1 = n2;
+        if (t + 2 * WG_PB_WAVES < tk.z) // two rounds ahead
+            n2 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t + 2 * WG_PB_WAVES, lane);
+        const uint4 raw = in.codes;
+        const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
+                                       raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
+        const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT;
+        V v[WG_PB_IPT];
+        bool e[WG_PB_IPT], bs[WG_PB_IPT];
+        V run = OP::zero();
+        bool has_end = false;
+#pragma unroll
+        for (int j = 0; j < WG_PB_IPT; ++j) {
+            const bool real = q0 + (uint32_t)j < tk.w;
+            v[j] = real ? load(s_win[c[j] & WG_PB_SRC]) : OP::zero();
+            e[j] = real && (c[j] & WG_PB_END);
+            bs[j] = real && (c[j] & WG_PB_BSTART);
+            run = OP::op(run, v[j]);
+            if (e[j]) {
+                run = OP::zero();
+                has_end = true;
+            }
+        }
+        // segmented inclusive scan of the open values over the lanes
+        V sc = run;
+        int f = has_end ? 1 : 0;
+#pragma unroll
+        for (int d = 1; d < WG_WAVE; d <<= 1) {
+            const V o = OP::shfl_up(sc, d);
+            const int of = __shfl_up(f, d, WG_WAVE);
+            if (lane >= d) {
+                if (!f)
+                    sc = OP::op(o, sc);
+                f |= of;
+            }
+        }
+        V carry = OP::shfl_up(sc, 1);
+        if (lane == 0)
+            carry = OP::zero();
+        uint32_t pre = 0, bpre = 0;
+#pragma unroll
+        for (int j = 0; j < WG_PB_IPT; ++j) {
+            pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
+            bpre += (uint32_t)__popcll(wg_ballot(bs[j]) & below);
+        }
+        uint32_t p = in.pbase + pre;      // window-order number of the lane's next pair
+        uint32_t k = bpre;                // its block - (in.bcnt - 1), before its own start flag
+        V r = carry;
+#pragma unroll
+        for (int j = 0; j < WG_PB_IPT; ++j) {
+            r = OP::op(r, v[j]);
+            k += bs[j] ? 1u : 0u;
+            // the block's offset from the lane that loaded it (every lane takes part)
+            const uint32_t from_lane = (uint32_t)__shfl((int)in.boff, (int)(k & (WG_WAVE - 1)), WG_WAVE);
+            if (e[j]) {
+                const uint32_t off = k < WG_WAVE ? from_lane : blk_off[in.bcnt - 1u + k];
+                part[p + off] = r;
+                ++p;
+                r = OP::zero();
+            }
+        }
+    }
+}
+
+// PageRank push: the window holds contributions (FP64, or FP32 stored).
+template <typename T, int WS>
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __restrict__ code, const T* __restrict__ cin,
+                                                           const uint4* __restrict__ task, uint32_t ntasks,
+                                                           const uint32_t* __restrict__ tile_pbase,
+                                                           const uint32_t* __restrict__ tile_bcnt,
+                                                           const uint32_t* __restrict__ blk_off, uint32_t nblocks,
+                                                           double* __restrict__ part, uint32_t V)
+{
+    static_assert(WG_WAVE * WG_PB_IPT == WG_PB_TILE, "a tile is one wavefront's entries");
+    static_assert(WS <= WG_PB_SRC + 1, "window larger than the source field");
+    __shared__ T s_win[WS];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
@@ -562,79 +749,116 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_win[i] = cin[v0 + i];
         __syncthreads();
-        for (uint32_t t = tk.y + (uint32_t)wave; t < tk.z; t += WG_PB_WAVES) {
-            const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * IPT;
-            uint32_t c[IPT];
-            pb_codes<T, IPT>::load(code + q0, c);
-            double v[IPT];
-            bool e[IPT];
-            double run = 0.0;
-            bool has_end = false;
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const bool real = q0 + (uint32_t)j < tk.w;
-                v[j] = real ? (double)s_win[c[j] & 0x7fffu] : 0.0;
-                e[j] = real && (c[j] & WG_PB_END);
-                run += v[j];
-                if (e[j]) {
-                    run = 0.0;
-                    has_end = true;
-                }
-            }
-            // segmented inclusive scan of the open sums over the lanes
-            double sc = run;
-            int f = has_end ? 1 : 0;
-#pragma unroll
-            for (int d = 1; d < WG_WAVE; d <<= 1) {
-                const double o = __shfl_up(sc, d, WG_WAVE);
-                const int of = __shfl_up(f, d, WG_WAVE);
-                if (lane >= d) {
-                    if (!f)
-                        sc = o + sc;
-                    f |= of;
-                }
-            }
-            double carry = __shfl_up(sc, 1, WG_WAVE);
-            if (lane == 0)
-                carry = 0.0;
-            uint32_t pre = 0;
-#pragma unroll
-            for (int j = 0; j < IPT; ++j)
-                pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
-            double* out = part + tile_pbase[t] + pre;
-            double r = carry;
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                r += v[j];
-                if (e[j]) {
-                    *out++ = r;
-                    r = 0.0;
-                }
-            }
-        }
+        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
+                              [](T x) { return (double)x; });
     }
 }
 
-// Pull: one workgroup per slice.  Adds the slice's partials into the bin's
-// accumulators in LDS; a bin cut into several slices has each slice write its
-// accumulators to its slot, and the last slice to finish adds the slots in
-// slice order.  The workgroup holding a bin's complete accumulators then runs
-// the vertex phase (MODE 0: contributions and the sum of ranks of vertices
-// with out-degree > 0; MODE 1: accumulators and their sum), and the last bin
-// to finish reduces the bins' sums into S_out.
+// The pull's stream over one slice, part[b0, b1): ADD(destination, partial)
+// for every pair.  Four pairs per load (8 bytes of destinations, 32 of
+// partials) and WG_PB_PULL_ROUNDS loads in flight per thread; the unaligned
+// ends of the range take one pair per thread.
+#define WG_PB_PULL_ROUNDS 4
+typedef double pb_f64x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long pb_u64x4 __attribute__((ext_vector_type(4)));
+
+template <typename X4, typename X, class ADD>
+__device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
+                                              uint32_t b0, uint32_t b1, ADD add)
+{
+    const uint32_t t = threadIdx.x;
+    uint32_t a0 = (b0 + 3u) & ~3u; // the aligned body [a0, a1)
+    if (a0 > b1)
+        a0 = b1;
+    uint32_t a1 = b1 & ~3u;
+    if (a1 < a0)
+        a1 = a0;
+    if (t < a0 - b0)
+        add(pdst[b0 + t], part[b0 + t]);
+    if (t < b1 - a1)
+        add(pdst[a1 + t], part[a1 + t]);
+    const uint32_t g1 = a1 >> 2;
+    const unsigned long long* __restrict__ pd4 = reinterpret_cast<const unsigned long long*>(pdst);
+    const X4* __restrict__ pp4 = reinterpret_cast<const X4*>(part);
+    for (uint32_t g = (a0 >> 2) + t; g < g1; g += WG_PB_THREADS * WG_PB_PULL_ROUNDS) {
+        unsigned long long dd[WG_PB_PULL_ROUNDS];
+        X4 xx[WG_PB_PULL_ROUNDS];
+#pragma unroll
+        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r) {
+            const uint32_t gi = g + (uint32_t)r * WG_PB_THREADS;
+            dd[r] = 0ull;
+            xx[r] = X4(0);
+            if (gi < g1) {
+                dd[r] = __builtin_nontemporal_load(pd4 + gi);
+                xx[r] = __builtin_nontemporal_load(pp4 + gi);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r)
+            if (g + (uint32_t)r * WG_PB_THREADS < g1) {
+                add((uint32_t)(dd[r] & 0xffffu), xx[r].x);
+                add((uint32_t)((dd[r] >> 16) & 0xffffu), xx[r].y);
+                add((uint32_t)((dd[r] >> 32) & 0xffffu), xx[r].z);
+                add((uint32_t)(dd[r] >> 48), xx[r].w);
+            }
+    }
+}
+
+// A bin cut into several slices: each slice stores its LDS words in its slot,
+// and the last slice of the bin to finish (a per-bin ticket) combines the
+// slots in slice order into its LDS words.  Returns false in the other slices.
+// One thread releases the workgroup's stores and acquires the other slices'
+// (device-scope fences after the workgroup barrier).
+template <typename X, class COMBINE>
+__device__ __forceinline__ bool pb_join_slices(X* s_acc, uint32_t nv, uint32_t b, uint32_t slot,
+                                               const uint2* __restrict__ bin_slot, X* __restrict__ slot_acc,
+                                               unsigned int* __restrict__ bin_ticket, uint32_t wd, int* s_flag,
+                                               COMBINE combine)
+{
+    const uint2 bs = bin_slot[b];
+    X* mine = slot_acc + (uint64_t)slot * wd;
+    for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
+        mine[i] = s_acc[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const bool last = atomicAdd(&bin_ticket[b], 1u) == bs.y - 1u;
+        if (last) {
+            __threadfence();
+            bin_ticket[b] = 0u; // for the next iteration (the kernel boundary orders it)
+        }
+        *s_flag = last ? 1 : 0;
+    }
+    __syncthreads();
+    if (!*s_flag)
+        return false;
+    for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
+        X a = slot_acc[(uint64_t)bs.x * wd + i];
+        for (uint32_t j = 1; j < bs.y; ++j)
+            a = combine(a, slot_acc[(uint64_t)(bs.x + j) * wd + i]);
+        s_acc[i] = a;
+    }
+    return true;
+}
+
+// Pull: one workgroup per slice, a contiguous range of part[] within one bin.
+// Adds the slice's partials into the bin's accumulators in LDS (ds_add_f64);
+// a bin cut into several slices joins them (pb_join_slices).  The workgroup
+// holding a bin's complete accumulators then runs the vertex phase (MODE 0:
+// contributions and the sum of ranks of vertices with out-degree > 0; MODE 1:
+// accumulators and their sum) and stores the bin's sum in bin_part[bin];
+// k_pb_total adds those.
 template <typename T, int MODE, int WD>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
-    const double* __restrict__ part, const uint16_t* __restrict__ pdst, const uint2* __restrict__ piece,
-    const uint4* __restrict__ slice, uint32_t nslices, const uint2* __restrict__ bin_slot, double* __restrict__ slot_acc,
-    unsigned int* __restrict__ bin_ticket, double* __restrict__ bin_part, uint32_t nbins, T* __restrict__ cout,
+    const double* __restrict__ part, const uint16_t* __restrict__ pdst, const uint4* __restrict__ slice,
+    uint32_t nslices, const uint2* __restrict__ bin_slot, double* __restrict__ slot_acc,
+    unsigned int* __restrict__ bin_ticket, double* __restrict__ bin_part, T* __restrict__ cout,
     double* __restrict__ acc_out, const uint32_t* __restrict__ od, uint32_t V, const double* __restrict__ S_in,
-    double damping, double N, double* __restrict__ S_out, unsigned int* __restrict__ ticket)
+    double damping, double N)
 {
     __shared__ double s_acc[WD];
     __shared__ double s_red[WG_PB_WAVES];
     __shared__ int s_flag;
-    const int lane = wg_lane();
-    const int wave = threadIdx.x / WG_WAVE;
     double corr = 0.0, omd_n = 0.0;
     if (MODE == 0) {
         corr = (1.0 - *S_in) / N;
@@ -648,34 +872,12 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
         __syncthreads();
-        for (uint32_t pc = sl.y + (uint32_t)wave; pc < sl.z; pc += WG_PB_WAVES) {
-            const uint2 pz = piece[pc];
-            for (uint32_t i = (uint32_t)lane; i < pz.y; i += WG_WAVE)
-                atomicAdd(&s_acc[pdst[pz.x + i]], __builtin_nontemporal_load(&part[pz.x + i]));
-        }
+        pb_pull_range<pb_f64x4>(part, pdst, sl.y, sl.z, [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
         __syncthreads();
-        if (sl.w != WG_PB_NOSLOT) {
-            const uint2 bs = bin_slot[b];
-            double* mine = slot_acc + ((uint64_t)sl.w * (uint32_t)WD);
-            for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
-                mine[i] = s_acc[i];
-            __threadfence();
-            __syncthreads();
-            if (threadIdx.x == 0)
-                s_flag = (atomicAdd(&bin_ticket[b], 1u) == bs.y - 1u) ? 1 : 0;
-            __syncthreads();
-            if (!s_flag)
-                continue;
-            __threadfence();
-            for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
-                double a = 0.0;
-                for (uint32_t j = 0; j < bs.y; ++j)
-                    a += slot_acc[((uint64_t)(bs.x + j) * (uint32_t)WD) + i];
-                s_acc[i] = a;
-            }
-            if (threadIdx.x == 0)
-                bin_ticket[b] = 0u;
-        }
+        if (sl.w != WG_PB_NOSLOT &&
+            !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
+                            [](double a, double x) { return a + x; }))
+            continue;
         double s_part = 0.0;
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
             const uint32_t v = v0 + i;
@@ -694,24 +896,22 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
             }
         }
         s_part = block_sum_pb(s_part, s_red);
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0)
             bin_part[b] = s_part;
-            __threadfence();
-            s_flag = (atomicAdd(ticket, 1u) == nbins - 1u) ? 1 : 0;
-        }
-        __syncthreads();
-        if (s_flag) {
-            __threadfence();
-            double s = 0.0;
-            for (uint32_t i = threadIdx.x; i < nbins; i += WG_PB_THREADS)
-                s += __builtin_nontemporal_load(&bin_part[i]);
-            s = block_sum_pb(s, s_red);
-            if (threadIdx.x == 0) {
-                *S_out = s;
-                *ticket = 0u;
-            }
-        }
     }
+}
+
+// S_out = the bins' sums, added in a fixed order (one workgroup).
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_total(const double* __restrict__ bin_part, uint32_t nbins,
+                                                            double* __restrict__ S_out)
+{
+    __shared__ double s_red[WG_PB_WAVES];
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < nbins; i += WG_PB_THREADS)
+        s += bin_part[i];
+    s = block_sum_pb(s, s_red);
+    if (threadIdx.x == 0)
+        *S_out = s;
 }
 
 // ---- host side --------------------------------------------------------------------
@@ -724,8 +924,8 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     const uint32_t V = (uint32_t)ctx->V;
     if (pb->ntasks) {
         const int slot = wg_prof_begin(ctx, WG_K_PB_PUSH);
-        hipLaunchKernelGGL((k_pb_push<T, 8, WS>), dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, cin,
-                           pb->task, pb->ntasks, pb->tile_pbase, pb->part, V);
+        hipLaunchKernelGGL((k_pb_push<T, WS>), dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, cin,
+                           pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, pb->part, V);
         int rc = wg_check_launch(ctx);
         if (rc)
             return rc;
@@ -734,16 +934,17 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     const int slot = wg_prof_begin(ctx, WG_K_PB_PULL);
     if (mode == 0)
         hipLaunchKernelGGL((k_pb_pull<T, 0, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
-                           pb->pdst, pb->piece, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket,
-                           pb->bin_part, pb->B, cout, nullptr, ctx->od, V, S_in, ctx->damping, N, S_out,
-                           &ctx->ctr->ticket);
+                           pb->pdst, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
+                           cout, nullptr, ctx->od, V, S_in, ctx->damping, N);
     else
         hipLaunchKernelGGL((k_pb_pull<T, 1, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
-                           pb->pdst, pb->piece, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket,
-                           pb->bin_part, pb->B, nullptr, ctx->acc, ctx->od, V, S_in, ctx->damping, N, S_out,
-                           &ctx->ctr->ticket);
-    const int rc = wg_check_launch(ctx);
+                           pb->pdst, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
+                           nullptr, ctx->acc, ctx->od, V, S_in, ctx->damping, N);
+    int rc = wg_check_launch(ctx);
     if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_pb_total, dim3(1), dim3(WG_PB_THREADS), 0, ctx->stream, pb->bin_part, pb->B, S_out);
+    if ((rc = wg_check_launch(ctx)))
         return rc;
     wg_prof_end(ctx, WG_K_PB_PULL, slot);
     return WG_OK;
@@ -756,8 +957,8 @@ int wg_pb_iteration(wg_ctx* ctx, int mode, const void* cin, void* cout, double* 
         return rc;
     const bool big = ctx->pr_bpc == 1;
     if (ctx->pr_fp32)
-        return big ? pb_launch<float, 32768, 16384>(ctx, mode, (const float*)cin, (float*)cout, S_in, S_out)
-                   : pb_launch<float, 16384, 8192>(ctx, mode, (const float*)cin, (float*)cout, S_in, S_out);
+        return big ? pb_launch<float, 16384, 16384>(ctx, mode, (const float*)cin, (float*)cout, S_in, S_out)
+                   : pb_launch<float, 8192, 8192>(ctx, mode, (const float*)cin, (float*)cout, S_in, S_out);
     return big ? pb_launch<double, 16384, 16384>(ctx, mode, (const double*)cin, (double*)cout, S_in, S_out)
                : pb_launch<double, 8192, 8192>(ctx, mode, (const double*)cin, (double*)cout, S_in, S_out);
 }
@@ -771,4 +972,98 @@ int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint
     *tasks = ctx->pb->ntasks;
     *slices = ctx->pb->nslices;
     return WG_OK;
+}
+
+// ---- 64-source BFS: a dense level on the binned layout ------------------------------
+//
+// The dense level of the batched BFS (wg_batch.hip) ORs, for every vertex, the
+// frontier masks of its in-neighbours (64 sources, one bit each).  OR is
+// associative, commutative and idempotent, so the same two passes apply, with
+// results independent of the order: the push ORs each (window, destination)
+// pair's frontier masks, the pull ORs the pairs into the bin's LDS words
+// (ds_or_b64) and keeps the sources each vertex still needs.
+
+// Push: as k_pb_push, with 64-bit masks (front & live) in place of
+// contributions and OR in place of the sum.
+template <int WS>
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __restrict__ code,
+                                                              const unsigned long long* __restrict__ front,
+                                                              unsigned long long live, const uint4* __restrict__ task,
+                                                              uint32_t ntasks, const uint32_t* __restrict__ tile_pbase,
+                                                              const uint32_t* __restrict__ tile_bcnt,
+                                                              const uint32_t* __restrict__ blk_off, uint32_t nblocks,
+                                                              unsigned long long* __restrict__ part, uint32_t V)
+{
+    __shared__ unsigned long long s_win[WS];
+    for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+        const uint4 tk = task[k];
+        const uint32_t v0 = tk.x * (uint32_t)WS;
+        const uint32_t nv = (V - v0 < (uint32_t)WS) ? V - v0 : (uint32_t)WS;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
+            s_win[i] = front[v0 + i] & live;
+        __syncthreads();
+        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
+                             [](unsigned long long x) { return x; });
+    }
+}
+
+// Pull: ORs a slice's masks into the bin's LDS words; the workgroup holding a
+// bin's complete words keeps, per vertex, the live sources not seen yet and
+// stores them (nonzero only) in nl.
+template <int WD>
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
+    const unsigned long long* __restrict__ part, const uint16_t* __restrict__ pdst, const uint4* __restrict__ slice,
+    uint32_t nslices, const uint2* __restrict__ bin_slot, unsigned long long* __restrict__ slot_acc,
+    unsigned int* __restrict__ bin_ticket, const unsigned long long* __restrict__ seen, unsigned long long live,
+    unsigned long long* __restrict__ nl, uint32_t V)
+{
+    __shared__ unsigned long long s_acc[WD];
+    __shared__ int s_flag;
+    for (uint32_t k = blockIdx.x; k < nslices; k += gridDim.x) {
+        const uint4 sl = slice[k];
+        const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
+        const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
+            s_acc[i] = 0ull;
+        __syncthreads();
+        pb_pull_range<pb_u64x4>(part, pdst, sl.y, sl.z, [&](uint32_t d, unsigned long long x) {
+            if (x)
+                atomicOr(&s_acc[d], x);
+        });
+        __syncthreads();
+        if (sl.w != WG_PB_NOSLOT &&
+            !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
+                            [](unsigned long long a, unsigned long long x) { return a | x; }))
+            continue;
+        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
+            const unsigned long long a = s_acc[i];
+            if (a) {
+                const unsigned long long n = a & live & ~seen[v0 + i];
+                if (n)
+                    nl[v0 + i] = n;
+            }
+        }
+    }
+}
+
+int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsigned long long* seen,
+                      unsigned long long* nl, unsigned long long live)
+{
+    int rc = pb_prepare_sizes(ctx, 14, 14); // 16K 8-byte masks per window, 16K words per bin
+    if (rc)
+        return rc;
+    const wg_pb* pb = ctx->pb;
+    const uint32_t V = (uint32_t)ctx->V;
+    unsigned long long* part = reinterpret_cast<unsigned long long*>(pb->part);
+    if (pb->ntasks)
+        hipLaunchKernelGGL(k_pb_push_or<16384>, dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, front,
+                           live, pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, part, V);
+    if ((rc = wg_check_launch(ctx)))
+        return rc;
+    hipLaunchKernelGGL(k_pb_pull_or<16384>, dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
+                       (const unsigned long long*)part, pb->pdst, pb->slice, pb->nslices, pb->bin_slot,
+                       reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, live, nl, V);
+    return wg_check_launch(ctx);
 }

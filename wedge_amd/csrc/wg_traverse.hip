@@ -132,6 +132,7 @@ __global__ void k_csr_dst(const uint32_t* __restrict__ csri_pos, const uint32_t*
 // finished by the whole wave, 64 edges per step, stopping at the first hit.
 // Vertices with more than WG_BFS_HEAVY in-edges are left to k_bfs_heavy.
 // s_cur: the frontier's first words in LDS (stage_cur).
+template <bool CW>
 __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                const uint32_t* __restrict__ first, const uint32_t* __restrict__ od,
                                const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
@@ -144,7 +145,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
     const uint64_t nwaves = (uint64_t)gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     for (uint64_t w = (uint64_t)blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
         const uint64_t vis = visited[w];
         if (vis == ~0ull)
@@ -222,6 +223,7 @@ __device__ void bfs_dense_body(const uint32_t* __restrict__ off, const uint32_t*
     add_work(work, ctr);
 }
 
+template <bool CW>
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ first,
                                                          const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
@@ -234,11 +236,12 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_dense(const uint32_t* __restri
         return;
     __shared__ uint64_t s_cur[WG_BFS_CUR_LDS];
     stage_cur(cur, V, s_cur);
-    bfs_dense_body(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
+    bfs_dense_body<CW>(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
 }
 
 // Dense level for chunks of WG_BFS_HEAVY in-edges of high in-degree vertices,
 // one wave per chunk.
+template <bool CW>
 __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he, uint32_t n,
                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
@@ -250,7 +253,7 @@ __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* 
     const int lane = wg_lane();
     const uint32_t nwaves = gridDim.x * (WG_BLOCK / WG_WAVE);
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     for (uint32_t c = blockIdx.x * (WG_BLOCK / WG_WAVE) + threadIdx.x / WG_WAVE; c < n; c += nwaves) {
         const uint32_t v = hv[c];
         if (test_bit64(visited, v))
@@ -293,6 +296,7 @@ __device__ void bfs_heavy_body(const uint32_t* __restrict__ hv, const uint32_t* 
     add_work(work, ctr);
 }
 
+template <bool CW>
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restrict__ hv, const uint32_t* __restrict__ he,
                                                          uint32_t n, const uint32_t* __restrict__ off,
                                                          const uint32_t* __restrict__ src, const uint32_t* __restrict__ od,
@@ -303,10 +307,11 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_heavy(const uint32_t* __restri
 {
     if (wg_gated_off(ctr, gate))
         return;
-    bfs_heavy_body(hv, he, n, off, src, od, cur, visited, next, packed, perm, ctr);
+    bfs_heavy_body<CW>(hv, he, n, off, src, od, cur, visited, next, packed, perm, ctr);
 }
 
 // Frontier-mode BFS level over the marked edge blocks.
+template <bool CW>
 __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
                                 const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                 const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
@@ -317,7 +322,7 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
     const uint32_t newdepth = ctr->level + 1u;
     const int lane = wg_lane();
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool first) {
         bool cand = false;
         uint32_t u = 0, v = 0;
@@ -360,6 +365,7 @@ __device__ void bfs_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 // block.  The out-edges of all roots are numbered one after the other
 // (root_off: the prefix of the roots' out-degrees) and spread over the grid;
 // each thread finds the root of its edge by binary search.
+template <bool CW>
 __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t nroots,
                                     const uint32_t* __restrict__ root_off,
                                     const uint32_t* __restrict__ csri_off, const uint32_t* __restrict__ csr_dst,
@@ -373,7 +379,7 @@ __device__ void bfs_push_roots_body(const uint32_t* __restrict__ roots, uint32_t
     const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t total = root_off[nroots];
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     for (uint64_t g = tid; g < total; g += nthreads) {
         work.edges += 1ull; // one out-edge of a root and its destination's state
         work.dests += 1ull;
@@ -438,6 +444,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_root_offsets(const uint32_t* __res
     }
 }
 
+template <bool CW>
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const uint32_t* __restrict__ od, const uint64_t* __restrict__ cur,
@@ -447,7 +454,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_blocks(wg_dev_counters* __rest
 {
     if (wg_gated_off(ctr, gate))
         return;
-    bfs_blocks_body(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
+    bfs_blocks_body<CW>(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
 
 // End of a BFS level: visited |= next, the new frontier replaces the old one
@@ -609,7 +616,7 @@ __device__ __forceinline__ void min_apply(typename P::T* __restrict__ prop, uint
 // order-independent, so pieces of a destination spread over lanes or tiles
 // meet through atomics (the reference's idempotent merge,
 // source/phases.c:54-65).
-template <class P, int IPT = WG_IPT>
+template <bool CW, class P, int IPT = WG_IPT>
 __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                               const void* __restrict__ w, typename P::T* __restrict__ prop,
                               const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
@@ -627,7 +634,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
     uint32_t* s_off = s_off_all[wave];
     T* s_min = s_min_all[wave];
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     const uint32_t stride = gridDim.x * WG_WPB;
     for (uint32_t t = blockIdx.x * WG_WPB + wave; t < ntiles; t += stride) {
         const uint32_t tv = tile_v[t];
@@ -728,7 +735,7 @@ __device__ void min_tile_body(const uint32_t* __restrict__ off, const uint32_t* 
     add_work(work, ctr);
 }
 
-template <class P, int IPT = WG_IPT>
+template <bool CW, class P, int IPT = WG_IPT>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                         const void* __restrict__ w, typename P::T* __restrict__ prop,
                                                         const uint32_t* __restrict__ od, const uint32_t* __restrict__ tile_v,
@@ -740,11 +747,11 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_tile(const uint32_t* __restric
         return;
     __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
     stage_hub<P>(prop, nhub, s_hub);
-    min_tile_body<P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
+    min_tile_body<CW, P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
 }
 
 // Frontier-mode pull over the marked edge blocks.
-template <class P>
+template <bool CW, class P>
 __device__ void min_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, uint32_t E,
                                 const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                 const void* __restrict__ w, typename P::T* __restrict__ prop,
@@ -753,7 +760,7 @@ __device__ void min_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
 {
     typedef typename P::T T;
     act_tally act = { 0ull, 0ull, 0ull };
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     walk_marked_blocks(wedge, blog2, E, [&](uint32_t p, bool ok, bool) {
         if (ok) {
             work.edges += 1ull;
@@ -768,7 +775,7 @@ __device__ void min_blocks_body(wg_dev_counters* __restrict__ ctr, int blog2, ui
     add_work(work, ctr);
 }
 
-template <class P>
+template <bool CW, class P>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __restrict__ ctr, int blog2_arg, uint32_t E,
                                                           const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst,
                                                           const void* __restrict__ w, typename P::T* __restrict__ prop,
@@ -778,7 +785,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_blocks(wg_dev_counters* __rest
 {
     if (wg_gated_off(ctr, gate))
         return;
-    min_blocks_body<P>(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
+    min_blocks_body<CW, P>(ctr, gate < 0 ? blog2_arg : ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
 }
 
 // ---- Wedge frontier ---------------------------------------------------------------
@@ -963,7 +970,7 @@ __device__ __forceinline__ void loop_choose_wave(const loop_cfg& g, const float*
         return;
     case LOOP_SCHED_LEARNED:
         // below the gate: the plain rule's frontier pull (wg_mlp_gated)
-        if (mlp && !wg_mlp_gated(act, g.threshold) && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
+        if (mlp && !wg_mlp_gated(act, g.ne) && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
             float x[WG_MLP_IN];
             float base[WG_MLP_ACTIONS];
             wg_mlp_features((int)g.app, iteration, g.nv, g.ne, nact, act, unvisited, x);
@@ -1093,6 +1100,7 @@ __global__ void k_decide(wg_dev_counters* c, const float* __restrict__ mlp, wg_r
 // small frontiers.  The decision stays a kernel of its
 // own: taking it in the last workgroup of the pull needs device-scope fences
 // in every wave, which cost more than the launch.
+template <bool CW>
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ dst, const uint32_t* __restrict__ od,
                                                          const uint64_t* __restrict__ cur, const uint64_t* __restrict__ visited,
@@ -1111,16 +1119,16 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_level(const uint32_t* __restri
     if (ctr->mode == WG_MODE_DENSE) {
         __shared__ uint64_t s_cur[WG_BFS_CUR_LDS];
         stage_cur(cur, V, s_cur);
-        bfs_dense_body(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
+        bfs_dense_body<CW>(off, src, first, od, cur, visited, next, packed, V, perm, ctr, s_cur);
         // and the high in-degree destinations the dense body leaves out
-        bfs_heavy_body(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
+        bfs_heavy_body<CW>(hv, he, nheavy, off, src, od, cur, visited, next, packed, perm, ctr);
     } else if (npush && ctr->iters == 0u) // the frontier is the npush roots; no blocks were marked
-        bfs_push_roots_body(roots, npush, root_off, csri_off, csr_dst, od, visited, next, packed, perm, ctr);
+        bfs_push_roots_body<CW>(roots, npush, root_off, csri_off, csr_dst, od, visited, next, packed, perm, ctr);
     else
-        bfs_blocks_body(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
+        bfs_blocks_body<CW>(ctr, ctr->blog2, E, src, dst, od, cur, visited, next, packed, wedge, perm);
 }
 
-template <class P, int IPT = WG_IPT>
+template <bool CW, class P, int IPT = WG_IPT>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restrict__ off, const uint32_t* __restrict__ src,
                                                          const uint32_t* __restrict__ dst, const void* __restrict__ w,
                                                          typename P::T* __restrict__ prop, const uint32_t* __restrict__ od,
@@ -1135,9 +1143,9 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restri
     if (ctr->mode == WG_MODE_DENSE) {
         __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
         stage_hub<P>(prop, nhub, s_hub);
-        min_tile_body<P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
+        min_tile_body<CW, P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
     } else
-        min_blocks_body<P>(ctr, ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
+        min_blocks_body<CW, P>(ctr, ctr->blog2, E, src, dst, w, prop, od, next, wedge, ovf);
 }
 
 // Advance (BFS) or count-and-clear (CC, SSSP) of the activated vertices, and
@@ -1520,6 +1528,7 @@ __device__ __forceinline__ uint32_t run_queue_len(const wg_run_ctl* rc, unsigned
     return n < WG_RUN_QCAP ? n : WG_RUN_QCAP;
 }
 
+template <bool CW>
 __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict__ csri_off,
                                                        const uint32_t* __restrict__ csr_dst, uint64_t* __restrict__ next,
                                                        uint64_t* __restrict__ visited, uint64_t* __restrict__ packed,
@@ -1538,7 +1547,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
         return;
     uint4* qc = qa;
     uint4* qn = qb;
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     for (unsigned int r = 0;; ++r) {
         const uint32_t nc = run_queue_len(rc, r);
         const unsigned long long newdepth = (unsigned long long)(st.level0 + 2u + r);
@@ -1593,7 +1602,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
         }
         bool ok = true;
         add_work(work, ctr);
-        work = { 0ull, 0ull };
+        work = work_tally<CW>();
         // a BFS activation is always a first visit
         if (!run_level_end(rc, ctr, r, act, nact, nact, st, s_mlp, learned, &ok))
             return;
@@ -1610,7 +1619,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_bfs_run(const uint32_t* __restrict
 // (even after some of its entries were expanded with the old value) is
 // appended again.  A run that ends after an even level moves its activated
 // set from spare to next.
-template <class P>
+template <bool CW, class P>
 __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict__ csri_off,
                                                        const uint32_t* __restrict__ csri_pos,
                                                        const uint32_t* __restrict__ dst, const void* __restrict__ w,
@@ -1631,7 +1640,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
         return;
     uint4* qc = qa;
     uint4* qn = qb;
-    work_tally work = { 0ull, 0ull };
+    work_tally<CW> work;
     for (unsigned int r = 0;; ++r) {
         uint64_t* dc = (r & 1u) ? spare : next;
         uint64_t* dn = (r & 1u) ? next : spare;
@@ -1692,7 +1701,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
         }
         bool ok = true;
         add_work(work, ctr);
-        work = { 0ull, 0ull };
+        work = work_tally<CW>();
         if (!run_level_end(rc, ctr, r, act, nact, fresh, st, s_mlp, learned, &ok)) {
             if (ok && (r & 1u) == 0u) // the last activations are in spare
                 for (uint64_t wd = (uint64_t)blockIdx.x * WG_BLOCK + threadIdx.x; wd < nwords;
@@ -1712,6 +1721,9 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_run(const uint32_t* __restrict
 }
 
 // ---- host side ----------------------------------------------------------------------
+
+// The instance of a pull kernel with (CW = true) or without the work counts.
+#define WG_CW(ctx, K, ...) ((ctx)->count_work ? K<true, ##__VA_ARGS__> : K<false, ##__VA_ARGS__>)
 
 // Folds the counter slots and copies the counters to the host mirror.
 int wg_read_counters(wg_ctx* ctx)
@@ -2028,7 +2040,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
     uint64_t* packed = (uint64_t*)ctx->prop[0];
     if (mode == WG_MODE_DENSE) {
         int slot = wg_prof_begin(ctx, WG_K_BFS_DENSE);
-        hipLaunchKernelGGL(k_bfs_dense, dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+        hipLaunchKernelGGL(WG_CW(ctx, k_bfs_dense), dim3(wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                            ctx->stream, ctx->off, ctx->src, ctx->first_src, ctx->od, ctx->bm_cur, ctx->bm_visited,
                            ctx->bm_next, packed, (uint32_t)ctx->V, ctx->perm, ctx->ctr, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -2036,7 +2048,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         wg_prof_end(ctx, WG_K_BFS_DENSE, slot);
         if (ctx->nbfs_heavy) {
             slot = wg_prof_begin(ctx, WG_K_BFS_HEAVY);
-            hipLaunchKernelGGL(k_bfs_heavy, dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
+            hipLaunchKernelGGL(WG_CW(ctx, k_bfs_heavy), dim3(wg_grid(ctx, ctx->nbfs_heavy, WG_BLOCK / WG_WAVE, 8)), dim3(WG_BLOCK), 0,
                                ctx->stream, ctx->bfs_heavy_v, ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->off, ctx->src,
                                ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed, ctx->perm, ctx->ctr,
                                WG_GATE_ALWAYS);
@@ -2046,7 +2058,7 @@ extern "C" int wg_edge_pull_bfs(wg_ctx* ctx, int mode, uint64_t* edges_activated
         }
     } else {
         int slot = wg_prof_begin(ctx, WG_K_BFS_BLOCKS);
-        hipLaunchKernelGGL(k_bfs_blocks, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
+        hipLaunchKernelGGL(WG_CW(ctx, k_bfs_blocks), dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited,
                            ctx->bm_next, packed, ctx->wedge, ctx->perm, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -2094,11 +2106,11 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
         const min_tiling mt = dense_min_tiling<P>(ctx);
         if (mt.ipt == 4)
-            hipLaunchKernelGGL((k_min_tile<P, 4>), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
+            hipLaunchKernelGGL(WG_CW(ctx, k_min_tile, P, 4), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
                                ctx->w, (typename P::T*)prop, ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, ctx->bm_next,
                                ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
         else
-            hipLaunchKernelGGL(k_min_tile<P>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
+            hipLaunchKernelGGL(WG_CW(ctx, k_min_tile, P), dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src,
                                ctx->w, (typename P::T*)prop, ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, ctx->bm_next,
                                ctx->ctr, &ctx->ctr->overflow, WG_GATE_ALWAYS, min_hub_count<P>(ctx));
         if ((rc = wg_check_launch(ctx)))
@@ -2106,7 +2118,7 @@ static int min_pull(wg_ctx* ctx, int mode, void* prop)
         wg_prof_end(ctx, WG_K_MIN_TILE, slot);
     } else {
         int slot = wg_prof_begin(ctx, WG_K_MIN_BLOCKS);
-        hipLaunchKernelGGL(k_min_blocks<P>, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
+        hipLaunchKernelGGL(WG_CW(ctx, k_min_blocks, P), dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, ctx->ctr,
                            ctx->wedge_log2, (uint32_t)ctx->E, ctx->src, ctx->dst, ctx->w, (typename P::T*)prop, ctx->od,
                            ctx->bm_next, ctx->wedge, &ctx->ctr->overflow, WG_GATE_ALWAYS);
         if ((rc = wg_check_launch(ctx)))
@@ -2202,7 +2214,7 @@ extern "C" int wg_read_sssp(wg_ctx* ctx, void* host_dist)
 
 extern "C" int wg_work_counts(wg_ctx* ctx, uint64_t* edges_read, uint64_t* dests_examined)
 {
-    if (!ctx || (ctx->app != WG_APP_BFS && ctx->app != WG_APP_SSSP && ctx->app != WG_APP_CC))
+    if (!ctx || !ctx->count_work || (ctx->app != WG_APP_BFS && ctx->app != WG_APP_SSSP && ctx->app != WG_APP_CC))
         return WG_E_STATE;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     const int rc = wg_read_counters(ctx);
@@ -2253,7 +2265,7 @@ static int enqueue_iteration(wg_ctx* ctx)
     const float* mlp = ctx->mlp_set ? ctx->mlp : nullptr;
     if (ctx->app == WG_APP_BFS) {
         const uint32_t dgrid = wg_grid(ctx, ctx->bm_words, WG_BLOCK / WG_WAVE, 8);
-        hipLaunchKernelGGL(k_bfs_level, dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
+        hipLaunchKernelGGL(WG_CW(ctx, k_bfs_level), dim3(dgrid > qgrid ? dgrid : qgrid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
                            ctx->src, ctx->dst, ctx->od, ctx->bm_cur, ctx->bm_visited, ctx->bm_next, packed,
                            (uint32_t)ctx->V, (uint32_t)ctx->E, ctx->wedge, ctx->perm, ctx->ctr, ctx->bfs_heavy_v,
                            ctx->bfs_heavy_e, ctx->nbfs_heavy, ctx->first_src, ctx->roots, ctx->push_roots,
@@ -2261,7 +2273,7 @@ static int enqueue_iteration(wg_ctx* ctx)
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
-    hipLaunchKernelGGL((k_min_level<P, IPT>), dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, \
+    hipLaunchKernelGGL(WG_CW(ctx, k_min_level, P, IPT), dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, \
                        ctx->w, (T*)ctx->prop[0], ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, (uint32_t)ctx->E,            \
                        ctx->bm_next, ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx))
 #define WG_MIN_LEVEL(P, T)                          \
@@ -2288,12 +2300,12 @@ static int enqueue_iteration(wg_ctx* ctx)
         const dim3 rg(run_wgs), rb(WG_BLOCK);
         uint4* qb = ctx->run_q + WG_RUN_QCAP;
         if (ctx->app == WG_APP_BFS) {
-            hipLaunchKernelGGL(k_bfs_run, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csr_dst, ctx->bm_next,
+            hipLaunchKernelGGL(WG_CW(ctx, k_bfs_run), rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csr_dst, ctx->bm_next,
                                ctx->bm_visited, packed, ctx->bm_words, ctx->perm, ctx->run_q, qb, ctx->run_ctl, ctx->ctr,
                                mlp);
         } else {
 #define WG_MIN_RUN(P, T)                                                                                               \
-    hipLaunchKernelGGL(k_min_run<P>, rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->w,            \
+    hipLaunchKernelGGL(WG_CW(ctx, k_min_run, P), rg, rb, 0, ctx->stream, ctx->csri_off, ctx->csri_pos, ctx->dst, ctx->w,            \
                        (T*)ctx->prop[0], ctx->bm_next, ctx->run_spare, ctx->bm_words, ctx->run_q, qb, ctx->run_ctl,     \
                        ctx->ctr, &ctx->ctr->overflow, mlp)
             if (ctx->app == WG_APP_CC)

@@ -19,7 +19,7 @@ void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_l
         return;
     case WG_SCHED_LEARNED:
         /* below the gate: the plain rule's frontier pull (wg_mlp_gated) */
-        if (!wg_mlp_gated(f->frontier_edges, threshold) &&
+        if (!wg_mlp_gated(f->frontier_edges, f->num_edges) &&
             wg_mlp_decide(f, f->frontier_edges > threshold, mode, block_log2) == WG_OK)
             return;
         /* no model loaded: the plain heuristic is the fallback */
