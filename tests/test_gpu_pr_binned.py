@@ -106,6 +106,29 @@ def test_binned_pair_count():
         c.close()
 
 
+def test_binned_tiles_over_many_bins():
+    """A window whose tiles span more than 64 destination bins (3000 edges from
+    the first window to destinations spread over 4M vertices, 256 or 512
+    bins): the block offsets past the 64 a tile's lanes hold come from the
+    wavefront's LDS row."""
+    V = 1 << 22
+    rng = np.random.default_rng(5)
+    src = np.concatenate([rng.integers(0, 16384, 3000), rng.integers(0, V, 4000)])
+    dst = np.concatenate([rng.integers(0, V, 3000), rng.integers(0, V, 4000)])
+    g = W.from_edges(V, src, dst, dedup=True)
+    ref, _ = oracle.Oracle.of(g, 8).pr(8, 0.85)
+    c = W.Context(0, relabel=False)
+    try:
+        c.load(g)
+        for shape in BINNED:
+            c.set_pr_shape(shape)
+            ranks, _ = c.run_pr(iterations=8, damping=0.85)
+            print(shape, c.pr_bins_info(), float(np.abs(ranks - ref).sum()))
+            assert float(np.abs(ranks - ref).sum()) <= PR_TOL
+    finally:
+        c.close()
+
+
 def test_binned_single_vertex_and_no_edges():
     c = W.Context(0)
     try:

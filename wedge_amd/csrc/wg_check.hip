@@ -154,17 +154,39 @@ __global__ void k_chk_sssp_vertices(const void* __restrict__ dist, uint64_t V, c
     count_bad(nbad, bad);
 }
 
+// Edges are grouped by destination, so a wavefront's 64 edges hold a few runs
+// of one destination each: the first lane of a run sets the destination's ok
+// bit once if any edge of the run justifies its label (on a graph with few
+// components nearly every edge does, and one atomic per edge on one word
+// serialised the kernel).  Every lane of a wavefront takes the same steps.
 __global__ void k_chk_cc_edges(const uint32_t* __restrict__ src, const uint32_t* __restrict__ dst, uint64_t E,
                                const uint32_t* __restrict__ label, uint64_t* __restrict__ ok,
                                unsigned long long* __restrict__ bad)
 {
     unsigned long long nbad = 0;
-    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < E; e += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t lu = label[src[e]], lv = label[dst[e]];
-        if (lv > lu)
-            ++nbad;
-        else if (lv == lu)
-            set_bit(ok, dst[e]);
+    const int lane = wg_lane();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e0 = blockIdx.x * (uint64_t)blockDim.x + (threadIdx.x & ~(WG_WAVE - 1u)); e0 < E; e0 += stride) {
+        const uint64_t e = e0 + (uint64_t)lane;
+        bool eq = false;
+        uint32_t d = 0xffffffffu;
+        if (e < E) {
+            d = dst[e];
+            const uint32_t lu = label[src[e]], lv = label[d];
+            if (lv > lu)
+                ++nbad;
+            else
+                eq = lv == lu;
+        }
+        const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1, WG_WAVE);
+        const unsigned long long heads = wg_ballot(lane == 0 || dprev != d);
+        const unsigned long long eqm = wg_ballot(eq);
+        if (((heads >> lane) & 1ull) && e < E) {
+            const unsigned long long above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
+            const unsigned long long upto = above ? (above & (0ull - above)) - 1ull : ~0ull; // lanes below the next run
+            if (eqm & upto & ~((1ull << lane) - 1ull))
+                set_bit(ok, d);
+        }
     }
     count_bad(nbad, bad);
 }

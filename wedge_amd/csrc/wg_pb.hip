@@ -75,6 +75,7 @@
 #define WG_PB_BSTART 0x4000u                    // code: that pair starts a block
 #define WG_PB_SRC 0x3fffu                       // code: source - window start
 #define WG_PB_MAX_WS_LOG 14
+#define WG_PB_SINK_ROWS 4096u                   // rows of WG_WAVE slots in the push's sink
 
 struct wg_pb {
     int ws_log, wd_log;
@@ -93,6 +94,7 @@ struct wg_pb {
     double* slot_acc;
     unsigned int* bin_ticket;
     double* bin_part;
+    double* sink;
 };
 
 namespace {
@@ -119,6 +121,7 @@ void pb_release(wg_pb* pb)
     pb_free_dev(pb->slot_acc);
     pb_free_dev(pb->bin_ticket);
     pb_free_dev(pb->bin_part);
+    pb_free_dev(pb->sink);
 }
 
 struct pb_tmp {
@@ -281,6 +284,8 @@ __global__ void k_pb_tile_blocks(const uint32_t* __restrict__ tile_pbase, uint32
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < ntiles)
         tile_bcnt[t] = pb_lower_bound(bstart, nb, tile_pbase[t]);
+    else if (t == ntiles) // the push reads tile t + 1's count as the end of tile t's blocks
+        tile_bcnt[t] = nb;
 }
 
 // Pair destinations into bin order: pair p of block i goes to p + blk_off[i].
@@ -538,6 +543,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     PB_TRY(hipMalloc((void**)&pb->bin_ticket, B * sizeof(unsigned int)));
     PB_TRY(hipMemsetAsync(pb->bin_ticket, 0, B * sizeof(unsigned int), st));
     PB_TRY(hipMalloc((void**)&pb->bin_part, B * sizeof(double)));
+    PB_TRY(hipMalloc((void**)&pb->sink, (uint64_t)WG_PB_SINK_ROWS * WG_WAVE * sizeof(double)));
     PB_TRY(hipStreamSynchronize(st));
     return WG_OK;
 }
@@ -590,6 +596,27 @@ static __device__ __forceinline__ double block_sum_pb(double v, double* s_red)
     return r;
 }
 
+// s[i] = f(src[i]) for i < n <= N: every load issued before the first LDS
+// store (one wait per window instead of one per step).
+template <int N, typename S, typename D, class F>
+__device__ __forceinline__ void pb_fill_lds(D* s, const S* __restrict__ src, uint32_t n, F f)
+{
+    static_assert(N % WG_PB_THREADS == 0, "whole rounds");
+    constexpr int R = N / WG_PB_THREADS;
+    S x[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+        x[u] = i < n ? src[i] : S(0);
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+        if (i < n)
+            s[i] = f(x[u]);
+    }
+}
+
 // Codes of the lane's WG_PB_IPT entries of a tile (one 16-byte load).
 #define WG_PB_IPT 8
 __device__ __forceinline__ uint4 pb_load_codes(const uint16_t* __restrict__ code, uint32_t t, int lane)
@@ -615,12 +642,22 @@ struct pb_or {
     }
 };
 
-// A tile's inputs, loaded one round ahead: the lane's codes, the tile's first
-// pair number and blocks started before it, and the offsets of the 64 blocks
-// from the one holding the tile's first pair (lane l: block bcnt - 1 + l).
+// Row of the push's sink (WG_WAVE slots each) for this wavefront: its stores
+// before its first pair go there; wavefronts that share a row only ever store
+// there values nobody reads.
+__device__ __forceinline__ uint32_t pb_sink_row()
+{
+    return ((blockIdx.x * (uint32_t)WG_PB_WAVES + threadIdx.x / WG_WAVE) & (WG_PB_SINK_ROWS - 1u)) * WG_WAVE;
+}
+
+// A tile's inputs: the lane's codes, the tile's first pair number, the blocks
+// started before it and before the next tile, and the offsets of the 64
+// blocks from the one holding the tile's first pair (lane l: block
+// bcnt - 1 + l).  The per-tile words are scalar loads (t is uniform), so the
+// one dependent load, the offsets, waits on the scalar counter only.
 struct pb_tile_in {
     uint4 codes;
-    uint32_t pbase, bcnt, boff;
+    uint32_t pbase, bcnt, bnext, boff;
 };
 
 __device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ code,
@@ -630,102 +667,154 @@ __device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ 
                                                    int lane)
 {
     pb_tile_in in;
+    t = __builtin_amdgcn_readfirstlane(t);
     in.codes = pb_load_codes(code, t, lane);
     in.pbase = tile_pbase[t];
     in.bcnt = tile_bcnt[t];
-    const uint32_t b = in.bcnt + (uint32_t)lane - 1u; // wraps to 2^32 - 1 for lane 0 of the first tile
-    in.boff = b < nblocks ? blk_off[b] : 0u;
+    in.bnext = tile_bcnt[t + 1];
+    if (in.bnext > nblocks || in.bnext < in.bcnt) // never on a layout pb_build wrote; keeps every read in bounds
+        in.bnext = in.bcnt;
+    uint32_t b = in.bcnt + (uint32_t)lane - 1u; // wraps to 2^32 - 1 for lane 0 of the first tile (never read)
+    if (b >= nblocks)
+        b = nblocks - 1u; // nblocks >= 1 whenever there are tiles
+    in.boff = blk_off[b];
     return in;
 }
 
+// Blocks a tile may touch beyond the 64 its lanes hold (a tile has at most
+// WG_PB_TILE pairs, so at most WG_PB_TILE + 1 blocks).
+#define WG_PB_OVF (WG_PB_TILE + 1u - WG_WAVE)
+
+// One tile of the push (one wavefront).  Lane l takes the tile's entries
+// [8l, 8l + 8): it combines its entries in order, and each entry flagged as a
+// pair's last emits that pair's value; the value still open at the lane's end
+// flows into the next lanes by a segmented inclusive scan (a lane with an end
+// starts a new segment).  A value goes to its position in bin order,
+// p + blk_off[block of p]: the first 64 blocks' offsets from the lanes that
+// loaded them, the rest (tiles of very sparse windows) from s_ovf, this
+// wavefront's LDS row, filled first.  Every lane stores at every entry: an
+// entry that ends no pair stores the lane's last emitted value again at the
+// same place (the sink before the first), so the number of stores per tile is
+// fixed and the compiler's waits for the next tile's loads never wait for
+// this tile's stores.
+template <class OP, typename W, class LOAD>
+__device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, const W* s_win, uint32_t wend,
+                                             const uint32_t* __restrict__ blk_off, typename OP::V* __restrict__ part,
+                                             uint32_t* s_ovf, typename OP::V*& last_at, typename OP::V& last_v,
+                                             LOAD load)
+{
+    typedef typename OP::V V;
+    const int lane = wg_lane();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t nb = in.bnext - in.bcnt + 1u; // blocks touched, counted from block bcnt - 1
+    if (nb > WG_PB_TILE + 1u)              // at most one block per pair (never more on a valid layout)
+        nb = WG_PB_TILE + 1u;
+    if (nb > (uint32_t)WG_WAVE) {          // the whole wavefront takes this branch
+        for (uint32_t i = (uint32_t)lane; i + WG_WAVE < nb; i += WG_WAVE)
+            s_ovf[i] = blk_off[in.bcnt - 1u + WG_WAVE + i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const uint4 raw = in.codes;
+    const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
+                                   raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
+    const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT;
+    V v[WG_PB_IPT];
+    bool e[WG_PB_IPT], bs[WG_PB_IPT];
+    V run = OP::zero();
+    bool has_end = false;
+#pragma unroll
+    for (int j = 0; j < WG_PB_IPT; ++j) {
+        const bool real = q0 + (uint32_t)j < wend;
+        v[j] = real ? load(s_win[c[j] & WG_PB_SRC]) : OP::zero();
+        e[j] = real && (c[j] & WG_PB_END);
+        bs[j] = real && (c[j] & WG_PB_BSTART);
+        run = OP::op(run, v[j]);
+        if (e[j]) {
+            run = OP::zero();
+            has_end = true;
+        }
+    }
+    // segmented inclusive scan of the open values over the lanes
+    V sc = run;
+    int f = has_end ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < WG_WAVE; d <<= 1) {
+        const V o = OP::shfl_up(sc, d);
+        const int of = __shfl_up(f, d, WG_WAVE);
+        if (lane >= d) {
+            if (!f)
+                sc = OP::op(o, sc);
+            f |= of;
+        }
+    }
+    V carry = OP::shfl_up(sc, 1);
+    if (lane == 0)
+        carry = OP::zero();
+    uint32_t pre = 0, bpre = 0;
+#pragma unroll
+    for (int j = 0; j < WG_PB_IPT; ++j) {
+        pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
+        bpre += (uint32_t)__popcll(wg_ballot(bs[j]) & below);
+    }
+    uint32_t p = in.pbase + pre; // window-order number of the lane's next pair
+    uint32_t k = bpre;           // its block - (in.bcnt - 1), before its own start flag
+    V r = carry;
+#pragma unroll
+    for (int j = 0; j < WG_PB_IPT; ++j) {
+        r = OP::op(r, v[j]);
+        k += bs[j] ? 1u : 0u;
+        // the block's offset from the lane that loaded it (every lane takes part)
+        const uint32_t from_lane = (uint32_t)__shfl((int)in.boff, (int)(k & (WG_WAVE - 1)), WG_WAVE);
+        if (e[j]) {
+            const uint32_t off = k < WG_WAVE ? from_lane : s_ovf[k - WG_WAVE];
+            last_at = part + (uint32_t)(p + off);
+            last_v = r;
+            ++p;
+            r = OP::zero();
+        }
+        *last_at = last_v;
+    }
+}
+
 // Push body: one workgroup per task (a window's run of tiles), one wavefront
-// per tile, each tile's inputs loaded two rounds ahead.  s_win holds the
-// window (filled by the caller, converted by LOAD on read).  Lane l takes the
-// tile's entries [8l, 8l + 8): it combines its entries in order, and each
-// entry flagged as a pair's last emits that pair's value; the value still open
-// at the lane's end flows into the next lanes by a segmented inclusive scan (a
-// lane with an end starts a new segment).  A value goes to its position in bin
-// order, p + blk_off[block of p].
+// per tile.  Two tiles' inputs alternate between two sets of registers, each
+// reloaded (index clamped, never skipped) as soon as its tile is done, so a
+// tile's loads are in flight while the other tile runs.  s_win holds the
+// window (filled by the caller, converted by LOAD on read); sink is this
+// wavefront's row of WG_WAVE slots for the stores before its first pair.
 template <class OP, typename W, class LOAD>
 __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, const uint4 tk,
                                               const uint32_t* __restrict__ tile_pbase,
                                               const uint32_t* __restrict__ tile_bcnt,
                                               const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                              typename OP::V* __restrict__ part, LOAD load)
+                                              typename OP::V* __restrict__ part, uint32_t* s_ovf,
+                                              typename OP::V* __restrict__ sink, LOAD load)
 {
     typedef typename OP::V V;
     const int lane = wg_lane();
-    const int wave = threadIdx.x / WG_WAVE;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t t = tk.y + (uint32_t)wave;
-    pb_tile_in n1 = {}, n2 = {};
-    if (t < tk.z)
-        n1 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t, lane);
-    if (t + WG_PB_WAVES < tk.z)
-        n2 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t + WG_PB_WAVES, lane);
-    for (; t < tk.z; t += WG_PB_WAVES) {
-        const pb_tile_in in = n1;
-        n1 = n2;
-        if (t + 2 * WG_PB_WAVES < tk.z) // two rounds ahead
-            n2 = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t + 2 * WG_PB_WAVES, lane);
-        const uint4 raw = in.codes;
-        const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
-                                       raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
-        const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT;
-        V v[WG_PB_IPT];
-        bool e[WG_PB_IPT], bs[WG_PB_IPT];
-        V run = OP::zero();
-        bool has_end = false;
-#pragma unroll
-        for (int j = 0; j < WG_PB_IPT; ++j) {
-            const bool real = q0 + (uint32_t)j < tk.w;
-            v[j] = real ? load(s_win[c[j] & WG_PB_SRC]) : OP::zero();
-            e[j] = real && (c[j] & WG_PB_END);
-            bs[j] = real && (c[j] & WG_PB_BSTART);
-            run = OP::op(run, v[j]);
-            if (e[j]) {
-                run = OP::zero();
-                has_end = true;
-            }
-        }
-        // segmented inclusive scan of the open values over the lanes
-        V sc = run;
-        int f = has_end ? 1 : 0;
-#pragma unroll
-        for (int d = 1; d < WG_WAVE; d <<= 1) {
-            const V o = OP::shfl_up(sc, d);
-            const int of = __shfl_up(f, d, WG_WAVE);
-            if (lane >= d) {
-                if (!f)
-                    sc = OP::op(o, sc);
-                f |= of;
-            }
-        }
-        V carry = OP::shfl_up(sc, 1);
-        if (lane == 0)
-            carry = OP::zero();
-        uint32_t pre = 0, bpre = 0;
-#pragma unroll
-        for (int j = 0; j < WG_PB_IPT; ++j) {
-            pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
-            bpre += (uint32_t)__popcll(wg_ballot(bs[j]) & below);
-        }
-        uint32_t p = in.pbase + pre;      // window-order number of the lane's next pair
-        uint32_t k = bpre;                // its block - (in.bcnt - 1), before its own start flag
-        V r = carry;
-#pragma unroll
-        for (int j = 0; j < WG_PB_IPT; ++j) {
-            r = OP::op(r, v[j]);
-            k += bs[j] ? 1u : 0u;
-            // the block's offset from the lane that loaded it (every lane takes part)
-            const uint32_t from_lane = (uint32_t)__shfl((int)in.boff, (int)(k & (WG_WAVE - 1)), WG_WAVE);
-            if (e[j]) {
-                const uint32_t off = k < WG_WAVE ? from_lane : blk_off[in.bcnt - 1u + k];
-                part[p + off] = r;
-                ++p;
-                r = OP::zero();
-            }
-        }
+    uint32_t t = tk.y + (uint32_t)(threadIdx.x / WG_WAVE);
+    if (t >= tk.z)
+        return;
+    const uint32_t tlast = tk.z - 1u;
+    V* last_at = sink + lane;
+    V last_v = OP::zero();
+    pb_tile_in a = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t, lane);
+    pb_tile_in b = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
+                                t + WG_PB_WAVES < tk.z ? t + WG_PB_WAVES : tlast, lane);
+    for (;;) {
+        pb_push_tile<OP>(a, t, s_win, tk.w, blk_off, part, s_ovf, last_at, last_v, load);
+        a = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
+                         t + 2 * WG_PB_WAVES < tk.z ? t + 2 * WG_PB_WAVES : tlast, lane);
+        if (t + WG_PB_WAVES >= tk.z)
+            break;
+        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, blk_off, part, s_ovf, last_at, last_v, load);
+        b = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
+                         t + 3 * WG_PB_WAVES < tk.z ? t + 3 * WG_PB_WAVES : tlast, lane);
+        t += 2 * WG_PB_WAVES;
+        if (t >= tk.z)
+            break;
     }
 }
 
@@ -736,21 +825,22 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
                                                            const uint32_t* __restrict__ tile_pbase,
                                                            const uint32_t* __restrict__ tile_bcnt,
                                                            const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                                           double* __restrict__ part, uint32_t V)
+                                                           double* __restrict__ part, double* __restrict__ sink,
+                                                           uint32_t V)
 {
     static_assert(WG_WAVE * WG_PB_IPT == WG_PB_TILE, "a tile is one wavefront's entries");
     static_assert(WS <= WG_PB_SRC + 1, "window larger than the source field");
     __shared__ T s_win[WS];
+    __shared__ uint32_t s_ovf[WG_PB_WAVES][WG_PB_OVF];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
         const uint32_t nv = (V - v0 < (uint32_t)WS) ? V - v0 : (uint32_t)WS;
         __syncthreads(); // the previous task's wavefronts are done with s_win
-        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
-            s_win[i] = cin[v0 + i];
+        pb_fill_lds<WS>(s_win, cin + v0, nv, [](T x) { return x; });
         __syncthreads();
         pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
-                              [](T x) { return (double)x; });
+                              s_ovf[threadIdx.x / WG_WAVE], sink + pb_sink_row(), [](T x) { return (double)x; });
     }
 }
 
@@ -868,6 +958,14 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         const uint4 sl = slice[k];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
+        // the bin's out-degrees for the vertex phase, loaded while the slice streams
+        constexpr int R = WD / WG_PB_THREADS;
+        uint32_t odr[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+            odr[u] = (MODE == 0 && i < nv) ? od[v0 + i] : 0u;
+        }
         __syncthreads(); // the previous slice is done with s_acc
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
@@ -879,11 +977,15 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
                             [](double a, double x) { return a + x; }))
             continue;
         double s_part = 0.0;
-        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+            if (i >= nv)
+                break;
             const uint32_t v = v0 + i;
             const double accv = s_acc[i];
             if (MODE == 0) {
-                const uint32_t odv = od[v];
+                const uint32_t odv = odr[u];
                 const double t1 = accv + corr;
                 const double t2 = t1 * damping;
                 const double rank = t2 + omd_n;
@@ -925,7 +1027,8 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     if (pb->ntasks) {
         const int slot = wg_prof_begin(ctx, WG_K_PB_PUSH);
         hipLaunchKernelGGL((k_pb_push<T, WS>), dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, cin,
-                           pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, pb->part, V);
+                           pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, pb->part,
+                           pb->sink, V);
         int rc = wg_check_launch(ctx);
         if (rc)
             return rc;
@@ -992,18 +1095,20 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
                                                               uint32_t ntasks, const uint32_t* __restrict__ tile_pbase,
                                                               const uint32_t* __restrict__ tile_bcnt,
                                                               const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                                              unsigned long long* __restrict__ part, uint32_t V)
+                                                              unsigned long long* __restrict__ part,
+                                                              double* __restrict__ sink, uint32_t V)
 {
     __shared__ unsigned long long s_win[WS];
+    __shared__ uint32_t s_ovf[WG_PB_WAVES][WG_PB_OVF];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
         const uint32_t nv = (V - v0 < (uint32_t)WS) ? V - v0 : (uint32_t)WS;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
-            s_win[i] = front[v0 + i] & live;
+        pb_fill_lds<WS>(s_win, front + v0, nv, [live](unsigned long long x) { return x & live; });
         __syncthreads();
         pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
+                             s_ovf[threadIdx.x / WG_WAVE], reinterpret_cast<unsigned long long*>(sink) + pb_sink_row(),
                              [](unsigned long long x) { return x; });
     }
 }
@@ -1024,6 +1129,14 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
         const uint4 sl = slice[k];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
+        // the bin's seen masks for the epilogue, loaded while the slice streams
+        constexpr int R = WD / WG_PB_THREADS;
+        unsigned long long sr[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+            sr[u] = i < nv ? seen[v0 + i] : ~0ull;
+        }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0ull;
@@ -1037,13 +1150,13 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
             !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
                             [](unsigned long long a, unsigned long long x) { return a | x; }))
             continue;
-        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS) {
-            const unsigned long long a = s_acc[i];
-            if (a) {
-                const unsigned long long n = a & live & ~seen[v0 + i];
-                if (n)
-                    nl[v0 + i] = n;
-            }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+            const unsigned long long a = i < nv ? s_acc[i] : 0ull;
+            const unsigned long long n = a & live & ~sr[u];
+            if (n)
+                nl[v0 + i] = n;
         }
     }
 }
@@ -1059,7 +1172,8 @@ int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsign
     unsigned long long* part = reinterpret_cast<unsigned long long*>(pb->part);
     if (pb->ntasks)
         hipLaunchKernelGGL(k_pb_push_or<16384>, dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, front,
-                           live, pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, part, V);
+                           live, pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, part,
+                           pb->sink, V);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     hipLaunchKernelGGL(k_pb_pull_or<16384>, dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
