@@ -20,15 +20,17 @@
 //   push     per window: a workgroup copies the window's contributions to LDS
 //            and streams the window's edges, ordered by (destination, source),
 //            2 bytes each; each run of edges with one destination (a "pair":
-//            window x destination) becomes one FP64 partial sum
-//   pull     per bin: a workgroup adds the partials of every window into LDS
-//            accumulators, then runs the vertex phase on the bin's vertices
+//            window x destination) becomes one FP64 partial sum, stored in
+//            window order (each wavefront writes one contiguous range)
+//   pull     per bin: a workgroup gathers the partials of every window (one
+//            run of consecutive positions per window) into LDS accumulators,
+//            then runs the vertex phase on the bin's vertices
 // Bytes per iteration: 2 per edge, 18 per pair (the partial written and read
 // back, and its 2-byte destination), the windows' LDS copies, and the vertex
 // phase's 16 (FP64) or 12 (FP32) per vertex plus offsets-free out-degrees.
 // After the degree relabelling most edges of a window share their destination
 // with other edges of that window (rmat-s24: 0.27 pairs per edge with 16K-vertex
-// windows, rmat-s25: 0.31), so an iteration moves about 7-11 bytes per edge.
+// windows, rmat-s27: 0.40), so an iteration moves about 7-11 bytes per edge.
 //
 // FP64 order: a partial adds its edges in source order within each lane, then
 // the lanes' open runs by a segmented scan (a fixed order); the pull adds
@@ -40,21 +42,21 @@
 //   code[q]        the edges by (window, destination, source), each window's run
 //                  padded to a multiple of WG_PB_TILE entries: bits 13..0 =
 //                  source - window start, bit 15 = last edge of its pair (also
-//                  forced on a tile's last entry, so no pair crosses tiles),
-//                  bit 14 = on a pair's last edge: the pair starts a block
-//   blocks         runs of pairs of one (window, bin); the push numbers the
-//                  pairs in window order, the pull reads them in bin order
-//                  (window order within a bin): pair p of block i goes to
-//                  position p + blk_off[i] of part[] (32-bit wrap-around)
-//   tile_pbase[t]  pairs before tile t (WG_PB_TILE entries per tile)
-//   tile_bcnt[t]   blocks started before tile t's first pair
+//                  forced on a tile's last entry, so no pair crosses tiles)
+//   tile_pbase[t]  pairs before tile t (WG_PB_TILE entries per tile): pair p's
+//                  partial is part[p]
 //   task[k]        push work: {window, first tile, end tile, end of the real
 //                  entries (code index)}, at most WG_PB_TASK_TILES tiles
-//   part[o]        partial sums in bin order (written by the push every
-//                  iteration), pdst[o] their destinations - bin start
+//   blocks         runs of pairs of one (window, bin), numbered in bin order
+//                  (window order within a bin); bin order position o of block
+//                  kb holds pair o - boff[kb] (32-bit wrap-around)
+//   pdst[o]        destination - bin start of the pair at bin-order position o,
+//                  bit 15 set where a block starts
+//   chunk_block[c] the block holding position 64c (c <= positions / 64: one
+//                  past the end holds the block count)
 //   slice[j]       pull work: {bin, first position, end position, accumulator
 //                  slot or WG_PB_NOSLOT when the slice is its bin's only one},
-//                  at most WG_PB_SLICE pairs, a contiguous range of part[]
+//                  at most WG_PB_SLICE pairs, a contiguous range of positions
 //   bin_slot[b]    {first slot, slices} of a bin cut into several slices: each
 //                  slice writes its accumulators to its slot, and the last one
 //                  to finish (a per-bin ticket) adds the slots in slice order
@@ -72,10 +74,10 @@
 #define WG_PB_SLICE (256u * 1024u)              // pairs per pull workgroup (default)
 #define WG_PB_NOSLOT 0xffffffffu
 #define WG_PB_END 0x8000u                       // code: last edge of a pair
-#define WG_PB_BSTART 0x4000u                    // code: that pair starts a block
 #define WG_PB_SRC 0x3fffu                       // code: source - window start
 #define WG_PB_MAX_WS_LOG 14
-#define WG_PB_SINK_ROWS 4096u                   // rows of WG_WAVE slots in the push's sink
+#define WG_PB_BFLAG 0x8000u                     // pdst: a block starts here
+#define WG_PB_DMASK 0x7fffu                     // pdst: destination - bin start
 
 struct wg_pb {
     int ws_log, wd_log;
@@ -84,9 +86,9 @@ struct wg_pb {
     uint32_t ntiles, ntasks, nblocks, nslices, nslots;
     uint16_t* code;
     uint32_t* tile_pbase;
-    uint32_t* tile_bcnt;
-    uint32_t* blk_off;
     uint4* task;
+    uint32_t* boff;
+    uint32_t* chunk_block;
     uint16_t* pdst;
     double* part;
     uint4* slice;
@@ -94,7 +96,6 @@ struct wg_pb {
     double* slot_acc;
     unsigned int* bin_ticket;
     double* bin_part;
-    double* sink;
 };
 
 namespace {
@@ -111,9 +112,9 @@ void pb_release(wg_pb* pb)
 {
     pb_free_dev(pb->code);
     pb_free_dev(pb->tile_pbase);
-    pb_free_dev(pb->tile_bcnt);
-    pb_free_dev(pb->blk_off);
     pb_free_dev(pb->task);
+    pb_free_dev(pb->boff);
+    pb_free_dev(pb->chunk_block);
     pb_free_dev(pb->pdst);
     pb_free_dev(pb->part);
     pb_free_dev(pb->slice);
@@ -121,7 +122,6 @@ void pb_release(wg_pb* pb)
     pb_free_dev(pb->slot_acc);
     pb_free_dev(pb->bin_ticket);
     pb_free_dev(pb->bin_part);
-    pb_free_dev(pb->sink);
 }
 
 struct pb_tmp {
@@ -252,17 +252,6 @@ __global__ void k_pb_block_keys(const uint32_t* __restrict__ start, uint32_t nb,
         key[i] = pkey[start[i]];
 }
 
-// Marks the last edge of every pair that starts a block.
-__global__ void k_pb_mark_blocks(uint64_t nq, uint16_t* __restrict__ code, const uint32_t* __restrict__ pidx,
-                                 const uint8_t* __restrict__ bflag)
-{
-    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nq; q += (uint64_t)gridDim.x * blockDim.x) {
-        const uint16_t c = code[q];
-        if ((c & WG_PB_END) && bflag[pidx[q]])
-            code[q] = (uint16_t)(c | WG_PB_BSTART);
-    }
-}
-
 // Number of entries of the ascending array a[0..n) below x.
 __device__ __forceinline__ uint32_t pb_lower_bound(const uint32_t* __restrict__ a, uint32_t n, uint32_t x)
 {
@@ -277,24 +266,14 @@ __device__ __forceinline__ uint32_t pb_lower_bound(const uint32_t* __restrict__ 
     return lo;
 }
 
-// tile_bcnt[t]: blocks started before tile t's first pair.
-__global__ void k_pb_tile_blocks(const uint32_t* __restrict__ tile_pbase, uint32_t ntiles,
-                                 const uint32_t* __restrict__ bstart, uint32_t nb, uint32_t* __restrict__ tile_bcnt)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ntiles)
-        tile_bcnt[t] = pb_lower_bound(bstart, nb, tile_pbase[t]);
-    else if (t == ntiles) // the push reads tile t + 1's count as the end of tile t's blocks
-        tile_bcnt[t] = nb;
-}
-
-// Pair destinations into bin order: pair p of block i goes to p + blk_off[i].
+// Pair destinations into bin order: pair p of window-order block i goes to
+// position p + woff[i]; the first pair of each block carries WG_PB_BFLAG.
 __global__ void k_pb_place(const uint16_t* __restrict__ pdst_w, uint32_t npairs, const uint32_t* __restrict__ bstart,
-                           uint32_t nb, const uint32_t* __restrict__ blk_off, uint16_t* __restrict__ pdst_b)
+                           uint32_t nb, const uint32_t* __restrict__ woff, uint16_t* __restrict__ pdst_b)
 {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += gridDim.x * blockDim.x) {
         const uint32_t i = pb_lower_bound(bstart, nb, p + 1u) - 1u; // the last block starting at or before p
-        pdst_b[p + blk_off[i]] = pdst_w[p];
+        pdst_b[p + woff[i]] = (uint16_t)(pdst_w[p] | (bstart[i] == p ? WG_PB_BFLAG : 0u));
     }
 }
 
@@ -321,7 +300,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     const uint64_t V = ctx->V, E = ctx->E;
     const uint32_t S = (uint32_t)((V + (1ull << ws_log) - 1) >> ws_log);
     const uint32_t B = (uint32_t)((V + (1ull << wd_log) - 1) >> wd_log);
-    if (S > 65536u || B > 65536u || ws_log > WG_PB_MAX_WS_LOG || wd_log > 16)
+    if (S > 65536u || B > 65536u || ws_log > WG_PB_MAX_WS_LOG || wd_log > 15)
         return WG_E_RANGE;
     hipStream_t st = ctx->stream;
     pb->ws_log = ws_log;
@@ -392,7 +371,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     pb->ntasks = (uint32_t)task.size();
 
     // 3. codes, pair numbers (window order), pair destinations and (window, bin) keys
-    pb_tmp twin, pqd, pidx, pkey, pdst_w, bflag, bst, nsel, bk, boff;
+    pb_tmp twin, pqd, pidx, pkey, pdst_w, bflag, bst, nsel, bk;
     uint32_t npairs = 0;
     int rc;
     if (nq) {
@@ -421,7 +400,6 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     }
     pb->npairs = npairs;
     PB_TRY(hipMalloc((void**)&pb->tile_pbase, ((uint64_t)pb->ntiles + 1) * sizeof(uint32_t)));
-    PB_TRY(hipMalloc((void**)&pb->tile_bcnt, ((uint64_t)pb->ntiles + 1) * sizeof(uint32_t)));
     PB_TRY(hipMalloc((void**)&pb->pdst, (npairs ? npairs : 1) * sizeof(uint16_t)));
     PB_TRY(hipMalloc((void**)&pb->part, (npairs ? npairs : 1) * sizeof(double)));
     std::vector<uint32_t> bstart, bkey;
@@ -438,16 +416,12 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         (void)hipFree(perm.p);
         perm.p = nullptr;
 
-        // 4. blocks: runs of equal (window, bin) among the pairs; their first
-        //    pairs are flagged in the codes
+        // 4. blocks: runs of equal (window, bin) among the pairs
         PB_TRY(hipMalloc(&bflag.p, npairs));
         PB_TRY(hipMalloc(&bst.p, (uint64_t)npairs * 4));
         PB_TRY(hipMalloc(&nsel.p, sizeof(uint64_t)));
         hipLaunchKernelGGL(k_pb_block_flags, dim3(wg_grid(ctx, npairs, 1024, 8)), dim3(256), 0, st,
                            (const uint32_t*)pkey.p, (uint64_t)npairs, (uint8_t*)bflag.p);
-        PB_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_pb_mark_blocks, dim3(gq), dim3(256), 0, st, nq, pb->code, (const uint32_t*)pidx.p,
-                           (const uint8_t*)bflag.p);
         PB_TRY(hipGetLastError());
         (void)hipFree(pidx.p);
         pidx.p = nullptr;
@@ -467,9 +441,6 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         hipLaunchKernelGGL(k_pb_block_keys, dim3((uint32_t)(nb / 256 + 1)), dim3(256), 0, st, (const uint32_t*)bst.p,
                            (uint32_t)nb, (const uint32_t*)pkey.p, (uint32_t*)bk.p);
         PB_TRY(hipGetLastError());
-        hipLaunchKernelGGL(k_pb_tile_blocks, dim3(pb->ntiles / 256 + 1), dim3(256), 0, st, pb->tile_pbase, pb->ntiles,
-                           (const uint32_t*)bst.p, (uint32_t)nb, pb->tile_bcnt);
-        PB_TRY(hipGetLastError());
         bstart.resize(nb);
         bkey.resize(nb);
         if (nb) {
@@ -480,7 +451,8 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     }
 
     // 5. bin order of the blocks (window order kept within a bin), their
-    //    offsets, slices and accumulator slots (host)
+    //    offsets, the block of every 64th position, slices and accumulator
+    //    slots (host)
     const uint64_t nb = bstart.size();
     pb->nblocks = (uint32_t)nb;
     std::vector<uint64_t> bin_first(B + 1, 0);
@@ -494,7 +466,10 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
         for (uint64_t i = 0; i < nb; ++i) // blocks are in (window, bin) order: stable by bin
             by_bin[fill[bkey[i] & 0xffffu]++] = (uint32_t)i;
     }
-    std::vector<uint32_t> blk_off(nb ? nb : 1, 0u);
+    std::vector<uint32_t> woff(nb ? nb : 1, 0u); // by window-order block
+    std::vector<uint32_t> boff(nb ? nb : 1, 0u); // by bin-order block
+    const uint64_t nchunks = ((uint64_t)npairs + 63u) / 64u;
+    std::vector<uint32_t> chunk_block(nchunks + 1, (uint32_t)nb);
     std::vector<uint32_t> bin_out(B + 1, 0u); // first position of each bin in part[]
     const uint32_t slice_max = ctx->pb_slice ? ctx->pb_slice : WG_PB_SLICE;
     std::vector<uint4> slice;
@@ -502,14 +477,18 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     uint32_t nslots = 0;
     {
         uint32_t o = 0;
+        uint64_t c = 0; // next chunk whose first position has no block yet
         for (uint32_t b = 0; b < B; ++b) {
             bin_out[b] = o;
             for (uint64_t k = bin_first[b]; k < bin_first[b + 1]; ++k) {
                 const uint32_t i = by_bin[k];
                 const uint32_t p0 = bstart[i];
                 const uint32_t p1 = (i + 1 < nb) ? bstart[i + 1] : npairs;
-                blk_off[i] = o - p0; // wraps around: p + blk_off[i] = o + (p - p0)
+                woff[i] = o - p0; // wraps around: p + woff[i] = o + (p - p0)
+                boff[k] = o - p0;
                 o += p1 - p0;
+                for (; c < nchunks && c * 64u < (uint64_t)o; ++c) // positions [o - (p1 - p0), o) are block k's
+                    chunk_block[c] = (uint32_t)k;
             }
             const uint32_t end = o;
             const size_t first_slice = slice.size();
@@ -531,19 +510,20 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     }
     pb->nslices = (uint32_t)slice.size();
     pb->nslots = nslots;
+    pb_tmp woffd;
     if ((rc = pb_upload(ctx, &pb->task, task)) || (rc = pb_upload(ctx, &pb->slice, slice)) ||
-        (rc = pb_upload(ctx, &pb->bin_slot, bin_slot)) || (rc = pb_upload(ctx, &pb->blk_off, blk_off)))
+        (rc = pb_upload(ctx, &pb->bin_slot, bin_slot)) || (rc = pb_upload(ctx, &pb->boff, boff)) ||
+        (rc = pb_upload(ctx, &pb->chunk_block, chunk_block)) || (rc = pb_upload(ctx, (uint32_t**)&woffd.p, woff)))
         return rc;
     if (npairs) // destinations in bin order
         hipLaunchKernelGGL(k_pb_place, dim3(wg_grid(ctx, npairs, 1024, 8)), dim3(256), 0, st,
-                           (const uint16_t*)pdst_w.p, npairs, (const uint32_t*)bst.p, (uint32_t)nb, pb->blk_off,
-                           pb->pdst);
+                           (const uint16_t*)pdst_w.p, npairs, (const uint32_t*)bst.p, (uint32_t)nb,
+                           (const uint32_t*)woffd.p, pb->pdst);
     PB_TRY(hipGetLastError());
     PB_TRY(hipMalloc((void**)&pb->slot_acc, ((uint64_t)(nslots ? nslots : 1) << wd_log) * sizeof(double)));
     PB_TRY(hipMalloc((void**)&pb->bin_ticket, B * sizeof(unsigned int)));
     PB_TRY(hipMemsetAsync(pb->bin_ticket, 0, B * sizeof(unsigned int), st));
     PB_TRY(hipMalloc((void**)&pb->bin_part, B * sizeof(double)));
-    PB_TRY(hipMalloc((void**)&pb->sink, (uint64_t)WG_PB_SINK_ROWS * WG_WAVE * sizeof(double)));
     PB_TRY(hipStreamSynchronize(st));
     return WG_OK;
 }
@@ -642,86 +622,42 @@ struct pb_or {
     }
 };
 
-// Row of the push's sink (WG_WAVE slots each) for this wavefront: its stores
-// before its first pair go there; wavefronts that share a row only ever store
-// there values nobody reads.
-__device__ __forceinline__ uint32_t pb_sink_row()
-{
-    return ((blockIdx.x * (uint32_t)WG_PB_WAVES + threadIdx.x / WG_WAVE) & (WG_PB_SINK_ROWS - 1u)) * WG_WAVE;
-}
-
-// A tile's inputs: the lane's codes, the tile's first pair number, the blocks
-// started before it and before the next tile, and the offsets of the 64
-// blocks from the one holding the tile's first pair (lane l: block
-// bcnt - 1 + l).  The per-tile words are scalar loads (t is uniform), so the
-// one dependent load, the offsets, waits on the scalar counter only.
+// A tile's inputs: the lane's codes and the tile's first pair number (a
+// scalar load: t is uniform).
 struct pb_tile_in {
     uint4 codes;
-    uint32_t pbase, bcnt, bnext, boff;
+    uint32_t pbase;
 };
 
 __device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ code,
-                                                   const uint32_t* __restrict__ tile_pbase,
-                                                   const uint32_t* __restrict__ tile_bcnt,
-                                                   const uint32_t* __restrict__ blk_off, uint32_t nblocks, uint32_t t,
-                                                   int lane)
+                                                   const uint32_t* __restrict__ tile_pbase, uint32_t t, int lane)
 {
     pb_tile_in in;
     t = __builtin_amdgcn_readfirstlane(t);
     in.codes = pb_load_codes(code, t, lane);
     in.pbase = tile_pbase[t];
-    in.bcnt = tile_bcnt[t];
-    in.bnext = tile_bcnt[t + 1];
-    if (in.bnext > nblocks || in.bnext < in.bcnt) // never on a layout pb_build wrote; keeps every read in bounds
-        in.bnext = in.bcnt;
-    uint32_t b = in.bcnt + (uint32_t)lane - 1u; // wraps to 2^32 - 1 for lane 0 of the first tile (never read)
-    if (b >= nblocks)
-        b = nblocks - 1u; // nblocks >= 1 whenever there are tiles
-    in.boff = blk_off[b];
     return in;
 }
 
-// Blocks a tile may touch beyond the 64 its lanes hold (a tile has at most
-// WG_PB_TILE pairs, so at most WG_PB_TILE + 1 blocks).
-#define WG_PB_OVF (WG_PB_TILE + 1u - WG_WAVE)
-
 // One tile of the push (one wavefront).  Lane l takes the tile's entries
 // [8l, 8l + 8): it combines its entries in order, and each entry flagged as a
-// pair's last emits that pair's value; the value still open at the lane's end
-// flows into the next lanes by a segmented inclusive scan (a lane with an end
-// starts a new segment).  A value goes to its position in bin order,
-// p + blk_off[block of p]: the first 64 blocks' offsets from the lanes that
-// loaded them, the rest (tiles of very sparse windows) from s_ovf, this
-// wavefront's LDS row, filled first.  Every lane stores at every entry: an
-// entry that ends no pair stores the lane's last emitted value again at the
-// same place (the sink before the first), so the number of stores per tile is
-// fixed and the compiler's waits for the next tile's loads never wait for
-// this tile's stores.
+// pair's last stores that pair's value at part[p]; the value still open at the
+// lane's end flows into the next lanes by a segmented inclusive scan (a lane
+// with an end starts a new segment).  A tile's pairs are consecutive, so its
+// stores fill one contiguous range of part[].
 template <class OP, typename W, class LOAD>
 __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, const W* s_win, uint32_t wend,
-                                             const uint32_t* __restrict__ blk_off, typename OP::V* __restrict__ part,
-                                             uint32_t* s_ovf, typename OP::V*& last_at, typename OP::V& last_v,
-                                             LOAD load)
+                                             typename OP::V* __restrict__ part, LOAD load)
 {
     typedef typename OP::V V;
     const int lane = wg_lane();
     const unsigned long long below = (1ull << lane) - 1ull;
-    uint32_t nb = in.bnext - in.bcnt + 1u; // blocks touched, counted from block bcnt - 1
-    if (nb > WG_PB_TILE + 1u)              // at most one block per pair (never more on a valid layout)
-        nb = WG_PB_TILE + 1u;
-    if (nb > (uint32_t)WG_WAVE) {          // the whole wavefront takes this branch
-        for (uint32_t i = (uint32_t)lane; i + WG_WAVE < nb; i += WG_WAVE)
-            s_ovf[i] = blk_off[in.bcnt - 1u + WG_WAVE + i];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
     const uint4 raw = in.codes;
     const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
                                    raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
     const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT;
     V v[WG_PB_IPT];
-    bool e[WG_PB_IPT], bs[WG_PB_IPT];
+    bool e[WG_PB_IPT];
     V run = OP::zero();
     bool has_end = false;
 #pragma unroll
@@ -729,7 +665,6 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
         const bool real = q0 + (uint32_t)j < wend;
         v[j] = real ? load(s_win[c[j] & WG_PB_SRC]) : OP::zero();
         e[j] = real && (c[j] & WG_PB_END);
-        bs[j] = real && (c[j] & WG_PB_BSTART);
         run = OP::op(run, v[j]);
         if (e[j]) {
             run = OP::zero();
@@ -752,29 +687,20 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
     V carry = OP::shfl_up(sc, 1);
     if (lane == 0)
         carry = OP::zero();
-    uint32_t pre = 0, bpre = 0;
+    uint32_t pre = 0;
 #pragma unroll
-    for (int j = 0; j < WG_PB_IPT; ++j) {
+    for (int j = 0; j < WG_PB_IPT; ++j)
         pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
-        bpre += (uint32_t)__popcll(wg_ballot(bs[j]) & below);
-    }
-    uint32_t p = in.pbase + pre; // window-order number of the lane's next pair
-    uint32_t k = bpre;           // its block - (in.bcnt - 1), before its own start flag
+    uint32_t p = in.pbase + pre; // the lane's next pair
     V r = carry;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
         r = OP::op(r, v[j]);
-        k += bs[j] ? 1u : 0u;
-        // the block's offset from the lane that loaded it (every lane takes part)
-        const uint32_t from_lane = (uint32_t)__shfl((int)in.boff, (int)(k & (WG_WAVE - 1)), WG_WAVE);
         if (e[j]) {
-            const uint32_t off = k < WG_WAVE ? from_lane : s_ovf[k - WG_WAVE];
-            last_at = part + (uint32_t)(p + off);
-            last_v = r;
+            part[p] = r;
             ++p;
             r = OP::zero();
         }
-        *last_at = last_v;
     }
 }
 
@@ -782,36 +708,26 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
 // per tile.  Two tiles' inputs alternate between two sets of registers, each
 // reloaded (index clamped, never skipped) as soon as its tile is done, so a
 // tile's loads are in flight while the other tile runs.  s_win holds the
-// window (filled by the caller, converted by LOAD on read); sink is this
-// wavefront's row of WG_WAVE slots for the stores before its first pair.
+// window (filled by the caller, converted by LOAD on read).
 template <class OP, typename W, class LOAD>
 __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, const uint4 tk,
                                               const uint32_t* __restrict__ tile_pbase,
-                                              const uint32_t* __restrict__ tile_bcnt,
-                                              const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                              typename OP::V* __restrict__ part, uint32_t* s_ovf,
-                                              typename OP::V* __restrict__ sink, LOAD load)
+                                              typename OP::V* __restrict__ part, LOAD load)
 {
-    typedef typename OP::V V;
     const int lane = wg_lane();
     uint32_t t = tk.y + (uint32_t)(threadIdx.x / WG_WAVE);
     if (t >= tk.z)
         return;
     const uint32_t tlast = tk.z - 1u;
-    V* last_at = sink + lane;
-    V last_v = OP::zero();
-    pb_tile_in a = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks, t, lane);
-    pb_tile_in b = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
-                                t + WG_PB_WAVES < tk.z ? t + WG_PB_WAVES : tlast, lane);
+    pb_tile_in a = pb_tile_load(code, tile_pbase, t, lane);
+    pb_tile_in b = pb_tile_load(code, tile_pbase, t + WG_PB_WAVES < tk.z ? t + WG_PB_WAVES : tlast, lane);
     for (;;) {
-        pb_push_tile<OP>(a, t, s_win, tk.w, blk_off, part, s_ovf, last_at, last_v, load);
-        a = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
-                         t + 2 * WG_PB_WAVES < tk.z ? t + 2 * WG_PB_WAVES : tlast, lane);
+        pb_push_tile<OP>(a, t, s_win, tk.w, part, load);
+        a = pb_tile_load(code, tile_pbase, t + 2 * WG_PB_WAVES < tk.z ? t + 2 * WG_PB_WAVES : tlast, lane);
         if (t + WG_PB_WAVES >= tk.z)
             break;
-        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, blk_off, part, s_ovf, last_at, last_v, load);
-        b = pb_tile_load(code, tile_pbase, tile_bcnt, blk_off, nblocks,
-                         t + 3 * WG_PB_WAVES < tk.z ? t + 3 * WG_PB_WAVES : tlast, lane);
+        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, part, load);
+        b = pb_tile_load(code, tile_pbase, t + 3 * WG_PB_WAVES < tk.z ? t + 3 * WG_PB_WAVES : tlast, lane);
         t += 2 * WG_PB_WAVES;
         if (t >= tk.z)
             break;
@@ -823,15 +739,11 @@ template <typename T, int WS>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __restrict__ code, const T* __restrict__ cin,
                                                            const uint4* __restrict__ task, uint32_t ntasks,
                                                            const uint32_t* __restrict__ tile_pbase,
-                                                           const uint32_t* __restrict__ tile_bcnt,
-                                                           const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                                           double* __restrict__ part, double* __restrict__ sink,
-                                                           uint32_t V)
+                                                           double* __restrict__ part, uint32_t V)
 {
     static_assert(WG_WAVE * WG_PB_IPT == WG_PB_TILE, "a tile is one wavefront's entries");
     static_assert(WS <= WG_PB_SRC + 1, "window larger than the source field");
     __shared__ T s_win[WS];
-    __shared__ uint32_t s_ovf[WG_PB_WAVES][WG_PB_OVF];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
@@ -839,58 +751,63 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
         __syncthreads(); // the previous task's wavefronts are done with s_win
         pb_fill_lds<WS>(s_win, cin + v0, nv, [](T x) { return x; });
         __syncthreads();
-        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
-                              s_ovf[threadIdx.x / WG_WAVE], sink + pb_sink_row(), [](T x) { return (double)x; });
+        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, part, [](T x) { return (double)x; });
     }
 }
 
-// The pull's stream over one slice, part[b0, b1): ADD(destination, partial)
-// for every pair.  Four pairs per load (8 bytes of destinations, 32 of
-// partials) and WG_PB_PULL_ROUNDS loads in flight per thread; the unaligned
-// ends of the range take one pair per thread.
+// The pull's pass over one slice, bin-order positions [b0, b1):
+// ADD(destination, partial) for every pair.  Position o of block kb holds pair
+// o - boff[kb] of part[] (window order).  Positions go 64 to a wavefront step
+// (chunk c = [64c, 64c + 64)): chunk_block[c] is the block holding 64c, the
+// block-start flags in pdst count the blocks after it, and the chunk's blocks'
+// offsets come from the lanes that loaded them.  WG_PB_PULL_ROUNDS chunks per
+// wavefront are in flight at a time.
 #define WG_PB_PULL_ROUNDS 4
-typedef double pb_f64x4 __attribute__((ext_vector_type(4)));
-typedef unsigned long long pb_u64x4 __attribute__((ext_vector_type(4)));
 
-template <typename X4, typename X, class ADD>
+template <typename X, class ADD>
 __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
+                                              const uint32_t* __restrict__ chunk_block,
+                                              const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs,
                                               uint32_t b0, uint32_t b1, ADD add)
 {
-    const uint32_t t = threadIdx.x;
-    uint32_t a0 = (b0 + 3u) & ~3u; // the aligned body [a0, a1)
-    if (a0 > b1)
-        a0 = b1;
-    uint32_t a1 = b1 & ~3u;
-    if (a1 < a0)
-        a1 = a0;
-    if (t < a0 - b0)
-        add(pdst[b0 + t], part[b0 + t]);
-    if (t < b1 - a1)
-        add(pdst[a1 + t], part[a1 + t]);
-    const uint32_t g1 = a1 >> 2;
-    const unsigned long long* __restrict__ pd4 = reinterpret_cast<const unsigned long long*>(pdst);
-    const X4* __restrict__ pp4 = reinterpret_cast<const X4*>(part);
-    for (uint32_t g = (a0 >> 2) + t; g < g1; g += WG_PB_THREADS * WG_PB_PULL_ROUNDS) {
-        unsigned long long dd[WG_PB_PULL_ROUNDS];
-        X4 xx[WG_PB_PULL_ROUNDS];
+    if (b0 >= b1)
+        return;
+    const int lane = wg_lane();
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull; // lanes 0..lane
+    const uint32_t wave = threadIdx.x / WG_WAVE;
+    const uint32_t c1 = ((b1 - 1u) >> 6) + 1u;
+    for (uint32_t c = (b0 >> 6) + wave; c < c1; c += WG_PB_WAVES * WG_PB_PULL_ROUNDS) {
+        uint32_t o[WG_PB_PULL_ROUNDS], pd[WG_PB_PULL_ROUNDS], bo[WG_PB_PULL_ROUNDS];
+        X x[WG_PB_PULL_ROUNDS];
 #pragma unroll
         for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r) {
-            const uint32_t gi = g + (uint32_t)r * WG_PB_THREADS;
-            dd[r] = 0ull;
-            xx[r] = X4(0);
-            if (gi < g1) {
-                dd[r] = __builtin_nontemporal_load(pd4 + gi);
-                xx[r] = __builtin_nontemporal_load(pp4 + gi);
+            const uint32_t cc = c + (uint32_t)r * WG_PB_WAVES; // uniform
+            o[r] = cc * 64u + (uint32_t)lane;
+            pd[r] = 0u;
+            bo[r] = 0u;
+            if (cc < c1) {
+                const uint32_t k0 = chunk_block[cc], k1 = chunk_block[cc + 1u];
+                const uint32_t nbc = k1 - k0 < (uint32_t)WG_WAVE ? k1 - k0 + 1u : (uint32_t)WG_WAVE;
+                if (o[r] < npairs)
+                    pd[r] = pdst[o[r]];
+                if ((uint32_t)lane < nbc && k0 + (uint32_t)lane < nblocks)
+                    bo[r] = boff[k0 + (uint32_t)lane];
             }
         }
 #pragma unroll
+        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r) {
+            // blocks started after the chunk's first position, up to this one
+            const uint32_t k =
+                (uint32_t)__popcll(wg_ballot(lane > 0 && (pd[r] & WG_PB_BFLAG)) & upto);
+            const uint32_t off = (uint32_t)__shfl((int)bo[r], (int)k, WG_WAVE);
+            x[r] = X(0);
+            if (o[r] >= b0 && o[r] < b1)
+                x[r] = __builtin_nontemporal_load(&part[o[r] - off]);
+        }
+#pragma unroll
         for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r)
-            if (g + (uint32_t)r * WG_PB_THREADS < g1) {
-                add((uint32_t)(dd[r] & 0xffffu), xx[r].x);
-                add((uint32_t)((dd[r] >> 16) & 0xffffu), xx[r].y);
-                add((uint32_t)((dd[r] >> 32) & 0xffffu), xx[r].z);
-                add((uint32_t)(dd[r] >> 48), xx[r].w);
-            }
+            if (o[r] >= b0 && o[r] < b1)
+                add(pd[r] & WG_PB_DMASK, x[r]);
     }
 }
 
@@ -940,7 +857,8 @@ __device__ __forceinline__ bool pb_join_slices(X* s_acc, uint32_t nv, uint32_t b
 // k_pb_total adds those.
 template <typename T, int MODE, int WD>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
-    const double* __restrict__ part, const uint16_t* __restrict__ pdst, const uint4* __restrict__ slice,
+    const double* __restrict__ part, const uint16_t* __restrict__ pdst, const uint32_t* __restrict__ chunk_block,
+    const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs, const uint4* __restrict__ slice,
     uint32_t nslices, const uint2* __restrict__ bin_slot, double* __restrict__ slot_acc,
     unsigned int* __restrict__ bin_ticket, double* __restrict__ bin_part, T* __restrict__ cout,
     double* __restrict__ acc_out, const uint32_t* __restrict__ od, uint32_t V, const double* __restrict__ S_in,
@@ -970,7 +888,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
         __syncthreads();
-        pb_pull_range<pb_f64x4>(part, pdst, sl.y, sl.z, [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
+        pb_pull_range(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
+                      [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
         __syncthreads();
         if (sl.w != WG_PB_NOSLOT &&
             !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
@@ -1027,8 +946,7 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     if (pb->ntasks) {
         const int slot = wg_prof_begin(ctx, WG_K_PB_PUSH);
         hipLaunchKernelGGL((k_pb_push<T, WS>), dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, cin,
-                           pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, pb->part,
-                           pb->sink, V);
+                           pb->task, pb->ntasks, pb->tile_pbase, pb->part, V);
         int rc = wg_check_launch(ctx);
         if (rc)
             return rc;
@@ -1037,11 +955,11 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     const int slot = wg_prof_begin(ctx, WG_K_PB_PULL);
     if (mode == 0)
         hipLaunchKernelGGL((k_pb_pull<T, 0, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
-                           pb->pdst, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
+                           pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
                            cout, nullptr, ctx->od, V, S_in, ctx->damping, N);
     else
         hipLaunchKernelGGL((k_pb_pull<T, 1, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
-                           pb->pdst, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
+                           pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
                            nullptr, ctx->acc, ctx->od, V, S_in, ctx->damping, N);
     int rc = wg_check_launch(ctx);
     if (rc)
@@ -1093,13 +1011,9 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
                                                               const unsigned long long* __restrict__ front,
                                                               unsigned long long live, const uint4* __restrict__ task,
                                                               uint32_t ntasks, const uint32_t* __restrict__ tile_pbase,
-                                                              const uint32_t* __restrict__ tile_bcnt,
-                                                              const uint32_t* __restrict__ blk_off, uint32_t nblocks,
-                                                              unsigned long long* __restrict__ part,
-                                                              double* __restrict__ sink, uint32_t V)
+                                                              unsigned long long* __restrict__ part, uint32_t V)
 {
     __shared__ unsigned long long s_win[WS];
-    __shared__ uint32_t s_ovf[WG_PB_WAVES][WG_PB_OVF];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
@@ -1107,9 +1021,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
         __syncthreads();
         pb_fill_lds<WS>(s_win, front + v0, nv, [live](unsigned long long x) { return x & live; });
         __syncthreads();
-        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, tile_bcnt, blk_off, nblocks, part,
-                             s_ovf[threadIdx.x / WG_WAVE], reinterpret_cast<unsigned long long*>(sink) + pb_sink_row(),
-                             [](unsigned long long x) { return x; });
+        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, part, [](unsigned long long x) { return x; });
     }
 }
 
@@ -1118,8 +1030,10 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
 // stores them (nonzero only) in nl.
 template <int WD>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
-    const unsigned long long* __restrict__ part, const uint16_t* __restrict__ pdst, const uint4* __restrict__ slice,
-    uint32_t nslices, const uint2* __restrict__ bin_slot, unsigned long long* __restrict__ slot_acc,
+    const unsigned long long* __restrict__ part, const uint16_t* __restrict__ pdst,
+    const uint32_t* __restrict__ chunk_block, const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs,
+    const uint4* __restrict__ slice, uint32_t nslices, const uint2* __restrict__ bin_slot,
+    unsigned long long* __restrict__ slot_acc,
     unsigned int* __restrict__ bin_ticket, const unsigned long long* __restrict__ seen, unsigned long long live,
     unsigned long long* __restrict__ nl, uint32_t V)
 {
@@ -1141,7 +1055,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0ull;
         __syncthreads();
-        pb_pull_range<pb_u64x4>(part, pdst, sl.y, sl.z, [&](uint32_t d, unsigned long long x) {
+        pb_pull_range(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z, [&](uint32_t d, unsigned long long x) {
             if (x)
                 atomicOr(&s_acc[d], x);
         });
@@ -1172,12 +1086,12 @@ int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsign
     unsigned long long* part = reinterpret_cast<unsigned long long*>(pb->part);
     if (pb->ntasks)
         hipLaunchKernelGGL(k_pb_push_or<16384>, dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, front,
-                           live, pb->task, pb->ntasks, pb->tile_pbase, pb->tile_bcnt, pb->blk_off, pb->nblocks, part,
-                           pb->sink, V);
+                           live, pb->task, pb->ntasks, pb->tile_pbase, part, V);
     if ((rc = wg_check_launch(ctx)))
         return rc;
     hipLaunchKernelGGL(k_pb_pull_or<16384>, dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
-                       (const unsigned long long*)part, pb->pdst, pb->slice, pb->nslices, pb->bin_slot,
+                       (const unsigned long long*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks,
+                       (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot,
                        reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, live, nl, V);
     return wg_check_launch(ctx);
 }
