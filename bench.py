@@ -516,6 +516,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-stored PageRank on the headline graph")
     ap.add_argument("--no-bfs", action="store_true", help="skip BFS on the headline graph")
     ap.add_argument("--no-pr", action="store_true", help="skip PageRank on the headline graph (profiling BFS alone)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the headline graph (profiling the other sections alone; no headline line)")
     args = ap.parse_args()
     args.pr_shape = tuple(int(x) for x in args.pr_shape.split(",")) if args.pr_shape else None
 
@@ -527,6 +529,14 @@ def main():
     copy_gbs = None
     with W.Context(0) as c:
         copy_gbs = round(c.measure_copy(4 << 30), 1)
+    if args.no_large:  # a profiling run of the other sections: no headline line
+        out = {"measured_copy_GBs": copy_gbs}
+        if not args.no_s20:
+            out["pagerank_s20"] = pagerank_s20(cpu=not args.no_cpu)
+        if not args.no_batch:
+            out["batched"] = {"graphs_256": batched_graphs(args.damping)}
+        print(json.dumps(out), flush=True)
+        return
     g27, large, cpu = large_graph(args)
     del g27
     if args.no_pr:  # BFS alone (a profiling run): no headline line

@@ -610,17 +610,47 @@ struct pb_sum {
     typedef double V;
     static __device__ __forceinline__ V zero() { return 0.0; }
     static __device__ __forceinline__ V op(V a, V b) { return a + b; }
-    static __device__ __forceinline__ V shfl_up(V x, int d) { return __shfl_up(x, d, WG_WAVE); }
+    static __device__ __forceinline__ unsigned long long bits(V x) { return (unsigned long long)__double_as_longlong(x); }
+    static __device__ __forceinline__ V from_bits(unsigned long long x) { return __longlong_as_double((long long)x); }
 };
 struct pb_or {
     typedef unsigned long long V;
     static __device__ __forceinline__ V zero() { return 0ull; }
     static __device__ __forceinline__ V op(V a, V b) { return a | b; }
-    static __device__ __forceinline__ V shfl_up(V x, int d)
-    {
-        return (unsigned long long)__shfl_up((long long)x, d, WG_WAVE);
-    }
+    static __device__ __forceinline__ unsigned long long bits(V x) { return x; }
+    static __device__ __forceinline__ V from_bits(unsigned long long x) { return x; }
 };
+
+// Cross-lane moves for the push's segmented scan, as DPP controls (GFX9
+// encodings): row_shr:n shifts within rows of 16 lanes, row_bcast:15 / :31
+// hand the last lane of a row (of two rows) to the next row(s), wave_shr:1
+// shifts the whole wavefront by one lane.  A lane with no source (or a row
+// left out by the row mask) receives 0, the identity of both combines.
+#define WG_DPP_ROW_SHR(n) (0x110 + (n))
+#define WG_DPP_ROW_BCAST15 0x142
+#define WG_DPP_ROW_BCAST31 0x143
+#define WG_DPP_WAVE_SHR1 0x138
+
+template <class OP, int CTRL, int ROWS>
+__device__ __forceinline__ typename OP::V pb_dpp_v(typename OP::V v)
+{
+    const unsigned long long x = OP::bits(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, ROWS, 0xf, false);
+    return OP::from_bits(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// One step of the segmented inclusive scan: (v, f) of this lane combined
+// with the partial result CTRL brings from a lower lane (a set flag starts a
+// new segment, so a lane with f keeps its own value).
+template <class OP, int CTRL, int ROWS>
+__device__ __forceinline__ void pb_seg_dpp(typename OP::V& v, int& f)
+{
+    const typename OP::V o = pb_dpp_v<OP, CTRL, ROWS>(v);
+    const int of = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, false);
+    if (!f)
+        v = OP::op(o, v);
+    f |= of;
+}
 
 // A tile's inputs: the lane's codes and the tile's first pair number (a
 // scalar load: t is uniform).
@@ -671,22 +701,19 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
             has_end = true;
         }
     }
-    // segmented inclusive scan of the open values over the lanes
+    // segmented inclusive scan of the open values over the lanes: shifts of
+    // 1, 2, 4, 8 within each row of 16, then the row totals into rows 1 and 3,
+    // then the first two rows' total into rows 2 and 3; carry = the scan one
+    // lane down (the value flowing into this lane's first entry)
     V sc = run;
     int f = has_end ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < WG_WAVE; d <<= 1) {
-        const V o = OP::shfl_up(sc, d);
-        const int of = __shfl_up(f, d, WG_WAVE);
-        if (lane >= d) {
-            if (!f)
-                sc = OP::op(o, sc);
-            f |= of;
-        }
-    }
-    V carry = OP::shfl_up(sc, 1);
-    if (lane == 0)
-        carry = OP::zero();
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(sc, f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(sc, f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(sc, f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(sc, f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(sc, f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(sc, f);
+    const V carry = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(sc);
     uint32_t pre = 0;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j)
@@ -762,7 +789,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
 // block-start flags in pdst count the blocks after it, and the chunk's blocks'
 // offsets come from the lanes that loaded them.  WG_PB_PULL_ROUNDS chunks per
 // wavefront are in flight at a time.
-#define WG_PB_PULL_ROUNDS 4
+#define WG_PB_PULL_ROUNDS 8
 
 template <typename X, class ADD>
 __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
