@@ -62,22 +62,23 @@ def pr_algorithmic_bytes(V: int, E: int, fp32: bool) -> int:
     return 4 * E + 4 * (V + 1) + 4 * V + 2 * c * V
 
 
-def traversal_bytes(app: str, V: int, st: dict, work: tuple[int, int], wbytes: int = 0) -> int:
+def traversal_bytes(app: str, V: int, st: dict, work: tuple[int, int], wbytes: int = 0, fused: int = 0) -> int:
     """Bytes a traversal must move at minimum, from the work the kernels
     counted: per edge read, its 4-byte source (plus its weight for SSSP); per
     destination state examined, that state (BFS depth/parent word 8 B, SSSP
-    distance 8 B FP64 or 4 B integer, CC label 4 B); per iteration one read of
-    the current frontier bitmap and one write of the next (2 * V/8); BFS also
+    distance 8 B FP64 or 4 B integer, CC label 4 B); per iteration that is not
+    a fused level (those pass their frontier in queues) one read of the
+    current frontier bitmap and one write of the next (2 * V/8); BFS also
     writes 8 B per newly reached vertex."""
     edges, dests = work
     state = {"bfs": 8, "cc": 4}.get(app, 8 if wbytes == 8 else 4)
-    b = (4 + wbytes) * edges + state * dests + st["iterations"] * 2 * ((V + 7) // 8)
+    b = (4 + wbytes) * edges + state * dests + (st["iterations"] - fused) * 2 * ((V + 7) // 8)
     if app == "bfs":
         b += 8 * st["vertices_reached"]
     return b
 
 
-def measured_traffic(name: str):
+def measured_traffic(name: str, key: str = "traffic_bytes_per_launch"):
     """HBM bytes per launch from a committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE passes, scaled by a copy kernel of known size),
     or None when no summary covers the workload."""
@@ -85,7 +86,7 @@ def measured_traffic(name: str):
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return int(line.split(":", 1)[1])
     return None
 
@@ -156,15 +157,15 @@ def run_best(ctx, app, root, reps, **kw):
     for _ in range(reps):
         res, st = run()
         if best is None or st["running_ms"] < best[1]["running_ms"]:
-            best = (res, st)
+            best = (res, st, ctx.fused_level_count())
     work, it = counted_run(ctx, run)
     if it != best[1]["iterations"]:
         log(f"{app} {kw}: the counted run took {it} iterations, the timed one {best[1]['iterations']}")
-    return best[0], best[1], work
+    return best[0], best[1], work, best[2]
 
 
-def traversal_entry(app, V, st, work, wbytes=0, violations=None):
-    b = traversal_bytes(app, V, st, work, wbytes)
+def traversal_entry(app, V, st, work, wbytes=0, violations=None, fused=0):
+    b = traversal_bytes(app, V, st, work, wbytes, fused)
     e = {"ms": round(st["running_ms"], 3), "iterations": st["iterations"],
          "frontier_iterations": st["frontier_iterations"],
          "gteps": round(st["edges_reached"] / (st["running_ms"] * 1e-3) / 1e9, 3),
@@ -184,11 +185,11 @@ def schedule_pair(ctx, g, root, reps, wbytes, orc=None):
     for app in ("bfs", "sssp"):
         res = {}
         for sched in ("plain", "learned"):
-            r, st, work = run_best(ctx, app, root, reps, schedule=sched)
+            r, st, work, fused = run_best(ctx, app, root, reps, schedule=sched)
             res[sched] = (r, st)
             out[f"{app}_{sched}"] = traversal_entry(app, g.num_vertices, st, work, 0 if app == "bfs" else wbytes,
-                                                    ctx.validate())
-            out[f"{app}_{sched}"]["fused_levels"] = ctx.fused_level_count()
+                                                    ctx.validate(), fused)
+            out[f"{app}_{sched}"]["fused_levels"] = fused
         out[f"{app}_learned_vs_plain"] = round(res["plain"][1]["running_ms"] / res["learned"][1]["running_ms"], 3)
         out[f"{app}_identical"] = bool(np.array_equal(res["plain"][0], res["learned"][0]))
         if orc is not None:
@@ -402,7 +403,9 @@ def large_graph(args):
         wall, kern, shape, times = pr_timed(ctx, args.damping, fp32, args.steps if not fp32 else 5, args.warmup,
                                             args.pr_shape)
         alg = pr_algorithmic_bytes(V, E, fp32)
-        traffic = measured_traffic(f"pr_rmat_s27_{'fp32_' if fp32 else ''}hbm_traffic.md")
+        traffic = (measured_traffic("k_pb_rmat_s27_hbm_traffic.md",
+                                    f"traffic_bytes_per_launch_{'float' if fp32 else 'double'}")
+                   if shape[2] == W.PR_BINNED else None)
         key = "pr_fp32" if fp32 else "pr_fp64"
         res[key] = {"ms_per_iteration": round(wall, 4), "gteps": round(E / (wall * 1e-3) / 1e9, 3),
                     "shape": {"edges_per_lane": shape[0], "blocks_per_cu": shape[1], "kind": shape_name(shape)},
@@ -423,9 +426,10 @@ def large_graph(args):
     W.mlp_load()
     bfs = {}
     for sched in ("plain", "learned") if not args.no_bfs else ():
-        r, st, work = run_best(ctx, "bfs", root, 3, schedule=sched)
+        r, st, work, fused = run_best(ctx, "bfs", root, 3, schedule=sched)
         bfs[sched] = r
-        out[f"bfs_{sched}"] = traversal_entry("bfs", V, st, work, violations=ctx.validate())
+        out[f"bfs_{sched}"] = traversal_entry("bfs", V, st, work, violations=ctx.validate(), fused=fused)
+        out[f"bfs_{sched}"]["fused_levels"] = fused
         log(f"s27 bfs {sched}: {out[f'bfs_{sched}']}")
     if not args.no_bfs:
         out["bfs_learned_vs_plain"] = round(out["bfs_plain"]["ms"] / out["bfs_learned"]["ms"], 3)
@@ -498,7 +502,7 @@ def pagerank_s20(damping_values=(0.85, 0.875), iterations=16, cpu=True):
     return out
 
 
-def main():
+def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -520,28 +524,17 @@ def main():
                     help="skip the headline graph (profiling the other sections alone; no headline line)")
     args = ap.parse_args()
     args.pr_shape = tuple(int(x) for x in args.pr_shape.split(",")) if args.pr_shape else None
+    return args
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
 
-    copy_gbs = None
-    with W.Context(0) as c:
-        copy_gbs = round(c.measure_copy(4 << 30), 1)
-    if args.no_large:  # a profiling run of the other sections: no headline line
-        out = {"measured_copy_GBs": copy_gbs}
-        if not args.no_s20:
-            out["pagerank_s20"] = pagerank_s20(cpu=not args.no_cpu)
-        if not args.no_batch:
-            out["batched"] = {"graphs_256": batched_graphs(args.damping)}
-        print(json.dumps(out), flush=True)
-        return
+def headline(args, copy_gbs):
+    """The headline line (rmat-s27 PageRank) and its large-graph section, or
+    None for a BFS-only profiling run (which prints that section alone)."""
     g27, large, cpu = large_graph(args)
     del g27
-    if args.no_pr:  # BFS alone (a profiling run): no headline line
+    if args.no_pr:
         print(json.dumps({"large": large}), flush=True)
-        return
+        return None
     pr = large["pr_fp64"]
     out = {
         "metric": METRIC,
@@ -567,6 +560,25 @@ def main():
     if cpu is not None:
         out["cpu_baseline"] = cpu
     out["large"] = large
+    return out
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    copy_gbs = None
+    with W.Context(0) as c:
+        copy_gbs = round(c.measure_copy(4 << 30), 1)
+    if args.no_large:  # a profiling run of the other sections: no headline line
+        out = {"measured_copy_GBs": copy_gbs}
+    else:
+        out = headline(args, copy_gbs)
+        if out is None:
+            return
     if not args.no_s20:
         out["pagerank_s20"] = pagerank_s20(cpu=not args.no_cpu)
 

@@ -71,6 +71,7 @@
 #define WG_PB_WAVES (WG_PB_THREADS / WG_WAVE)
 #define WG_PB_TILE 512u                         // code entries per wavefront tile
 #define WG_PB_TASK_TILES 1024u                  // tiles per push workgroup (512K edges)
+#define WG_PB_STAGE 256u                        // partials a push wavefront stages in LDS
 #define WG_PB_SLICE (256u * 1024u)              // pairs per pull workgroup (default)
 #define WG_PB_NOSLOT 0xffffffffu
 #define WG_PB_END 0x8000u                       // code: last edge of a pair
@@ -674,10 +675,14 @@ __device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ 
 // pair's last stores that pair's value at part[p]; the value still open at the
 // lane's end flows into the next lanes by a segmented inclusive scan (a lane
 // with an end starts a new segment).  A tile's pairs are consecutive, so its
-// stores fill one contiguous range of part[].
+// stores fill one contiguous range of part[]: a tile of at most WG_PB_STAGE
+// pairs passes them through s_stage, this wavefront's LDS row, so that each
+// store instruction writes 64 consecutive partials (one lane's pairs are
+// consecutive, 64 lanes' are not: stored directly, each 8-byte store was its
+// own L2 request).
 template <class OP, typename W, class LOAD>
 __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, const W* s_win, uint32_t wend,
-                                             typename OP::V* __restrict__ part, LOAD load)
+                                             typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
     typedef typename OP::V V;
     const int lane = wg_lane();
@@ -714,12 +719,35 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
     pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(sc, f);
     pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(sc, f);
     const V carry = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(sc);
-    uint32_t pre = 0;
+    uint32_t pre = 0, all = 0;
 #pragma unroll
-    for (int j = 0; j < WG_PB_IPT; ++j)
-        pre += (uint32_t)__popcll(wg_ballot(e[j]) & below);
-    uint32_t p = in.pbase + pre; // the lane's next pair
+    for (int j = 0; j < WG_PB_IPT; ++j) {
+        const unsigned long long m = wg_ballot(e[j]);
+        pre += (uint32_t)__popcll(m & below);
+        all += (uint32_t)__popcll(m);
+    }
     V r = carry;
+    if (all <= WG_PB_STAGE) { // the whole wavefront: values to the LDS row in pair order, then coalesced stores
+        uint32_t i = pre;
+#pragma unroll
+        for (int j = 0; j < WG_PB_IPT; ++j) {
+            r = OP::op(r, v[j]);
+            if (e[j]) {
+                s_stage[i++] = r;
+                r = OP::zero();
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t k = (uint32_t)lane; k < all; k += WG_WAVE)
+            part[in.pbase + k] = s_stage[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // reads done before the next tile's writes
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        return;
+    }
+    uint32_t p = in.pbase + pre; // the lane's next pair
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
         r = OP::op(r, v[j]);
@@ -739,7 +767,7 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
 template <class OP, typename W, class LOAD>
 __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, const uint4 tk,
                                               const uint32_t* __restrict__ tile_pbase,
-                                              typename OP::V* __restrict__ part, LOAD load)
+                                              typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
     const int lane = wg_lane();
     uint32_t t = tk.y + (uint32_t)(threadIdx.x / WG_WAVE);
@@ -749,11 +777,11 @@ __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code,
     pb_tile_in a = pb_tile_load(code, tile_pbase, t, lane);
     pb_tile_in b = pb_tile_load(code, tile_pbase, t + WG_PB_WAVES < tk.z ? t + WG_PB_WAVES : tlast, lane);
     for (;;) {
-        pb_push_tile<OP>(a, t, s_win, tk.w, part, load);
+        pb_push_tile<OP>(a, t, s_win, tk.w, part, s_stage, load);
         a = pb_tile_load(code, tile_pbase, t + 2 * WG_PB_WAVES < tk.z ? t + 2 * WG_PB_WAVES : tlast, lane);
         if (t + WG_PB_WAVES >= tk.z)
             break;
-        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, part, load);
+        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, part, s_stage, load);
         b = pb_tile_load(code, tile_pbase, t + 3 * WG_PB_WAVES < tk.z ? t + 3 * WG_PB_WAVES : tlast, lane);
         t += 2 * WG_PB_WAVES;
         if (t >= tk.z)
@@ -771,6 +799,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
     static_assert(WG_WAVE * WG_PB_IPT == WG_PB_TILE, "a tile is one wavefront's entries");
     static_assert(WS <= WG_PB_SRC + 1, "window larger than the source field");
     __shared__ T s_win[WS];
+    __shared__ double s_stage[WG_PB_WAVES][WG_PB_STAGE];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
@@ -778,7 +807,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
         __syncthreads(); // the previous task's wavefronts are done with s_win
         pb_fill_lds<WS>(s_win, cin + v0, nv, [](T x) { return x; });
         __syncthreads();
-        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, part, [](T x) { return (double)x; });
+        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
+                              [](T x) { return (double)x; });
     }
 }
 
@@ -1041,6 +1071,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
                                                               unsigned long long* __restrict__ part, uint32_t V)
 {
     __shared__ unsigned long long s_win[WS];
+    __shared__ unsigned long long s_stage[WG_PB_WAVES][WG_PB_STAGE];
     for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
         const uint4 tk = task[k];
         const uint32_t v0 = tk.x * (uint32_t)WS;
@@ -1048,7 +1079,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
         __syncthreads();
         pb_fill_lds<WS>(s_win, front + v0, nv, [live](unsigned long long x) { return x & live; });
         __syncthreads();
-        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, part, [](unsigned long long x) { return x; });
+        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
+                             [](unsigned long long x) { return x; });
     }
 }
 
