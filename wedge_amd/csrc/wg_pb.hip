@@ -670,69 +670,98 @@ __device__ __forceinline__ pb_tile_in pb_tile_load(const uint16_t* __restrict__ 
     return in;
 }
 
-// One tile of the push (one wavefront).  Lane l takes the tile's entries
-// [8l, 8l + 8): it combines its entries in order, and each entry flagged as a
-// pair's last stores that pair's value at part[p]; the value still open at the
-// lane's end flows into the next lanes by a segmented inclusive scan (a lane
-// with an end starts a new segment).  A tile's pairs are consecutive, so its
-// stores fill one contiguous range of part[]: a tile of at most WG_PB_STAGE
-// pairs passes them through s_stage, this wavefront's LDS row, so that each
-// store instruction writes 64 consecutive partials (one lane's pairs are
-// consecutive, 64 lanes' are not: stored directly, each 8-byte store was its
-// own L2 request).
+// One lane's share of a push tile in flight: its entries' values and end
+// flags, then its open run and the run's segmented scan.
+template <class OP>
+struct pb_lane_tile {
+    typename OP::V v[WG_PB_IPT];
+    bool e[WG_PB_IPT];
+    typename OP::V sc;
+    int f;
+};
+
+// Tile phase 1 (one wavefront): lane l takes the tile's entries [8l, 8l + 8)
+// and combines them in order; an entry flagged as a pair's last closes a pair,
+// and the value still open at the lane's end is its run.  Every entry reads
+// the window (padding entries read slot 0), so the pass has no branches and
+// two tiles' passes interleave.  An invalid tile (past the task's end) has no
+// entries.
 template <class OP, typename W, class LOAD>
-__device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, const W* s_win, uint32_t wend,
-                                             typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
+__device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, bool valid, const W* s_win,
+                                             uint32_t wmask, uint32_t wend, pb_lane_tile<OP>& w, LOAD load)
 {
     typedef typename OP::V V;
-    const int lane = wg_lane();
-    const unsigned long long below = (1ull << lane) - 1ull;
     const uint4 raw = in.codes;
     const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
                                    raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
-    const uint32_t q0 = t * WG_PB_TILE + (uint32_t)lane * WG_PB_IPT;
-    V v[WG_PB_IPT];
-    bool e[WG_PB_IPT];
+    const uint32_t q0 = t * WG_PB_TILE + (uint32_t)wg_lane() * WG_PB_IPT;
     V run = OP::zero();
     bool has_end = false;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        const bool real = q0 + (uint32_t)j < wend;
-        v[j] = real ? load(s_win[c[j] & WG_PB_SRC]) : OP::zero();
-        e[j] = real && (c[j] & WG_PB_END);
-        run = OP::op(run, v[j]);
-        if (e[j]) {
-            run = OP::zero();
-            has_end = true;
-        }
+        const V x = load(s_win[c[j] & wmask]);
+        const bool real = valid && q0 + (uint32_t)j < wend;
+        w.v[j] = real ? x : OP::zero();
+        w.e[j] = real && (c[j] & WG_PB_END);
+        run = OP::op(run, w.v[j]);
+        run = w.e[j] ? OP::zero() : run;
+        has_end = has_end || w.e[j];
     }
-    // segmented inclusive scan of the open values over the lanes: shifts of
-    // 1, 2, 4, 8 within each row of 16, then the row totals into rows 1 and 3,
-    // then the first two rows' total into rows 2 and 3; carry = the scan one
-    // lane down (the value flowing into this lane's first entry)
-    V sc = run;
-    int f = has_end ? 1 : 0;
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(sc, f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(sc, f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(sc, f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(sc, f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(sc, f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(sc, f);
-    const V carry = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(sc);
+    w.sc = run;
+    w.f = has_end ? 1 : 0;
+}
+
+// Tile phase 2 for two tiles at once: the segmented inclusive scan of the
+// lanes' runs (a lane with an end starts a new segment): shifts of 1, 2, 4, 8
+// within each row of 16, then the row totals into rows 1 and 3, then the
+// first two rows' total into rows 2 and 3.  Each tile's sc becomes its carry,
+// the scan one lane down (the value flowing into the lane's first entry).
+template <class OP>
+__device__ __forceinline__ void pb_tile_scan2(pb_lane_tile<OP>& a, pb_lane_tile<OP>& b)
+{
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(b.sc, b.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(b.sc, b.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(b.sc, b.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(b.sc, b.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(b.sc, b.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(a.sc, a.f);
+    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(b.sc, b.f);
+    a.sc = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(a.sc);
+    b.sc = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(b.sc);
+}
+
+// Tile phase 3: each entry flagged as a pair's last stores that pair's value
+// at part[p].  A tile's pairs are consecutive, so its stores fill one
+// contiguous range of part[]: a tile of at most WG_PB_STAGE pairs passes them
+// through s_stage, this wavefront's LDS row, so that each store instruction
+// writes 64 consecutive partials (one lane's pairs are consecutive, 64
+// lanes' are not: stored directly, each 8-byte store is its own L2 request).
+template <class OP>
+__device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lane_tile<OP>& w,
+                                              typename OP::V* __restrict__ part, typename OP::V* s_stage)
+{
+    typedef typename OP::V V;
+    const int lane = wg_lane();
+    const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t pre = 0, all = 0;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        const unsigned long long m = wg_ballot(e[j]);
+        const unsigned long long m = wg_ballot(w.e[j]);
         pre += (uint32_t)__popcll(m & below);
         all += (uint32_t)__popcll(m);
     }
-    V r = carry;
+    V r = w.sc;
     if (all <= WG_PB_STAGE) { // the whole wavefront: values to the LDS row in pair order, then coalesced stores
         uint32_t i = pre;
 #pragma unroll
         for (int j = 0; j < WG_PB_IPT; ++j) {
-            r = OP::op(r, v[j]);
-            if (e[j]) {
+            r = OP::op(r, w.v[j]);
+            if (w.e[j]) {
                 s_stage[i++] = r;
                 r = OP::zero();
             }
@@ -750,8 +779,8 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
     uint32_t p = in.pbase + pre; // the lane's next pair
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        r = OP::op(r, v[j]);
-        if (e[j]) {
+        r = OP::op(r, w.v[j]);
+        if (w.e[j]) {
             part[p] = r;
             ++p;
             r = OP::zero();
@@ -759,31 +788,51 @@ __device__ __forceinline__ void pb_push_tile(const pb_tile_in& in, uint32_t t, c
     }
 }
 
-// Push body: one workgroup per task (a window's run of tiles), one wavefront
-// per tile.  Two tiles' inputs alternate between two sets of registers, each
-// reloaded (index clamped, never skipped) as soon as its tile is done, so a
-// tile's loads are in flight while the other tile runs.  s_win holds the
-// window (filled by the caller, converted by LOAD on read).
+// Two tiles of one wavefront, their phases interleaved (independent
+// instructions for the scheduler: a wavefront's tile is a chain of dependent
+// steps, and a CU holds only the 16 wavefronts of one workgroup).
 template <class OP, typename W, class LOAD>
-__device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, const uint4 tk,
-                                              const uint32_t* __restrict__ tile_pbase,
+__device__ __forceinline__ void pb_push_pair(const pb_tile_in& x, uint32_t tx, const pb_tile_in& y, uint32_t ty,
+                                             bool y_valid, const W* s_win, uint32_t wmask, uint32_t wend,
+                                             typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
+{
+    pb_lane_tile<OP> a, b;
+    pb_tile_read<OP>(x, tx, true, s_win, wmask, wend, a, load);
+    pb_tile_read<OP>(y, ty, y_valid, s_win, wmask, wend, b, load);
+    pb_tile_scan2<OP>(a, b);
+    pb_tile_store<OP>(x, a, part, s_stage);
+    pb_tile_store<OP>(y, b, part, s_stage);
+}
+
+// Push body: one workgroup per task (a window's run of tiles), one wavefront
+// per tile pair.  Four tiles' inputs rotate through four sets of registers,
+// each pair reloaded (index clamped, never skipped) as soon as it is done, so
+// one pair's loads are in flight while the other pair runs.  s_win holds the
+// window (filled by the caller, converted by LOAD on read; wmask = its size
+// - 1).
+template <class OP, typename W, class LOAD>
+__device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, uint32_t wmask,
+                                              const uint4 tk, const uint32_t* __restrict__ tile_pbase,
                                               typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
+    constexpr uint32_t S = WG_PB_WAVES; // tiles between one wavefront's consecutive tiles
     const int lane = wg_lane();
     uint32_t t = tk.y + (uint32_t)(threadIdx.x / WG_WAVE);
     if (t >= tk.z)
         return;
     const uint32_t tlast = tk.z - 1u;
-    pb_tile_in a = pb_tile_load(code, tile_pbase, t, lane);
-    pb_tile_in b = pb_tile_load(code, tile_pbase, t + WG_PB_WAVES < tk.z ? t + WG_PB_WAVES : tlast, lane);
+    auto ld = [&](uint32_t u) { return pb_tile_load(code, tile_pbase, u < tk.z ? u : tlast, lane); };
+    pb_tile_in a = ld(t), b = ld(t + S), c = ld(t + 2 * S), d = ld(t + 3 * S);
     for (;;) {
-        pb_push_tile<OP>(a, t, s_win, tk.w, part, s_stage, load);
-        a = pb_tile_load(code, tile_pbase, t + 2 * WG_PB_WAVES < tk.z ? t + 2 * WG_PB_WAVES : tlast, lane);
-        if (t + WG_PB_WAVES >= tk.z)
+        pb_push_pair<OP>(a, t, b, t + S, t + S < tk.z, s_win, wmask, tk.w, part, s_stage, load);
+        a = ld(t + 4 * S);
+        b = ld(t + 5 * S);
+        if (t + 2 * S >= tk.z)
             break;
-        pb_push_tile<OP>(b, t + WG_PB_WAVES, s_win, tk.w, part, s_stage, load);
-        b = pb_tile_load(code, tile_pbase, t + 3 * WG_PB_WAVES < tk.z ? t + 3 * WG_PB_WAVES : tlast, lane);
-        t += 2 * WG_PB_WAVES;
+        pb_push_pair<OP>(c, t + 2 * S, d, t + 3 * S, t + 3 * S < tk.z, s_win, wmask, tk.w, part, s_stage, load);
+        c = ld(t + 6 * S);
+        d = ld(t + 7 * S);
+        t += 4 * S;
         if (t >= tk.z)
             break;
     }
@@ -807,7 +856,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
         __syncthreads(); // the previous task's wavefronts are done with s_win
         pb_fill_lds<WS>(s_win, cin + v0, nv, [](T x) { return x; });
         __syncthreads();
-        pb_push_tiles<pb_sum>(code, s_win, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
+        pb_push_tiles<pb_sum>(code, s_win, (uint32_t)WS - 1u, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
                               [](T x) { return (double)x; });
     }
 }
@@ -1079,7 +1128,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
         __syncthreads();
         pb_fill_lds<WS>(s_win, front + v0, nv, [live](unsigned long long x) { return x & live; });
         __syncthreads();
-        pb_push_tiles<pb_or>(code, s_win, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
+        pb_push_tiles<pb_or>(code, s_win, (uint32_t)WS - 1u, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
                              [](unsigned long long x) { return x; });
     }
 }
