@@ -360,8 +360,13 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             const bool dense = params->schedule == 2 || (params->schedule != 3 && act > threshold);
             if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
                 goto out;
-            if (dense && binned) {
-                if ((rc = wg_pb_msbfs_level(ctx, dfront, dseen, dnl, live)))
+            if (level + 1 > MS_LEVEL_MAX) {
+                rc = WG_E_OVERFLOW; // depths are stored in 16 bits
+                goto out;
+            }
+            const bool committed = dense && binned; // the binned pull commits the level itself
+            if (committed) {
+                if ((rc = wg_pb_msbfs_level(ctx, dfront, dseen, (uint16_t*)depth, live, level + 1)))
                     goto out;
             } else if (dense) {
                 if (fverts < (uint64_t)V / 8)
@@ -380,12 +385,9 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             }
             if ((rc = wg_check_launch(ctx)))
                 goto out;
-            if (level + 1 > MS_LEVEL_MAX) {
-                rc = WG_E_OVERFLOW; // depths are stored in 16 bits
-                goto out;
-            }
-            hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen, dfront,
-                               (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
+            if (!committed)
+                hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen, dfront,
+                                   (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
             if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
                 goto out;
             ++level;

@@ -360,9 +360,11 @@ int wg_build_pr_tilings(wg_ctx* ctx);
 int wg_pb_iteration(wg_ctx* ctx, int mode, const void* cin, void* cout, double* S_in, double* S_out);
 int wg_pb_prepare(wg_ctx* ctx);
 int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* tasks, uint32_t* slices);
-// a dense level of the 64-source BFS on the binned layout: nl[v] = (OR of the
-// in-neighbours' front & live) & ~seen[v] where nonzero (other entries untouched)
-int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsigned long long* seen,
-                      unsigned long long* nl, unsigned long long live);
+// a dense level of the 64-source BFS on the binned layout, committed: with
+// n[v] = (OR of the in-neighbours' front & live) & ~seen[v], seen |= n,
+// front = n, depth entries of the new sources = level, the frontier vertex
+// bitmask (bm_next) and the level's counts in ctx->ctr
+int wg_pb_msbfs_level(wg_ctx* ctx, unsigned long long* front, unsigned long long* seen, uint16_t* depth,
+                      unsigned long long live, uint32_t level);
 // releases the binned layout (graph change)
 void wg_free_pb(wg_ctx* ctx);

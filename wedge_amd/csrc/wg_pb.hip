@@ -60,7 +60,7 @@
 //   bin_slot[b]    {first slot, slices} of a bin cut into several slices: each
 //                  slice writes its accumulators to its slot, and the last one
 //                  to finish (a per-bin ticket) adds the slots in slice order
-#include "wg_internal.h"
+#include "wg_frontier.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -1134,30 +1134,38 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
 }
 
 // Pull: ORs a slice's masks into the bin's LDS words; the workgroup holding a
-// bin's complete words keeps, per vertex, the live sources not seen yet and
-// stores them (nonzero only) in nl.
+// bin's complete words then commits the level for the bin's vertices (what
+// k_ms_commit does after a frontier level): n = the live sources not seen yet,
+// seen |= n, front = n, the depth row entries of the new sources, the frontier
+// vertex bitmask, and the counts (out-degree and number of vertices with new
+// sources, new (vertex, source) pairs).
 template <int WD>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
     const unsigned long long* __restrict__ part, const uint16_t* __restrict__ pdst,
     const uint32_t* __restrict__ chunk_block, const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs,
     const uint4* __restrict__ slice, uint32_t nslices, const uint2* __restrict__ bin_slot,
-    unsigned long long* __restrict__ slot_acc,
-    unsigned int* __restrict__ bin_ticket, const unsigned long long* __restrict__ seen, unsigned long long live,
-    unsigned long long* __restrict__ nl, uint32_t V)
+    unsigned long long* __restrict__ slot_acc, unsigned int* __restrict__ bin_ticket,
+    unsigned long long* __restrict__ seen, unsigned long long* __restrict__ front, uint16_t* __restrict__ depth,
+    uint64_t* __restrict__ bm, const uint32_t* __restrict__ od, unsigned long long live, uint32_t level,
+    wg_dev_counters* __restrict__ ctr, uint32_t V)
 {
     __shared__ unsigned long long s_acc[WD];
     __shared__ int s_flag;
+    const int lane = wg_lane();
+    act_tally t = { 0ull, 0ull, 0ull };
     for (uint32_t k = blockIdx.x; k < nslices; k += gridDim.x) {
         const uint4 sl = slice[k];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
-        // the bin's seen masks for the epilogue, loaded while the slice streams
+        // the bin's seen masks and out-degrees for the commit, loaded while the slice streams
         constexpr int R = WD / WG_PB_THREADS;
         unsigned long long sr[R];
+        uint32_t odr[R];
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
             sr[u] = i < nv ? seen[v0 + i] : ~0ull;
+            odr[u] = i < nv ? od[v0 + i] : 0u;
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
@@ -1174,17 +1182,37 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
             continue;
 #pragma unroll
         for (int u = 0; u < R; ++u) {
-            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS; // a wavefront: 64 consecutive vertices
+            const uint32_t v = v0 + i;
             const unsigned long long a = i < nv ? s_acc[i] : 0ull;
             const unsigned long long n = a & live & ~sr[u];
-            if (n)
-                nl[v0 + i] = n;
+            if (i < nv) {
+                if (n) {
+                    seen[v] = sr[u] | n;
+                    t.edges += odr[u];
+                    t.verts += 1ull;
+                    t.fresh += (unsigned long long)__popcll(n);
+                }
+                front[v] = n;
+            }
+            unsigned long long has = wg_ballot(n != 0ull);
+            const uint32_t w0 = v - (uint32_t)lane; // the wavefront's first vertex (a multiple of 64)
+            if (lane == 0 && i < nv)
+                bm[w0 >> 6] = has;
+            while (has) { // one row store per vertex with new sources
+                const int l = __ffsll((long long)has) - 1;
+                has &= has - 1ull;
+                const unsigned long long bits = (unsigned long long)__shfl((long long)n, l);
+                if ((bits >> lane) & 1ull)
+                    depth[((uint64_t)w0 + (uint64_t)l) * WG_MS_MAX_SOURCES + (uint64_t)lane] = (uint16_t)level;
+            }
         }
     }
+    add_tally(t, ctr);
 }
 
-int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsigned long long* seen,
-                      unsigned long long* nl, unsigned long long live)
+int wg_pb_msbfs_level(wg_ctx* ctx, unsigned long long* front, unsigned long long* seen, uint16_t* depth,
+                      unsigned long long live, uint32_t level)
 {
     int rc = pb_prepare_sizes(ctx, 14, 14); // 16K 8-byte masks per window, 16K words per bin
     if (rc)
@@ -1200,6 +1228,7 @@ int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front, const unsign
     hipLaunchKernelGGL(k_pb_pull_or<16384>, dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
                        (const unsigned long long*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks,
                        (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot,
-                       reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, live, nl, V);
+                       reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, front, depth,
+                       ctx->bm_next, ctx->od, live, level, ctx->ctr, V);
     return wg_check_launch(ctx);
 }
