@@ -735,12 +735,12 @@ __device__ __forceinline__ void pb_tile_scan2(pb_lane_tile<OP>& a, pb_lane_tile<
     b.sc = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(b.sc);
 }
 
-// Tile phase 3: each entry flagged as a pair's last stores that pair's value
-// at part[p].  A tile's pairs are consecutive, so its stores fill one
-// contiguous range of part[]: a tile of at most WG_PB_STAGE pairs passes them
-// through s_stage, this wavefront's LDS row, so that each store instruction
-// writes 64 consecutive partials (one lane's pairs are consecutive, 64
-// lanes' are not: stored directly, each 8-byte store is its own L2 request).
+// Tile phase 3: each entry flagged as a pair's last gives that pair's value,
+// for part[p].  A tile's pairs are consecutive, so its stores fill one
+// contiguous range of part[]: they pass through s_stage, this wavefront's LDS
+// row, so that each store instruction writes 64 consecutive partials (one
+// lane's pairs are consecutive, 64 lanes' are not: stored directly, each
+// 8-byte store was its own L2 request).
 template <class OP>
 __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lane_tile<OP>& w,
                                               typename OP::V* __restrict__ part, typename OP::V* s_stage)
@@ -755,36 +755,30 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
         pre += (uint32_t)__popcll(m & below);
         all += (uint32_t)__popcll(m);
     }
-    V r = w.sc;
-    if (all <= WG_PB_STAGE) { // the whole wavefront: values to the LDS row in pair order, then coalesced stores
+    // values to the LDS row in pair order, WG_PB_STAGE pairs at a time (the
+    // whole wavefront takes the same passes), then coalesced stores
+    for (uint32_t h = 0; h < all; h += WG_PB_STAGE) {
+        V r = w.sc;
         uint32_t i = pre;
 #pragma unroll
         for (int j = 0; j < WG_PB_IPT; ++j) {
             r = OP::op(r, w.v[j]);
             if (w.e[j]) {
-                s_stage[i++] = r;
+                if (i - h < WG_PB_STAGE) // i >= h: unsigned wrap puts i < h out of range
+                    s_stage[i - h] = r;
+                ++i;
                 r = OP::zero();
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t k = (uint32_t)lane; k < all; k += WG_WAVE)
-            part[in.pbase + k] = s_stage[k];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // reads done before the next tile's writes
+        const uint32_t n = all - h < WG_PB_STAGE ? all - h : WG_PB_STAGE;
+        for (uint32_t k = (uint32_t)lane; k < n; k += WG_WAVE)
+            part[in.pbase + h + k] = s_stage[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // reads done before the row is written again
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        return;
-    }
-    uint32_t p = in.pbase + pre; // the lane's next pair
-#pragma unroll
-    for (int j = 0; j < WG_PB_IPT; ++j) {
-        r = OP::op(r, w.v[j]);
-        if (w.e[j]) {
-            part[p] = r;
-            ++p;
-            r = OP::zero();
-        }
     }
 }
 
@@ -866,11 +860,10 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
 // o - boff[kb] of part[] (window order).  Positions go 64 to a wavefront step
 // (chunk c = [64c, 64c + 64)): chunk_block[c] is the block holding 64c, the
 // block-start flags in pdst count the blocks after it, and the chunk's blocks'
-// offsets come from the lanes that loaded them.  WG_PB_PULL_ROUNDS chunks per
-// wavefront are in flight at a time.
-#define WG_PB_PULL_ROUNDS 8
-
-template <typename X, class ADD>
+// offsets come from the lanes that loaded them.  R chunks per wavefront are
+// in flight at a time (8 for the PageRank pull, 4 for the 64-source BFS, whose
+// epilogue keeps more registers).
+template <int R, typename X, class ADD>
 __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
                                               const uint32_t* __restrict__ chunk_block,
                                               const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs,
@@ -882,11 +875,11 @@ __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const 
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull; // lanes 0..lane
     const uint32_t wave = threadIdx.x / WG_WAVE;
     const uint32_t c1 = ((b1 - 1u) >> 6) + 1u;
-    for (uint32_t c = (b0 >> 6) + wave; c < c1; c += WG_PB_WAVES * WG_PB_PULL_ROUNDS) {
-        uint32_t o[WG_PB_PULL_ROUNDS], pd[WG_PB_PULL_ROUNDS], bo[WG_PB_PULL_ROUNDS];
-        X x[WG_PB_PULL_ROUNDS];
+    for (uint32_t c = (b0 >> 6) + wave; c < c1; c += WG_PB_WAVES * R) {
+        uint32_t o[R], pd[R], bo[R];
+        X x[R];
 #pragma unroll
-        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r) {
+        for (int r = 0; r < R; ++r) {
             const uint32_t cc = c + (uint32_t)r * WG_PB_WAVES; // uniform
             o[r] = cc * 64u + (uint32_t)lane;
             pd[r] = 0u;
@@ -901,7 +894,7 @@ __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const 
             }
         }
 #pragma unroll
-        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r) {
+        for (int r = 0; r < R; ++r) {
             // blocks started after the chunk's first position, up to this one
             const uint32_t k =
                 (uint32_t)__popcll(wg_ballot(lane > 0 && (pd[r] & WG_PB_BFLAG)) & upto);
@@ -911,7 +904,7 @@ __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const 
                 x[r] = __builtin_nontemporal_load(&part[o[r] - off]);
         }
 #pragma unroll
-        for (int r = 0; r < WG_PB_PULL_ROUNDS; ++r)
+        for (int r = 0; r < R; ++r)
             if (o[r] >= b0 && o[r] < b1)
                 add(pd[r] & WG_PB_DMASK, x[r]);
     }
@@ -994,8 +987,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
         __syncthreads();
-        pb_pull_range(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
-                      [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
+        pb_pull_range<8>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
+                         [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
         __syncthreads();
         if (sl.w != WG_PB_NOSLOT &&
             !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
@@ -1157,21 +1150,19 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
         const uint4 sl = slice[k];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
-        // the bin's seen masks and out-degrees for the commit, loaded while the slice streams
+        // the bin's seen masks for the commit, loaded while the slice streams
         constexpr int R = WD / WG_PB_THREADS;
         unsigned long long sr[R];
-        uint32_t odr[R];
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
             sr[u] = i < nv ? seen[v0 + i] : ~0ull;
-            odr[u] = i < nv ? od[v0 + i] : 0u;
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0ull;
         __syncthreads();
-        pb_pull_range(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z, [&](uint32_t d, unsigned long long x) {
+        pb_pull_range<4>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z, [&](uint32_t d, unsigned long long x) {
             if (x)
                 atomicOr(&s_acc[d], x);
         });
@@ -1189,7 +1180,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
             if (i < nv) {
                 if (n) {
                     seen[v] = sr[u] | n;
-                    t.edges += odr[u];
+                    t.edges += od[v];
                     t.verts += 1ull;
                     t.fresh += (unsigned long long)__popcll(n);
                 }
