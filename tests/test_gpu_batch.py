@@ -218,3 +218,40 @@ def test_msbfs_binned_dense_levels_match_oracle(slice_pairs):
                     assert np.array_equal(c.read_msbfs(i), want[i]), f"{name} {opts} root #{i}"
     finally:
         c.close()
+
+
+def test_msbfs_binned_kept_levels_reuse_and_stop_early():
+    """Binned dense levels keep their new-source masks and write the depth rows
+    after the run: runs that follow one another on one graph, runs stopped
+    after a few levels (rows of the kept levels only), and a graph with more
+    dense levels than masks kept (the later levels write their rows at once)."""
+    c = W.Context(0)
+    INF = np.uint32(2**32 - 1)
+    try:
+        c.set_option(W.OPT_MS_BINNED, 1)
+        g = W.rmat(12, 4, seed=6)
+        c.load(g)
+        orc = oracle.Oracle.of(g, 4)
+        first = _roots(g, 64, seed=11)
+        od = g.outdegrees()
+        low = np.flatnonzero((od > 0) & (od <= 2))
+        second = np.random.default_rng(12).choice(low, 5, replace=False).astype(np.uint64)
+        for roots, opts in ((first, dict(schedule="dense")), (second, dict(schedule="dense")),
+                            (first[:9], dict(schedule="plain")), (first, dict(schedule="dense", max_iterations=2))):
+            c.msbfs(roots, **opts)
+            stop = opts.get("max_iterations", 0)
+            for i in range(len(roots)):
+                want = orc.bfs(int(roots[i]))[1]
+                if stop:
+                    want = np.where((want != INF) & (want <= stop), want, INF)
+                assert np.array_equal(c.read_msbfs(i), want), f"{opts} root #{i}"
+        road = W.grid(30, 50, seed=5)  # about 80 levels, all dense
+        c.load(road)
+        orc = oracle.Oracle.of(road, 4)
+        roots = _roots(road, 64, seed=13)
+        levels, _ = c.msbfs(roots, schedule="dense")
+        assert levels > 8
+        for i in (0, 1, 30, 63):
+            assert np.array_equal(c.read_msbfs(i), orc.bfs(int(roots[i]))[1]), f"grid root #{i}"
+    finally:
+        c.close()

@@ -6,10 +6,14 @@
 //   seen[v]   sources that have reached v
 //   front[v]  sources that reached v at the last level (the frontier)
 //   nl[v]     sources reaching v at the running level (filled by the pull)
+//   save[k][v]  the new sources of level k (k < 16): each level's frontier
+//                 masks are kept, and the depth rows written from them once
+//                 per vertex after the run (k_ms_rows); levels past the 16th
+//                 use front and write their rows as they commit
 //   depth[v][64]  uint16 level at which source s reached v, valid where
 //                 seen[v] has bit s (vertex-major: a vertex's row is written
-//                 by one store; the entries of unreached sources are never
-//                 written, so the matrix needs no clearing between runs)
+//                 by one store; reads mask entries with seen, so the matrix
+//                 needs no clearing between runs)
 // A dense level is the bottom-up pull of source/phase_edge_pull_bfs.c:109-210
 // for all sources at once: a vertex ORs the frontier masks of its in-edges,
 // restricted to the sources that have not reached it.  A frontier level walks the Wedge frontier
@@ -22,6 +26,12 @@
 #include <vector>
 
 #define MS_LEVEL_MAX 0xFFFEu
+#define MS_SAVE 16u // levels per run whose new-source masks are kept for k_ms_rows
+
+// The levels of the kept masks: slot k holds the new sources of level level[k].
+struct ms_saved {
+    uint32_t level[MS_SAVE];
+};
 
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long x)
 {
@@ -219,6 +229,7 @@ __global__ __launch_bounds__(WG_BLOCK) void k_ms_blocks(int blog2, uint32_t E, c
 // seen, and write the vertex's depth row; the frontier vertex bitmask is
 // rewritten word by word for the next frontier build; counts: out-degree and
 // number of vertices with any new source, and (vertex, source) pairs reached.
+template <bool ROWS>
 __global__ __launch_bounds__(WG_BLOCK) void k_ms_commit(unsigned long long* __restrict__ nl,
                                                          unsigned long long* __restrict__ seen,
                                                          unsigned long long* __restrict__ front,
@@ -248,6 +259,8 @@ __global__ __launch_bounds__(WG_BLOCK) void k_ms_commit(unsigned long long* __re
         unsigned long long has = wg_ballot(n != 0ull);
         if (lane == 0)
             bm[w] = has;
+        if (!ROWS) // front is a kept slot: k_ms_rows writes the rows
+            continue;
         while (has) { // one row store per vertex with new sources
             const int l = __ffsll((long long)has) - 1;
             has &= has - 1ull;
@@ -257,6 +270,49 @@ __global__ __launch_bounds__(WG_BLOCK) void k_ms_commit(unsigned long long* __re
         }
     }
     add_tally(t, ctr);
+}
+
+// Depth rows written once per vertex: levels 0 .. nsave - 1 keep their
+// new-source masks (slot k of `save`, V words each, for level lv.level[k])
+// instead of storing a partial row per vertex per level, and each vertex
+// reached in them gets its whole row in one 128-byte store.  A source reaches
+// a vertex at one level only, so a lane takes the level of the one slot
+// holding its bit; lanes in no slot (sources that have not reached the vertex)
+// take 0xFFFF, which reads never see (they mask with seen), and which a later
+// level overwrites when its source arrives.
+__global__ __launch_bounds__(WG_BLOCK) void k_ms_rows(const unsigned long long* __restrict__ save, uint32_t nsave,
+                                                       ms_saved lv, uint32_t V, uint16_t* __restrict__ depth)
+{
+    const int lane = wg_lane();
+    const uint64_t nwords = ((uint64_t)V + 63) / 64;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WG_WPB;
+    for (uint64_t w = (uint64_t)blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
+        const uint32_t v = (uint32_t)(w * 64 + lane);
+        unsigned long long m[MS_SAVE];
+        unsigned long long any = 0ull;
+#pragma unroll
+        for (uint32_t k = 0; k < MS_SAVE; ++k) {
+            m[k] = (k < nsave && v < V) ? save[(uint64_t)k * V + v] : 0ull;
+            any |= m[k];
+        }
+        unsigned long long has = wg_ballot(any != 0ull);
+        while (has) { // one row store per vertex reached at a kept level
+            const int l = __ffsll((long long)has) - 1;
+            has &= has - 1ull;
+            uint32_t level = 0xffffu;
+#pragma unroll
+            for (uint32_t k = 0; k < MS_SAVE; ++k) {
+                if (k < nsave) { // uniform
+                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)m[k], l);
+                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(m[k] >> 32), l);
+                    const uint32_t bits = lane < 32 ? lo >> lane : hi >> (lane - 32);
+                    if (bits & 1u)
+                        level = lv.level[k];
+                }
+            }
+            depth[(w * 64 + (uint64_t)l) * WG_MS_MAX_SOURCES + (uint64_t)lane] = (uint16_t)level;
+        }
+    }
 }
 
 // Edges reached over all sources: sum of od[v] * |seen[v]|.
@@ -302,9 +358,8 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         return WG_E_INVAL;
     std::vector<uint32_t> hroots(nroots);
     int rc;
-    for (uint32_t i = 0; i < nroots; ++i)
-        if ((rc = wg_to_device_id(ctx, roots[i], &hroots[i])))
-            return rc;
+    if ((rc = wg_to_device_ids(ctx, roots, nroots, hroots.data())))
+        return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint32_t V = (uint32_t)ctx->V, E = (uint32_t)ctx->E;
@@ -317,7 +372,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
     } else {
         wg_free_app_state(ctx);
         if ((rc = ms_alloc(ctx, (size_t)V * 8, &seen)) || (rc = ms_alloc(ctx, (size_t)V * 8, &front)) ||
-            (rc = ms_alloc(ctx, (size_t)V * 8, &nl)) ||
+            (rc = ms_alloc(ctx, (size_t)V * 8 + WG_MS_MAX_SOURCES * 4, &nl)) || // nl, then the device roots
             (rc = ms_alloc(ctx, (size_t)V * WG_MS_MAX_SOURCES * 2, &depth))) {
             (void)hipFree(seen);
             (void)hipFree(front);
@@ -331,31 +386,40 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         ctx->ms[2] = nl;
         ctx->app = WG_APP_MSBFS;
     }
-    if ((rc = ms_alloc(ctx, (size_t)nroots * 4, &droots)))
+    droots = (char*)nl + (size_t)V * 8;
+    if (!ctx->ms[3] && (rc = ms_alloc(ctx, (size_t)V * 8 * MS_SAVE, &ctx->ms[3])))
         return rc;
     ctx->ms_n = nroots;
-    unsigned long long *dseen = (unsigned long long*)seen, *dfront = (unsigned long long*)front,
-                       *dnl = (unsigned long long*)nl;
-    const unsigned long long live = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
-    WG_HIP(ctx, hipMemcpyAsync(droots, hroots.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, ctx->stream));
-    if ((rc = wg_clear_u64(ctx, (uint64_t*)seen, V)) || (rc = wg_clear_u64(ctx, (uint64_t*)front, V)) ||
-        (rc = wg_clear_u64(ctx, (uint64_t*)nl, V)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
-        (rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u)))
-        goto out;
-    if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
-        goto out;
-    hipLaunchKernelGGL(k_ms_roots, dim3((nroots + 63) / 64), dim3(64), 0, ctx->stream, (const uint32_t*)droots, nroots,
-                       dseen, dfront, (uint16_t*)depth, (unsigned long long*)ctx->bm_next, ctx->od, ctx->ctr);
-    if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
-        goto out;
     {
+        unsigned long long *dseen = (unsigned long long*)seen, *dfront = (unsigned long long*)front,
+                           *dnl = (unsigned long long*)nl;
+        unsigned long long* const save = (unsigned long long*)ctx->ms[3];
+        const unsigned long long live = nroots == 64 ? ~0ull : ((1ull << nroots) - 1ull);
+        // Each level's new sources go to the next kept slot (the roots to slot
+        // 0) and k_ms_rows writes the depth rows after the run, whole.  When
+        // the slots run out, k_ms_rows writes the rows of the kept levels and
+        // the later levels write theirs as they commit, into front.
+        ms_saved kept = {};
+        uint32_t nsave = 1, level = 0;
+        bool keeping = true;
+        unsigned long long* cur = save; // the last level's new sources (the frontier masks)
+        WG_HIP(ctx, hipMemcpyAsync(droots, hroots.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = wg_clear_u64(ctx, (uint64_t*)seen, V)) || (rc = wg_clear_u64(ctx, (uint64_t*)save, V)) ||
+            (rc = wg_clear_u64(ctx, (uint64_t*)nl, V)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
+            (rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u)))
+            goto out;
+        if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
+            goto out;
+        hipLaunchKernelGGL(k_ms_roots, dim3((nroots + 63) / 64), dim3(64), 0, ctx->stream, (const uint32_t*)droots,
+                           nroots, dseen, cur, (uint16_t*)depth, (unsigned long long*)ctx->bm_next, ctx->od, ctx->ctr);
+        if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
+            goto out;
         unsigned long long act = ctx->hctr->activity; // summed out-degree of the distinct roots
         uint64_t fverts = nroots;                     // vertices in the frontier (at most)
         const unsigned long long threshold = (ctx->E * (unsigned long long)params->threshold_pct) / 100ull;
         const uint32_t cgrid = wg_grid(ctx, ctx->bm_words, WG_WPB, 16);
         const int opt = ctx->ms_binned ? ctx->ms_binned - 1 : 2;
         const bool binned = opt == 1 || (opt == 2 && ctx->E >= (1ull << 24));
-        uint32_t level = 0;
         for (;;) {
             const bool dense = params->schedule == 2 || (params->schedule != 3 && act > threshold);
             if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
@@ -364,32 +428,47 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
                 rc = WG_E_OVERFLOW; // depths are stored in 16 bits
                 goto out;
             }
-            const bool committed = dense && binned; // the binned pull commits the level itself
-            if (committed) {
-                if ((rc = wg_pb_msbfs_level(ctx, dfront, dseen, (uint16_t*)depth, live, level + 1)))
+            if (keeping && nsave == MS_SAVE) { // no slot left: the kept levels' rows now
+                hipLaunchKernelGGL(k_ms_rows, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream,
+                                   (const unsigned long long*)save, nsave, kept, V, (uint16_t*)depth);
+                if ((rc = wg_check_launch(ctx)))
                     goto out;
-            } else if (dense) {
-                if (fverts < (uint64_t)V / 8)
-                    hipLaunchKernelGGL(k_ms_tile<true>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
-                                       ctx->src, ctx->bm_next, dfront, dseen, dnl, ctx->tile_v, ctx->tile_e, ctx->ntiles,
-                                       live);
-                else
-                    hipLaunchKernelGGL(k_ms_tile<false>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
-                                       ctx->src, ctx->bm_next, dfront, dseen, dnl, ctx->tile_v, ctx->tile_e, ctx->ntiles,
-                                       live);
-            } else {
-                if ((rc = wg_launch_wedge_build(ctx, ctx->bm_next, blog2)))
-                    goto out;
-                hipLaunchKernelGGL(k_ms_blocks, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, blog2, E,
-                                   ctx->src, ctx->dst, ctx->bm_next, dfront, dseen, dnl, ctx->wedge, live);
+                keeping = false;
             }
-            if ((rc = wg_check_launch(ctx)))
-                goto out;
-            if (!committed)
-                hipLaunchKernelGGL(k_ms_commit, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen, dfront,
-                                   (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
+            unsigned long long* out = keeping ? save + (size_t)nsave * V : dfront;
+            if (dense && binned) { // the binned pull commits the level itself
+                if ((rc = wg_pb_msbfs_level(ctx, cur, out, dseen, (uint16_t*)depth, live, level + 1, !keeping)))
+                    goto out;
+            } else {
+                if (dense && fverts < (uint64_t)V / 8)
+                    hipLaunchKernelGGL(k_ms_tile<true>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
+                                       ctx->src, ctx->bm_next, cur, dseen, dnl, ctx->tile_v, ctx->tile_e, ctx->ntiles,
+                                       live);
+                else if (dense)
+                    hipLaunchKernelGGL(k_ms_tile<false>, dim3(ctx->pr_grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off,
+                                       ctx->src, ctx->bm_next, cur, dseen, dnl, ctx->tile_v, ctx->tile_e, ctx->ntiles,
+                                       live);
+                else {
+                    if ((rc = wg_launch_wedge_build(ctx, ctx->bm_next, blog2)))
+                        goto out;
+                    hipLaunchKernelGGL(k_ms_blocks, dim3(wg_frontier_grid(ctx)), dim3(WG_BLOCK), 0, ctx->stream, blog2,
+                                       E, ctx->src, ctx->dst, ctx->bm_next, cur, dseen, dnl, ctx->wedge, live);
+                }
+                if ((rc = wg_check_launch(ctx)))
+                    goto out;
+                // out may be cur (front, once the slots are gone): the pull above has read it
+                if (keeping)
+                    hipLaunchKernelGGL(k_ms_commit<false>, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen,
+                                       out, (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
+                else
+                    hipLaunchKernelGGL(k_ms_commit<true>, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen,
+                                       out, (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
+            }
             if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
                 goto out;
+            if (keeping)
+                kept.level[nsave++] = level + 1;
+            cur = out;
             ++level;
             if (ctx->hctr->activated == 0)
                 break;
@@ -397,6 +476,12 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             fverts = ctx->hctr->activated;
             if (params->max_iterations && level >= params->max_iterations)
                 break;
+        }
+        if (keeping) {
+            hipLaunchKernelGGL(k_ms_rows, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream,
+                               (const unsigned long long*)save, nsave, kept, V, (uint16_t*)depth);
+            if ((rc = wg_check_launch(ctx)))
+                goto out;
         }
         if (levels)
             *levels = level;
@@ -410,7 +495,6 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         }
     }
 out:
-    (void)hipFree(droots);
     if (rc == WG_OK)
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return rc;
@@ -736,9 +820,8 @@ extern "C" int wg_mssssp(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, co
         return WG_E_INVAL;
     std::vector<uint32_t> hroots(nroots);
     int rc;
-    for (uint32_t i = 0; i < nroots; ++i)
-        if ((rc = wg_to_device_id(ctx, roots[i], &hroots[i])))
-            return rc;
+    if ((rc = wg_to_device_ids(ctx, roots, nroots, hroots.data())))
+        return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     wg_free_app_state(ctx);

@@ -360,9 +360,8 @@ extern "C" int wg_validate_values(wg_ctx* ctx, int app, const uint64_t* roots, u
         return WG_E_STATE;
     std::vector<uint32_t> dr(nroots);
     int rc;
-    for (uint32_t i = 0; i < nroots; ++i)
-        if ((rc = wg_to_device_id(ctx, roots[i], &dr[i])))
-            return rc;
+    if ((rc = wg_to_device_ids(ctx, roots, (uint32_t)nroots, dr.data())))
+        return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     const size_t esz = (app == WG_APP_BFS || (app == WG_APP_SSSP && ctx->wkind == WG_WEIGHT_F64)) ? 8 : 4;
     const uint64_t V = ctx->V;

@@ -317,6 +317,45 @@ int wg_to_device_id(wg_ctx* ctx, uint64_t orig, uint32_t* out)
     return WG_OK;
 }
 
+__global__ void k_map_ids(const uint64_t* __restrict__ in, uint32_t n, const uint32_t* __restrict__ inv,
+                          uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = inv[in[i]];
+}
+
+int wg_to_device_ids(wg_ctx* ctx, const uint64_t* orig, uint32_t n, uint32_t* out)
+{
+    for (uint32_t i = 0; i < n; ++i)
+        if (orig[i] >= ctx->V)
+            return WG_E_RANGE;
+    if (!ctx->inv || n == 0) {
+        for (uint32_t i = 0; i < n; ++i)
+            out[i] = (uint32_t)orig[i];
+        return WG_OK;
+    }
+    WG_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t need = (size_t)n * (sizeof(uint64_t) + sizeof(uint32_t));
+    if (ctx->idmap_bytes < need) {
+        if (ctx->idmap)
+            (void)hipFree(ctx->idmap);
+        ctx->idmap = nullptr;
+        ctx->idmap_bytes = 0;
+        WG_HIP(ctx, hipMalloc(&ctx->idmap, need));
+        ctx->idmap_bytes = need;
+    }
+    uint64_t* din = (uint64_t*)ctx->idmap;
+    uint32_t* dout = (uint32_t*)(din + n);
+    WG_HIP(ctx, hipMemcpyAsync(din, orig, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_map_ids, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, (const uint64_t*)din, n,
+                       (const uint32_t*)ctx->inv, dout);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipMemcpyAsync(out, dout, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
 int wg_read_vertex_array(wg_ctx* ctx, const void* dev, void* host, size_t esz)
 {
     if (!ctx->inv) {
@@ -422,6 +461,7 @@ int wg_free_app_state(wg_ctx* ctx)
     free_dev(&ctx->ms[0]);
     free_dev(&ctx->ms[1]);
     free_dev(&ctx->ms[2]);
+    free_dev(&ctx->ms[3]);
     ctx->ms_n = 0;
     free_dev((void**)&ctx->acc);
     free_dev((void**)&ctx->roots);
@@ -490,6 +530,7 @@ extern "C" void wg_close(wg_ctx* ctx)
         free(ctx->prof.kid);
     }
     (void)hipFree(ctx->mlp);
+    (void)hipFree(ctx->idmap);
     (void)hipFree(ctx->run_ctl);
     (void)hipFree(ctx->run_q);
     (void)hipFree(ctx->ctr);

@@ -193,9 +193,11 @@ struct wg_ctx {
     // application state
     int app;
     void* prop[2];            // PR contributions (double buffer) / BFS packed / CC labels / SSSP dist
-    void* ms[3];              // batched sources: per-vertex masks or rows (freed with the app state)
+    void* ms[4];              // batched sources: per-vertex masks or rows (freed with the app state)
     uint32_t ms_n;            // batched sources: number of roots / seeds
     size_t ms_aux_bytes;      // personalised PageRank: size of ms[2] (its reduction partials)
+    void* idmap;              // wg_to_device_ids: original ids in, device ids out (grown as needed)
+    size_t idmap_bytes;
     int pr_cur;               // PR: index of the current contribution buffer
     int pr_parity;            // PR: index of S[] holding the current iteration's sum
     int pr_fp32;
@@ -342,6 +344,8 @@ int wg_fill_u32(wg_ctx* ctx, uint32_t* p, uint64_t n, uint32_t value);
 int wg_fill_u64(wg_ctx* ctx, uint64_t* p, uint64_t n, uint64_t value);
 // original vertex id -> device vertex id (identity without relabelling)
 int wg_to_device_id(wg_ctx* ctx, uint64_t orig, uint32_t* out);
+// the same for n ids with one round trip to the device (WG_E_RANGE if any is out of range)
+int wg_to_device_ids(wg_ctx* ctx, const uint64_t* orig, uint32_t n, uint32_t* out);
 // copies V values stored in device order to host memory in original order
 int wg_read_vertex_array(wg_ctx* ctx, const void* dev, void* host, size_t elem_size);
 // edge tiles of at most tile_edges edges (and WG_TILE_VERTS destinations)
@@ -361,10 +365,12 @@ int wg_pb_iteration(wg_ctx* ctx, int mode, const void* cin, void* cout, double* 
 int wg_pb_prepare(wg_ctx* ctx);
 int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint32_t* tasks, uint32_t* slices);
 // a dense level of the 64-source BFS on the binned layout, committed: with
-// n[v] = (OR of the in-neighbours' front & live) & ~seen[v], seen |= n,
-// front = n, depth entries of the new sources = level, the frontier vertex
-// bitmask (bm_next) and the level's counts in ctx->ctr
-int wg_pb_msbfs_level(wg_ctx* ctx, unsigned long long* front, unsigned long long* seen, uint16_t* depth,
-                      unsigned long long live, uint32_t level);
+// n[v] = (OR of the in-neighbours' front_in & live) & ~seen[v], seen |= n,
+// front_out = n (front_out may be front_in), the depth entries of the new
+// sources = level when rows is set (else the caller writes them from
+// front_out), the frontier vertex bitmask (bm_next) and the level's counts in
+// ctx->ctr
+int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front_in, unsigned long long* front_out,
+                      unsigned long long* seen, uint16_t* depth, unsigned long long live, uint32_t level, bool rows);
 // releases the binned layout (graph change)
 void wg_free_pb(wg_ctx* ctx);

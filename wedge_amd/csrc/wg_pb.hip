@@ -855,6 +855,21 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
     }
 }
 
+// The slice a pull workgroup takes.  Slices of neighbouring bins share the
+// lines where a window's blocks for the two bins meet, and the hardware hands
+// workgroups to the 8 XCDs in turn (workgroup i to XCD i mod 8), each with its
+// own L2.  Within each group of 64 workgroups, workgroup 8a + b takes slice
+// 8b + a, so each XCD runs 8 consecutive slices side by side and reads their
+// shared lines from one L2.  A bijection on [0, n): the last partial group
+// keeps its order.
+__device__ __forceinline__ uint32_t pb_slice_of(uint32_t i, uint32_t n)
+{
+    const uint32_t g = i & ~63u;
+    if (g + 64u > n)
+        return i;
+    return g + (i & 7u) * 8u + ((i & 63u) >> 3);
+}
+
 // The pull's pass over one slice, bin-order positions [b0, b1):
 // ADD(destination, partial) for every pair.  Position o of block kb holds pair
 // o - boff[kb] of part[] (window order).  Positions go 64 to a wavefront step
@@ -972,7 +987,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         omd_n = (1.0 - damping) / N;
     }
     for (uint32_t k = blockIdx.x; k < nslices; k += gridDim.x) {
-        const uint4 sl = slice[k];
+        const uint4 sl = slice[pb_slice_of(k, nslices)];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
         // the bin's out-degrees for the vertex phase, loaded while the slice streams
@@ -998,8 +1013,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
 #pragma unroll
         for (int u = 0; u < R; ++u) {
             const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS;
-            if (i >= nv)
-                break;
+            if (i >= nv) // and every later u
+                continue;
             const uint32_t v = v0 + i;
             const double accv = s_acc[i];
             if (MODE == 0) {
@@ -1132,7 +1147,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
 // seen |= n, front = n, the depth row entries of the new sources, the frontier
 // vertex bitmask, and the counts (out-degree and number of vertices with new
 // sources, new (vertex, source) pairs).
-template <int WD>
+template <int WD, bool ROWS>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
     const unsigned long long* __restrict__ part, const uint16_t* __restrict__ pdst,
     const uint32_t* __restrict__ chunk_block, const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs,
@@ -1147,7 +1162,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
     const int lane = wg_lane();
     act_tally t = { 0ull, 0ull, 0ull };
     for (uint32_t k = blockIdx.x; k < nslices; k += gridDim.x) {
-        const uint4 sl = slice[k];
+        const uint4 sl = slice[pb_slice_of(k, nslices)];
         const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
         const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
         // the bin's seen masks for the commit, loaded while the slice streams
@@ -1190,6 +1205,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
             const uint32_t w0 = v - (uint32_t)lane; // the wavefront's first vertex (a multiple of 64)
             if (lane == 0 && i < nv)
                 bm[w0 >> 6] = has;
+            if (!ROWS) // the caller keeps front and writes the rows later (k_ms_rows)
+                continue;
             while (has) { // one row store per vertex with new sources
                 const int l = __ffsll((long long)has) - 1;
                 has &= has - 1ull;
@@ -1202,8 +1219,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_or(
     add_tally(t, ctr);
 }
 
-int wg_pb_msbfs_level(wg_ctx* ctx, unsigned long long* front, unsigned long long* seen, uint16_t* depth,
-                      unsigned long long live, uint32_t level)
+int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front_in, unsigned long long* front_out,
+                      unsigned long long* seen, uint16_t* depth, unsigned long long live, uint32_t level, bool rows)
 {
     int rc = pb_prepare_sizes(ctx, 14, 14); // 16K 8-byte masks per window, 16K words per bin
     if (rc)
@@ -1212,14 +1229,21 @@ int wg_pb_msbfs_level(wg_ctx* ctx, unsigned long long* front, unsigned long long
     const uint32_t V = (uint32_t)ctx->V;
     unsigned long long* part = reinterpret_cast<unsigned long long*>(pb->part);
     if (pb->ntasks)
-        hipLaunchKernelGGL(k_pb_push_or<16384>, dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, front,
-                           live, pb->task, pb->ntasks, pb->tile_pbase, part, V);
+        hipLaunchKernelGGL(k_pb_push_or<16384>, dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code,
+                           front_in, live, pb->task, pb->ntasks, pb->tile_pbase, part, V);
     if ((rc = wg_check_launch(ctx)))
         return rc;
-    hipLaunchKernelGGL(k_pb_pull_or<16384>, dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
-                       (const unsigned long long*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks,
-                       (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot,
-                       reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, front, depth,
-                       ctx->bm_next, ctx->od, live, level, ctx->ctr, V);
+    if (rows)
+        hipLaunchKernelGGL((k_pb_pull_or<16384, true>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const unsigned long long*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks,
+                           (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot,
+                           reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, front_out,
+                           depth, ctx->bm_next, ctx->od, live, level, ctx->ctr, V);
+    else
+        hipLaunchKernelGGL((k_pb_pull_or<16384, false>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const unsigned long long*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks,
+                           (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot,
+                           reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, front_out,
+                           depth, ctx->bm_next, ctx->od, live, level, ctx->ctr, V);
     return wg_check_launch(ctx);
 }

@@ -231,9 +231,8 @@ extern "C" int wg_ppr(wg_ctx* ctx, const uint64_t* seeds, uint32_t nseeds, uint3
         return WG_E_INVAL;
     std::vector<uint32_t> hs(nseeds);
     int rc;
-    for (uint32_t i = 0; i < nseeds; ++i)
-        if ((rc = wg_to_device_id(ctx, seeds[i], &hs[i])))
-            return rc;
+    if ((rc = wg_to_device_ids(ctx, seeds, nseeds, hs.data())))
+        return rc;
     WG_HIP(ctx, hipSetDevice(ctx->device));
     WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const uint64_t V = ctx->V;

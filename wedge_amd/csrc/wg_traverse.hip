@@ -1819,9 +1819,8 @@ static int init_roots(wg_ctx* ctx, int app, const uint64_t* roots, uint32_t n)
         return WG_E_STATE;
     std::vector<uint32_t> dev(n);
     int rc;
-    for (uint32_t i = 0; i < n; ++i)
-        if ((rc = wg_to_device_id(ctx, roots[i], &dev[i])))
-            return rc;
+    if ((rc = wg_to_device_ids(ctx, roots, n, dev.data())))
+        return rc;
     std::sort(dev.begin(), dev.end());
     dev.erase(std::unique(dev.begin(), dev.end()), dev.end());
     if ((rc = init_common(ctx, app)))
