@@ -268,9 +268,12 @@ int wg_fused_level_count(wg_ctx* ctx, uint64_t* levels);
 /* levels: BFS levels run; edges_reached: sum over sources of the out-degrees
  * of the vertices each reached (both optional).  WG_E_OVERFLOW if a depth
  * would exceed 65534.  Its device buffers (128V bytes of 16-bit depth rows,
- * 64 per vertex, and three 8V-byte masks) stay allocated until another
- * application is initialised or the graph changes, so that further calls on
- * the same graph do not allocate them again. */
+ * 64 per vertex, three 8V-byte masks and 128V bytes keeping the new-source
+ * masks of the first 16 levels, from which the depth rows are written once
+ * per vertex) stay allocated until another application is initialised or the
+ * graph changes, so that further calls on the same graph do not allocate
+ * them again.  A threshold_pct of 1-2 suits batched levels (the Python
+ * binding's default is 2). */
 int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, const wg_loop_params* params, uint64_t* levels,
              uint64_t* edges_reached);
 /* Depths (V entries) from root number `source`, UINT32_MAX if unreached. */

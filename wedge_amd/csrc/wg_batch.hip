@@ -412,7 +412,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             goto out;
         hipLaunchKernelGGL(k_ms_roots, dim3((nroots + 63) / 64), dim3(64), 0, ctx->stream, (const uint32_t*)droots,
                            nroots, dseen, cur, (uint16_t*)depth, (unsigned long long*)ctx->bm_next, ctx->od, ctx->ctr);
-        if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
+        if ((rc = wg_check_launch(ctx)) || (rc = wg_take_activity(ctx)))
             goto out;
         unsigned long long act = ctx->hctr->activity; // summed out-degree of the distinct roots
         uint64_t fverts = nroots;                     // vertices in the frontier (at most)
@@ -422,8 +422,6 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         const bool binned = opt == 1 || (opt == 2 && ctx->E >= (1ull << 24));
         for (;;) {
             const bool dense = params->schedule == 2 || (params->schedule != 3 && act > threshold);
-            if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
-                goto out;
             if (level + 1 > MS_LEVEL_MAX) {
                 rc = WG_E_OVERFLOW; // depths are stored in 16 bits
                 goto out;
@@ -464,7 +462,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
                     hipLaunchKernelGGL(k_ms_commit<true>, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream, dnl, dseen,
                                        out, (uint16_t*)depth, ctx->bm_next, ctx->od, V, level + 1, ctx->ctr);
             }
-            if ((rc = wg_check_launch(ctx)) || (rc = wg_read_counters(ctx)))
+            if ((rc = wg_check_launch(ctx)) || (rc = wg_take_activity(ctx))) // the counters start the next level at 0
                 goto out;
             if (keeping)
                 kept.level[nsave++] = level + 1;

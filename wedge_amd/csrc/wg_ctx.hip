@@ -441,6 +441,10 @@ extern "C" int wg_open(wg_ctx** out, int device)
         return WG_E_NOMEM;
     }
     memset(ctx->hctr, 0, sizeof(wg_dev_counters));
+    if (hipHostGetDevicePointer((void**)&ctx->hctr_dev, ctx->hctr, 0) != hipSuccess) {
+        ctx->hctr_dev = nullptr; // wg_take_activity copies instead
+        (void)hipGetLastError();
+    }
     ctx->run_wgs[WG_APP_BFS] = WG_RUN_DEFAULT_WGS;
     *out = ctx;
     return WG_OK;

@@ -230,6 +230,7 @@ struct wg_ctx {
     int mlp_set;              // a model is set for the learned schedule
     wg_dev_counters* ctr;     // device
     wg_dev_counters* hctr;    // pinned host mirror
+    wg_dev_counters* hctr_dev; // the mirror's device address (null if it has none)
     uint32_t root_dev;        // BFS / SSSP root in device numbering (the first root)
     uint32_t* roots;          // BFS / SSSP roots in device numbering, distinct
     uint32_t* root_off;       // BFS, at most WG_PUSH_ROOTS_MAX roots: exclusive prefix of the roots'
@@ -332,6 +333,9 @@ __device__ __forceinline__ void wg_wave_sync()
 // internal entry points shared across translation units
 int wg_reset_counters(wg_ctx* ctx, unsigned int flags); // RST_* flags (wg_frontier.h)
 int wg_read_counters(wg_ctx* ctx);                      // fold the slots, copy to ctx->hctr, wait
+// fold the slots, publish activity / activated / fresh to ctx->hctr and clear
+// them on the device for the next level (one launch), wait
+int wg_take_activity(wg_ctx* ctx);
 int wg_launch_wedge_build(wg_ctx* ctx, const uint64_t* active, int block_log2);
 uint32_t wg_frontier_grid(const wg_ctx* ctx);
 uint32_t wg_wedge_build_grid(const wg_ctx* ctx);

@@ -858,16 +858,19 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
 // The slice a pull workgroup takes.  Slices of neighbouring bins share the
 // lines where a window's blocks for the two bins meet, and the hardware hands
 // workgroups to the 8 XCDs in turn (workgroup i to XCD i mod 8), each with its
-// own L2.  Within each group of 64 workgroups, workgroup 8a + b takes slice
-// 8b + a, so each XCD runs 8 consecutive slices side by side and reads their
-// shared lines from one L2.  A bijection on [0, n): the last partial group
-// keeps its order.
+// own L2.  Within each group of 128 workgroups, workgroup 8a + b takes slice
+// 16b + a, so each XCD runs 16 consecutive slices side by side and reads their
+// shared lines from one L2 (runs of 4, 8, 16 and 32 measured: 16 is best).  A
+// bijection on [0, n): the last partial group keeps its order.
+#define WG_PB_XCD_RUN 16u
 __device__ __forceinline__ uint32_t pb_slice_of(uint32_t i, uint32_t n)
 {
-    const uint32_t g = i & ~63u;
-    if (g + 64u > n)
+    constexpr uint32_t G = 8u * WG_PB_XCD_RUN;
+    const uint32_t g = i - i % G;
+    if (g + G > n)
         return i;
-    return g + (i & 7u) * 8u + ((i & 63u) >> 3);
+    const uint32_t r = i % G;
+    return g + (r % 8u) * WG_PB_XCD_RUN + r / 8u;
 }
 
 // The pull's pass over one slice, bin-order positions [b0, b1):

@@ -1735,6 +1735,35 @@ int wg_read_counters(wg_ctx* ctx)
     return WG_OK;
 }
 
+// Folds the slots and hands the level's activity counts to the host mirror
+// (mapped pinned memory), then clears them for the next level.
+__global__ void k_take_activity(wg_dev_counters* c, wg_dev_counters* host)
+{
+    if (threadIdx.x < WG_WAVE)
+        fold_counts(c);
+    if (threadIdx.x == 0) { // the lane that folded the totals
+        host->activity = c->activity;
+        host->activated = c->activated;
+        host->fresh = c->fresh;
+        c->activity = 0ull;
+        c->activated = 0ull;
+        c->fresh = 0ull;
+        __threadfence_system();
+    }
+}
+
+int wg_take_activity(wg_ctx* ctx)
+{
+    if (!ctx->hctr_dev) {
+        int rc = wg_read_counters(ctx);
+        return rc ? rc : wg_reset_counters(ctx, RST_ACTIVITY);
+    }
+    hipLaunchKernelGGL(k_take_activity, dim3(1), dim3(WG_WAVE), 0, ctx->stream, ctx->ctr, ctx->hctr_dev);
+    WG_HIP(ctx, hipGetLastError());
+    WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return WG_OK;
+}
+
 static int clear_wedge_state(wg_ctx* ctx)
 {
     int rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u);
