@@ -26,12 +26,8 @@
 #include <vector>
 
 #define MS_LEVEL_MAX 0xFFFEu
-#define MS_SAVE 16u // levels per run whose new-source masks are kept for k_ms_rows
-
-// The levels of the kept masks: slot k holds the new sources of level level[k].
-struct ms_saved {
-    uint32_t level[MS_SAVE];
-};
+#define MS_SAVE 16u // levels per run whose new-source masks are kept for k_ms_rows (slot k: level k)
+#define MS_ROW_PAD 9u // uint4 per staged depth row in LDS (8 used, one of padding against bank conflicts)
 
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long x)
 {
@@ -273,45 +269,69 @@ __global__ __launch_bounds__(WG_BLOCK) void k_ms_commit(unsigned long long* __re
 }
 
 // Depth rows written once per vertex: levels 0 .. nsave - 1 keep their
-// new-source masks (slot k of `save`, V words each, for level lv.level[k])
-// instead of storing a partial row per vertex per level, and each vertex
-// reached in them gets its whole row in one 128-byte store.  A source reaches
-// a vertex at one level only, so a lane takes the level of the one slot
-// holding its bit; lanes in no slot (sources that have not reached the vertex)
-// take 0xFFFF, which reads never see (they mask with seen), and which a later
-// level overwrites when its source arrives.
+// new-source masks (slot k of `save`, V words each, holds level k) instead of
+// storing a partial row per vertex per level.  A source reaches a vertex at
+// one level only, so its entry is the index of the one slot holding its bit:
+// bit j of that index is bit s of plane j, the OR of the slots whose index
+// has bit j.  Entries of sources that have not reached the vertex (in no
+// slot) are 0xFFFF, which reads never see (they mask with seen) and which a
+// later level overwrites when its source arrives.  Each lane builds its
+// vertex's 128-byte row in LDS; the wavefront then stores its 64 rows (8 KB,
+// contiguous in depth) 1 KB per instruction, skipping rows of vertices that
+// no kept level reached.
 __global__ __launch_bounds__(WG_BLOCK) void k_ms_rows(const unsigned long long* __restrict__ save, uint32_t nsave,
-                                                       ms_saved lv, uint32_t V, uint16_t* __restrict__ depth)
+                                                       uint32_t V, uint16_t* __restrict__ depth)
 {
+    __shared__ uint4 s_rows_all[WG_WPB][WG_WAVE * MS_ROW_PAD];
     const int lane = wg_lane();
+    uint4* s_rows = s_rows_all[threadIdx.x / WG_WAVE];
     const uint64_t nwords = ((uint64_t)V + 63) / 64;
     const uint64_t nwaves = (uint64_t)gridDim.x * WG_WPB;
     for (uint64_t w = (uint64_t)blockIdx.x * WG_WPB + threadIdx.x / WG_WAVE; w < nwords; w += nwaves) {
         const uint32_t v = (uint32_t)(w * 64 + lane);
-        unsigned long long m[MS_SAVE];
-        unsigned long long any = 0ull;
+        unsigned long long pl[4] = { 0ull, 0ull, 0ull, 0ull }, any = 0ull;
 #pragma unroll
         for (uint32_t k = 0; k < MS_SAVE; ++k) {
-            m[k] = (k < nsave && v < V) ? save[(uint64_t)k * V + v] : 0ull;
-            any |= m[k];
-        }
-        unsigned long long has = wg_ballot(any != 0ull);
-        while (has) { // one row store per vertex reached at a kept level
-            const int l = __ffsll((long long)has) - 1;
-            has &= has - 1ull;
-            uint32_t level = 0xffffu;
+            const unsigned long long m = (k < nsave && v < V) ? save[(uint64_t)k * V + v] : 0ull;
+            any |= m;
 #pragma unroll
-            for (uint32_t k = 0; k < MS_SAVE; ++k) {
-                if (k < nsave) { // uniform
-                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)m[k], l);
-                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(m[k] >> 32), l);
-                    const uint32_t bits = lane < 32 ? lo >> lane : hi >> (lane - 32);
-                    if (bits & 1u)
-                        level = lv.level[k];
-                }
-            }
-            depth[(w * 64 + (uint64_t)l) * WG_MS_MAX_SOURCES + (uint64_t)lane] = (uint16_t)level;
+            for (int j = 0; j < 4; ++j)
+                if ((k >> j) & 1u)
+                    pl[j] |= m;
         }
+        const uint32_t h[5][2] = { { (uint32_t)pl[0], (uint32_t)(pl[0] >> 32) },
+                                   { (uint32_t)pl[1], (uint32_t)(pl[1] >> 32) },
+                                   { (uint32_t)pl[2], (uint32_t)(pl[2] >> 32) },
+                                   { (uint32_t)pl[3], (uint32_t)(pl[3] >> 32) },
+                                   { (uint32_t)any, (uint32_t)(any >> 32) } };
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { // sources 8q .. 8q + 7
+            uint32_t wd[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                uint32_t pair = 0u;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int src = 8 * q + 2 * t + u, hw = src >> 5, b = src & 31;
+                    const uint32_t e = ((h[0][hw] >> b) & 1u) | (((h[1][hw] >> b) & 1u) << 1) |
+                                       (((h[2][hw] >> b) & 1u) << 2) | (((h[3][hw] >> b) & 1u) << 3);
+                    pair |= (((h[4][hw] >> b) & 1u) ? e : 0xffffu) << (16 * u);
+                }
+                wd[t] = pair;
+            }
+            s_rows[lane * MS_ROW_PAD + q] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+        }
+        const unsigned long long has = wg_ballot(any != 0ull);
+        wg_wave_sync();
+        uint4* out = reinterpret_cast<uint4*>(depth + w * 64 * WG_MS_MAX_SOURCES);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t idx = (uint32_t)i * WG_WAVE + (uint32_t)lane; // uint4 of the wavefront's 8 KB
+            const uint32_t row = idx >> 3;
+            if ((has >> row) & 1ull)
+                out[idx] = s_rows[row * MS_ROW_PAD + (idx & 7u)];
+        }
+        wg_wave_sync();
     }
 }
 
@@ -385,6 +405,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         ctx->ms[1] = front;
         ctx->ms[2] = nl;
         ctx->app = WG_APP_MSBFS;
+        ctx->ms_nl_clean = false;
     }
     droots = (char*)nl + (size_t)V * 8;
     if (!ctx->ms[3] && (rc = ms_alloc(ctx, (size_t)V * 8 * MS_SAVE, &ctx->ms[3])))
@@ -399,15 +420,17 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         // 0) and k_ms_rows writes the depth rows after the run, whole.  When
         // the slots run out, k_ms_rows writes the rows of the kept levels and
         // the later levels write theirs as they commit, into front.
-        ms_saved kept = {};
         uint32_t nsave = 1, level = 0;
         bool keeping = true;
         unsigned long long* cur = save; // the last level's new sources (the frontier masks)
         WG_HIP(ctx, hipMemcpyAsync(droots, hroots.data(), (size_t)nroots * 4, hipMemcpyHostToDevice, ctx->stream));
+        // nl is zero after a completed run (each commit clears what it takes)
         if ((rc = wg_clear_u64(ctx, (uint64_t*)seen, V)) || (rc = wg_clear_u64(ctx, (uint64_t*)save, V)) ||
-            (rc = wg_clear_u64(ctx, (uint64_t*)nl, V)) || (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
+            (!ctx->ms_nl_clean && (rc = wg_clear_u64(ctx, (uint64_t*)nl, V))) ||
+            (rc = wg_clear_u64(ctx, ctx->bm_next, ctx->bm_words)) ||
             (rc = wg_fill_u32(ctx, ctx->wedge, ctx->wedge_words, 0u)))
             goto out;
+        ctx->ms_nl_clean = false; // until this run completes
         if ((rc = wg_reset_counters(ctx, RST_ACTIVITY)))
             goto out;
         hipLaunchKernelGGL(k_ms_roots, dim3((nroots + 63) / 64), dim3(64), 0, ctx->stream, (const uint32_t*)droots,
@@ -428,7 +451,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             }
             if (keeping && nsave == MS_SAVE) { // no slot left: the kept levels' rows now
                 hipLaunchKernelGGL(k_ms_rows, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream,
-                                   (const unsigned long long*)save, nsave, kept, V, (uint16_t*)depth);
+                                   (const unsigned long long*)save, nsave, V, (uint16_t*)depth);
                 if ((rc = wg_check_launch(ctx)))
                     goto out;
                 keeping = false;
@@ -465,7 +488,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
             if ((rc = wg_check_launch(ctx)) || (rc = wg_take_activity(ctx))) // the counters start the next level at 0
                 goto out;
             if (keeping)
-                kept.level[nsave++] = level + 1;
+                ++nsave; // slot nsave - 1 holds level nsave - 1
             cur = out;
             ++level;
             if (ctx->hctr->activated == 0)
@@ -477,7 +500,7 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         }
         if (keeping) {
             hipLaunchKernelGGL(k_ms_rows, dim3(cgrid), dim3(WG_BLOCK), 0, ctx->stream,
-                               (const unsigned long long*)save, nsave, kept, V, (uint16_t*)depth);
+                               (const unsigned long long*)save, nsave, V, (uint16_t*)depth);
             if ((rc = wg_check_launch(ctx)))
                 goto out;
         }
@@ -493,8 +516,10 @@ extern "C" int wg_msbfs(wg_ctx* ctx, const uint64_t* roots, uint32_t nroots, con
         }
     }
 out:
-    if (rc == WG_OK)
+    if (rc == WG_OK) {
         WG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->ms_nl_clean = true;
+    }
     return rc;
 }
 

@@ -195,6 +195,7 @@ struct wg_ctx {
     void* prop[2];            // PR contributions (double buffer) / BFS packed / CC labels / SSSP dist
     void* ms[4];              // batched sources: per-vertex masks or rows (freed with the app state)
     uint32_t ms_n;            // batched sources: number of roots / seeds
+    bool ms_nl_clean;         // 64-source BFS: nl is all zero (the last run completed)
     size_t ms_aux_bytes;      // personalised PageRank: size of ms[2] (its reduction partials)
     void* idmap;              // wg_to_device_ids: original ids in, device ids out (grown as needed)
     size_t idmap_bytes;
