@@ -115,6 +115,17 @@ int wg_sync(wg_ctx* ctx);
                                   wg_work_counts reports (kernels built with the
                                   counts, a little slower); 0 (default): they do not.
                                   May be changed between runs */
+#define WG_OPT_MIN_BINNED 5    /* dense CC / SSSP pulls (wg_edge_pull_cc / _sssp in WG_MODE_DENSE,
+                                  and the dense iterations of wg_converge) on the binned
+                                  layout of WG_PR_BINNED: per window-destination pair the
+                                  minimum of the window's offers (label, or distance plus
+                                  weight), then per bin the minimum against the current
+                                  value.  A dense pull then reads the values of the
+                                  previous iteration (the tile pull may also see values
+                                  lowered earlier in the same pull), so iteration counts
+                                  can differ; the fixpoint is the same.  0 off, 1 on,
+                                  2 (default) on for graphs of 2^24 edges or more and at
+                                  least 8 edges per vertex */
 int wg_set_option(wg_ctx* ctx, int option, int64_t value);
 
 /* Uploads a destination-grouped CSC and builds every derived device structure:

@@ -57,6 +57,7 @@ OPT_RELABEL = 1
 OPT_PR_BIN_SLICE = 2
 OPT_MS_BINNED = 3
 OPT_COUNT_WORK = 4
+OPT_MIN_BINNED = 5
 RELABEL_OFF, RELABEL_ON, RELABEL_AUTO = 0, 1, 2
 
 UNREACHED_U64 = np.uint64(0xFFFFFFFFFFFFFFFF)

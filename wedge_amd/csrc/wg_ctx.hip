@@ -283,6 +283,11 @@ extern "C" int wg_set_option(wg_ctx* ctx, int option, int64_t value)
             return WG_E_INVAL;
         ctx->ms_binned = (int)value + 1; // 0 = unset: the default
         return WG_OK;
+    case WG_OPT_MIN_BINNED:
+        if (value < 0 || value > 2)
+            return WG_E_INVAL;
+        ctx->min_binned = (int)value + 1; // 0 = unset: the default
+        return WG_OK;
     case WG_OPT_COUNT_WORK:
         if (value < 0 || value > 1)
             return WG_E_INVAL;

@@ -217,6 +217,7 @@ struct wg_ctx {
     struct wg_pb* pb;         // layout of the binned PageRank pull (wg_pb.hip), built on first use
     uint32_t pb_slice;        // option: pairs per pull workgroup of that layout (0 = default)
     int ms_binned;            // option WG_OPT_MS_BINNED + 1 (0 = unset: the default)
+    int min_binned;           // option WG_OPT_MIN_BINNED + 1 (0 = unset: the default)
     int count_work;           // option WG_OPT_COUNT_WORK
     uint64_t bm_words;        // 64-bit words per vertex bitmask
     uint64_t* bm_cur;         // BFS: current frontier
@@ -375,6 +376,13 @@ int wg_pb_stats(const wg_ctx* ctx, uint64_t* pairs, uint64_t* code_entries, uint
 // sources = level when rows is set (else the caller writes them from
 // front_out), the frontier vertex bitmask (bm_next) and the level's counts in
 // ctx->ctr
+// dense CC / SSSP pulls on the binned layout (WG_OPT_MIN_BINNED): whether the
+// context's graph and application use them, the layout (with the weights in
+// code order) built ahead of a loop, and one dense pull over ctx->prop[0]
+// (push and pull kernels gated by `gate`, as the tile pull they replace)
+bool wg_pb_min_on(const wg_ctx* ctx);
+int wg_pb_min_prepare(wg_ctx* ctx);
+int wg_pb_min_pull(wg_ctx* ctx, int gate);
 int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front_in, unsigned long long* front_out,
                       unsigned long long* seen, uint16_t* depth, unsigned long long live, uint32_t level, bool rows);
 // releases the binned layout (graph change)

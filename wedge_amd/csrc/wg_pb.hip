@@ -60,6 +60,8 @@
 //   bin_slot[b]    {first slot, slices} of a bin cut into several slices: each
 //                  slice writes its accumulators to its slot, and the last one
 //                  to finish (a per-bin ticket) adds the slots in slice order
+//   wq[q]          on a weighted graph, code entry q's edge weight (FP64 or
+//                  u32; 0 for padding), for the binned SSSP pull
 #include "wg_frontier.h"
 
 #include <hipcub/hipcub.hpp>
@@ -97,6 +99,7 @@ struct wg_pb {
     double* slot_acc;
     unsigned int* bin_ticket;
     double* bin_part;
+    void* wq;
 };
 
 namespace {
@@ -123,6 +126,7 @@ void pb_release(wg_pb* pb)
     pb_free_dev(pb->slot_acc);
     pb_free_dev(pb->bin_ticket);
     pb_free_dev(pb->bin_part);
+    pb_free_dev(pb->wq);
 }
 
 struct pb_tmp {
@@ -202,6 +206,20 @@ __global__ void k_pb_codes(uint64_t nq, const uint32_t* __restrict__ tile_win, c
             c = (uint16_t)((src[e] & wmask) | (end ? WG_PB_END : 0u));
         }
         code[q] = c;
+    }
+}
+
+// wq[q] = the weight of code entry q's edge (0 for padding).
+template <typename W>
+__global__ void k_pb_weights(uint64_t nq, const uint32_t* __restrict__ tile_win, const uint64_t* __restrict__ ws,
+                             const uint64_t* __restrict__ pq, const uint32_t* __restrict__ perm,
+                             const W* __restrict__ w, W* __restrict__ wq)
+{
+    for (uint64_t q = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; q < nq; q += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t s;
+        uint64_t r, cnt;
+        const uint64_t p = pb_entry(q, tile_win, ws, pq, &s, &r, &cnt);
+        wq[q] = p != ~0ull ? w[perm[p]] : W(0);
     }
 }
 
@@ -384,6 +402,20 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
                            (const uint64_t*)pqd.p, (const uint32_t*)perm.p, ctx->src, ctx->dst,
                            (1u << ws_log) - 1u, pb->code);
         PB_TRY(hipGetLastError());
+        if (ctx->w && ctx->wkind != WG_WEIGHT_NONE) { // weights in code order, for the binned SSSP pull
+            if (ctx->wkind == WG_WEIGHT_F64) {
+                PB_TRY(hipMalloc(&pb->wq, nq * sizeof(double)));
+                hipLaunchKernelGGL(k_pb_weights<double>, dim3(gq), dim3(256), 0, st, nq, (const uint32_t*)twin.p,
+                                   (const uint64_t*)wsd.p, (const uint64_t*)pqd.p, (const uint32_t*)perm.p,
+                                   (const double*)ctx->w, (double*)pb->wq);
+            } else {
+                PB_TRY(hipMalloc(&pb->wq, nq * sizeof(uint32_t)));
+                hipLaunchKernelGGL(k_pb_weights<uint32_t>, dim3(gq), dim3(256), 0, st, nq, (const uint32_t*)twin.p,
+                                   (const uint64_t*)wsd.p, (const uint64_t*)pqd.p, (const uint32_t*)perm.p,
+                                   (const uint32_t*)ctx->w, (uint32_t*)pb->wq);
+            }
+            PB_TRY(hipGetLastError());
+        }
         PB_TRY(hipMalloc(&pidx.p, nq * sizeof(uint32_t)));
         hipcub::TransformInputIterator<uint32_t, pb_end_flag, const uint16_t*> flags(pb->code, pb_end_flag());
         size_t tb = 0;
@@ -621,12 +653,22 @@ struct pb_or {
     static __device__ __forceinline__ unsigned long long bits(V x) { return x; }
     static __device__ __forceinline__ V from_bits(unsigned long long x) { return x; }
 };
+// Minimum of CC labels or SSSP distances (P::T: u32, or FP64 bit patterns,
+// which order as the distances do since they are never negative).
+template <class P>
+struct pb_min {
+    typedef typename P::T V;
+    static __device__ __forceinline__ V zero() { return P::INF; }
+    static __device__ __forceinline__ V op(V a, V b) { return b < a ? b : a; }
+    static __device__ __forceinline__ unsigned long long bits(V x) { return (unsigned long long)x; }
+    static __device__ __forceinline__ V from_bits(unsigned long long x) { return (V)x; }
+};
 
 // Cross-lane moves for the push's segmented scan, as DPP controls (GFX9
 // encodings): row_shr:n shifts within rows of 16 lanes, row_bcast:15 / :31
 // hand the last lane of a row (of two rows) to the next row(s), wave_shr:1
 // shifts the whole wavefront by one lane.  A lane with no source (or a row
-// left out by the row mask) receives 0, the identity of both combines.
+// left out by the row mask) receives the combine's identity (OP::zero()).
 #define WG_DPP_ROW_SHR(n) (0x110 + (n))
 #define WG_DPP_ROW_BCAST15 0x142
 #define WG_DPP_ROW_BCAST31 0x143
@@ -635,9 +677,11 @@ struct pb_or {
 template <class OP, int CTRL, int ROWS>
 __device__ __forceinline__ typename OP::V pb_dpp_v(typename OP::V v)
 {
-    const unsigned long long x = OP::bits(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)x, CTRL, ROWS, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(x >> 32), CTRL, ROWS, 0xf, false);
+    const unsigned long long x = OP::bits(v), z = OP::bits(OP::zero());
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)z, (int)(unsigned)x, CTRL, ROWS, 0xf, false);
+    if constexpr (sizeof(typename OP::V) == 4)
+        return OP::from_bits((unsigned)lo);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(z >> 32), (int)(unsigned)(x >> 32), CTRL, ROWS, 0xf, false);
     return OP::from_bits(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 // One step of the segmented inclusive scan: (v, f) of this lane combined
@@ -694,12 +738,15 @@ __device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, b
     const uint4 raw = in.codes;
     const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
                                    raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
-    const uint32_t q0 = t * WG_PB_TILE + (uint32_t)wg_lane() * WG_PB_IPT;
+    // code index of each entry, for loaders that read per-entry data (edge
+    // weights): an invalid tile's t is past the task (its codes are another
+    // tile's), so its entries use index 0, which every layout has
+    const uint32_t q0 = valid ? t * WG_PB_TILE + (uint32_t)wg_lane() * WG_PB_IPT : 0u;
     V run = OP::zero();
     bool has_end = false;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        const V x = load(s_win[c[j] & wmask]);
+        const V x = load(s_win[c[j] & wmask], valid ? q0 + (uint32_t)j : 0u);
         const bool real = valid && q0 + (uint32_t)j < wend;
         w.v[j] = real ? x : OP::zero();
         w.e[j] = real && (c[j] & WG_PB_END);
@@ -851,7 +898,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __res
         pb_fill_lds<WS>(s_win, cin + v0, nv, [](T x) { return x; });
         __syncthreads();
         pb_push_tiles<pb_sum>(code, s_win, (uint32_t)WS - 1u, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
-                              [](T x) { return (double)x; });
+                              [](T x, uint32_t) { return (double)x; });
     }
 }
 
@@ -1140,7 +1187,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_or(const uint16_t* __
         pb_fill_lds<WS>(s_win, front + v0, nv, [live](unsigned long long x) { return x & live; });
         __syncthreads();
         pb_push_tiles<pb_or>(code, s_win, (uint32_t)WS - 1u, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
-                             [](unsigned long long x) { return x; });
+                             [](unsigned long long x, uint32_t) { return x; });
     }
 }
 
@@ -1249,4 +1296,180 @@ int wg_pb_msbfs_level(wg_ctx* ctx, const unsigned long long* front_in, unsigned 
                            reinterpret_cast<unsigned long long*>(pb->slot_acc), pb->bin_ticket, seen, front_out,
                            depth, ctx->bm_next, ctx->od, live, level, ctx->ctr, V);
     return wg_check_launch(ctx);
+}
+
+// ---- dense CC / SSSP pulls (minimum) ---------------------------------------------
+//
+// A dense pull of perform_edge_pull_phase_cc / _sssp (source/phase_edge_pull_cc.c,
+// source/phase_edge_pull_sssp.c): every destination takes the minimum of what
+// its in-edges offer (the source's label, or its distance plus the edge
+// weight) when that is below its value, and is activated.  On the binned
+// layout the push takes, per (window, destination) pair, the minimum of the
+// window's offers (the window's values in LDS, the weights streamed in code
+// order) and the pull the minimum of a bin's pairs with LDS atomics, then
+// applies it: a lower value is stored, the vertex is set in `next` and
+// counted (out-degree, and whether it was reached for the first time) as the
+// tile pull counts it.  The push reads the values the previous pull left, so
+// the pull is a Jacobi step (the tile pull may see values lowered earlier in
+// the same pull); the fixpoint is the same, since minimum is idempotent.
+
+// Push: the window holds values (labels, or distances as bit patterns).
+template <bool CW, class P, int WS>
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push_min(const uint16_t* __restrict__ code,
+                                                               const typename P::T* __restrict__ prop,
+                                                               const void* __restrict__ wq, const uint4* __restrict__ task,
+                                                               uint32_t ntasks, const uint32_t* __restrict__ tile_pbase,
+                                                               typename P::T* __restrict__ part, uint32_t V,
+                                                               wg_dev_counters* __restrict__ ctr,
+                                                               unsigned int* __restrict__ ovf, int gate)
+{
+    typedef typename P::T T;
+    if (wg_gated_off(ctr, gate))
+        return;
+    __shared__ T s_win[WS];
+    __shared__ T s_stage[WG_PB_WAVES][WG_PB_STAGE];
+    work_tally<CW> work;
+    for (uint32_t k = blockIdx.x; k < ntasks; k += gridDim.x) {
+        const uint4 tk = task[k];
+        const uint32_t v0 = tk.x * (uint32_t)WS;
+        const uint32_t nv = (V - v0 < (uint32_t)WS) ? V - v0 : (uint32_t)WS;
+        __syncthreads(); // the previous task's wavefronts are done with s_win
+        pb_fill_lds<WS>(s_win, prop + v0, nv, [](T x) { return x; });
+        __syncthreads();
+        pb_push_tiles<pb_min<P>>(code, s_win, (uint32_t)WS - 1u, tk, tile_pbase, part, s_stage[threadIdx.x / WG_WAVE],
+                                 [&](T du, uint32_t q) { return P::val_from(du, wq, q, ovf); });
+        if (threadIdx.x == 0) { // the task's edges: its real entries
+            const uint32_t end = tk.z * WG_PB_TILE < tk.w ? tk.z * WG_PB_TILE : tk.w;
+            work.edges += end - tk.y * WG_PB_TILE;
+        }
+    }
+    add_work(work, ctr);
+}
+
+// Pull: the minimum of a slice's pairs per destination in LDS; the workgroup
+// holding a bin's complete minima applies them to the bin's vertices.
+template <bool CW, class P, int WD>
+__global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull_min(
+    const typename P::T* __restrict__ part, const uint16_t* __restrict__ pdst, const uint32_t* __restrict__ chunk_block,
+    const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs, const uint4* __restrict__ slice,
+    uint32_t nslices, const uint2* __restrict__ bin_slot, typename P::T* __restrict__ slot_acc,
+    unsigned int* __restrict__ bin_ticket, typename P::T* __restrict__ prop, const uint32_t* __restrict__ od,
+    uint64_t* __restrict__ next, uint32_t V, wg_dev_counters* __restrict__ ctr, int gate)
+{
+    typedef typename P::T T;
+    if (wg_gated_off(ctr, gate))
+        return;
+    __shared__ T s_acc[WD];
+    __shared__ int s_flag;
+    act_tally t = { 0ull, 0ull, 0ull };
+    work_tally<CW> work;
+    for (uint32_t k = blockIdx.x; k < nslices; k += gridDim.x) {
+        const uint4 sl = slice[pb_slice_of(k, nslices)];
+        const uint32_t b = sl.x, v0 = b * (uint32_t)WD;
+        const uint32_t nv = (V - v0 < (uint32_t)WD) ? V - v0 : (uint32_t)WD;
+        __syncthreads(); // the previous slice is done with s_acc
+        for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
+            s_acc[i] = P::INF;
+        __syncthreads();
+        pb_pull_range<8>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z, [&](uint32_t d, T x) {
+            if (x < s_acc[d])
+                atomicMin(&s_acc[d], x);
+        });
+        __syncthreads();
+        if (sl.w != WG_PB_NOSLOT &&
+            !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
+                            [](T a, T x) { return x < a ? x : a; }))
+            continue;
+        if (threadIdx.x == 0)
+            work.dests += nv;
+        constexpr int R = WD / WG_PB_THREADS;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * WG_PB_THREADS; // a wavefront: 64 consecutive vertices
+            bool changed = false;
+            if (i < nv) {
+                const T m = s_acc[i];
+                if (m != P::INF) {
+                    const T old = prop[v0 + i];
+                    if (m < old) { // the bin's vertices are this workgroup's alone
+                        prop[v0 + i] = m;
+                        changed = true;
+                        if (old == P::INF)
+                            t.fresh += 1ull;
+                    }
+                }
+            }
+            activate_contiguous(next, od, v0 + i - (uint32_t)wg_lane(), changed, t);
+        }
+    }
+    add_tally(t, ctr);
+    add_work(work, ctr);
+}
+
+bool wg_pb_min_on(const wg_ctx* ctx)
+{
+    if (ctx->app != WG_APP_CC && ctx->app != WG_APP_SSSP)
+        return false;
+    if (ctx->app == WG_APP_SSSP && ctx->wkind == WG_WEIGHT_NONE)
+        return false;
+    const int opt = ctx->min_binned ? ctx->min_binned - 1 : 2;
+    return opt == 1 || (opt == 2 && ctx->E >= (1ull << 24) && ctx->E >= 8ull * ctx->V);
+}
+
+int wg_pb_min_prepare(wg_ctx* ctx)
+{
+    int rc = pb_prepare_sizes(ctx, 14, 14); // 16K values per window, 16K minima per bin
+    if (rc)
+        return rc;
+    if (ctx->app == WG_APP_SSSP && ctx->pb->nq && !ctx->pb->wq) // built before the weights were loaded
+        return WG_E_STATE;
+    return WG_OK;
+}
+
+template <class P>
+static int pb_min_launch(wg_ctx* ctx, int gate)
+{
+    typedef typename P::T T;
+    const wg_pb* pb = ctx->pb;
+    const uint32_t V = (uint32_t)ctx->V;
+    T* part = reinterpret_cast<T*>(pb->part);
+    T* prop = (T*)ctx->prop[0];
+    // gated-off launches stay cheap: at most 4 workgroups per CU, each looping over tasks / slices
+    const uint32_t cap = (uint32_t)ctx->num_cu * 4u;
+    const uint32_t pg = pb->ntasks < cap ? pb->ntasks : cap, lg = pb->nslices < cap ? pb->nslices : cap;
+    if (pb->ntasks) {
+        if (ctx->count_work)
+            hipLaunchKernelGGL((k_pb_push_min<true, P, 16384>), dim3(pg), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code,
+                               (const T*)prop, pb->wq, pb->task, pb->ntasks, pb->tile_pbase, part, V, ctx->ctr,
+                               &ctx->ctr->overflow, gate);
+        else
+            hipLaunchKernelGGL((k_pb_push_min<false, P, 16384>), dim3(pg), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code,
+                               (const T*)prop, pb->wq, pb->task, pb->ntasks, pb->tile_pbase, part, V, ctx->ctr,
+                               &ctx->ctr->overflow, gate);
+    }
+    int rc = wg_check_launch(ctx);
+    if (rc)
+        return rc;
+    if (ctx->count_work)
+        hipLaunchKernelGGL((k_pb_pull_min<true, P, 16384>), dim3(lg), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const T*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs,
+                           pb->slice, pb->nslices, pb->bin_slot, reinterpret_cast<T*>(pb->slot_acc), pb->bin_ticket,
+                           prop, ctx->od, ctx->bm_next, V, ctx->ctr, gate);
+    else
+        hipLaunchKernelGGL((k_pb_pull_min<false, P, 16384>), dim3(lg), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const T*)part, pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs,
+                           pb->slice, pb->nslices, pb->bin_slot, reinterpret_cast<T*>(pb->slot_acc), pb->bin_ticket,
+                           prop, ctx->od, ctx->bm_next, V, ctx->ctr, gate);
+    return wg_check_launch(ctx);
+}
+
+int wg_pb_min_pull(wg_ctx* ctx, int gate)
+{
+    if (!ctx->pb || ctx->pb->ws_log != 14 || ctx->pb->wd_log != 14)
+        return WG_E_STATE; // wg_pb_min_prepare first
+    if (ctx->app == WG_APP_CC)
+        return pb_min_launch<PolCC>(ctx, gate);
+    if (ctx->wkind == WG_WEIGHT_F64)
+        return pb_min_launch<PolSF>(ctx, gate);
+    return pb_min_launch<PolSU>(ctx, gate);
 }

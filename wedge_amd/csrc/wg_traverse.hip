@@ -512,70 +512,6 @@ __global__ void k_count_clear(uint64_t* __restrict__ bm, uint64_t nwords, wg_dev
 
 // ---- CC / SSSP (min propagation) -------------------------------------------------
 
-// val_from(du, w, p): the value edge p offers its destination when its
-// source holds du; val reads du from prop.
-struct PolCC {
-    typedef uint32_t T;
-    static constexpr uint32_t INF = 0xffffffffu;
-    static constexpr bool HUB = true; // dense pulls read the first labels from LDS
-    static constexpr int DENSE_IPT = WG_IPT; // 512-edge tiles (11.2 ms against 11.6 with 256 on rmat-s24 CC)
-    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
-    {
-        (void)w;
-        (void)p;
-        (void)ovf;
-        return du;
-    }
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
-    {
-        return val_from(prop[u], w, p, ovf);
-    }
-};
-
-struct PolSF { // FP64 distances as bit patterns
-    typedef unsigned long long T;
-    static constexpr unsigned long long INF = 0x7FF0000000000000ull;
-    static constexpr bool HUB = false;
-    static constexpr int DENSE_IPT = 4; // 256-edge tiles (rmat-s24 SSSP 37.5 -> 32.1 ms)
-    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
-    {
-        (void)ovf;
-        if (du >= INF)
-            return INF;
-        const double s = __longlong_as_double((long long)du) + ((const double*)w)[p];
-        return (T)__double_as_longlong(s);
-    }
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
-    {
-        return val_from(prop[u], w, p, ovf);
-    }
-};
-
-struct PolSU { // 32-bit integer distances
-    typedef uint32_t T;
-    static constexpr uint32_t INF = 0xffffffffu;
-    static constexpr bool HUB = false;
-    static constexpr int DENSE_IPT = 4;
-    __device__ static T val_from(T du, const void* __restrict__ w, uint32_t p, unsigned int* __restrict__ ovf)
-    {
-        if (du == INF)
-            return INF;
-        const uint32_t s = du + ((const uint32_t*)w)[p];
-        if (s < du || s == INF) {
-            *ovf = 1u;
-            return INF;
-        }
-        return s;
-    }
-    __device__ static T val(const T* __restrict__ prop, const void* __restrict__ w, uint32_t u, uint32_t p,
-                            unsigned int* __restrict__ ovf)
-    {
-        return val_from(prop[u], w, p, ovf);
-    }
-};
-
 // Values of the first vertices in LDS for the dense pulls (policies with HUB):
 // after the degree relabelling they are the sources of a large share of the
 // edges.  The copy is taken when the pull starts; a value that drops during the
@@ -1136,11 +1072,13 @@ __global__ __launch_bounds__(WG_BLOCK) void k_min_level(const uint32_t* __restri
                                                          const uint32_t* __restrict__ tile_e, uint32_t ntiles, uint32_t E,
                                                          uint64_t* __restrict__ next, uint32_t* __restrict__ wedge,
                                                          wg_dev_counters* __restrict__ ctr, unsigned int* __restrict__ ovf,
-                                                         uint32_t nhub)
+                                                         uint32_t nhub, int binned)
 {
     if (wg_gated_off(ctr, WG_GATE_LIVE))
         return;
     if (ctr->mode == WG_MODE_DENSE) {
+        if (binned) // the binned push and pull launched ahead of this kernel did the dense pull
+            return;
         __shared__ typename P::T s_hub[P::HUB ? WG_MIN_HUB_BYTES / sizeof(typename P::T) : 1];
         stage_hub<P>(prop, nhub, s_hub);
         min_tile_body<CW, P, IPT>(off, src, w, prop, od, tile_v, tile_e, ntiles, next, ctr, ovf, s_hub, nhub);
@@ -2130,7 +2068,14 @@ template <class P>
 static int min_pull(wg_ctx* ctx, int mode, void* prop)
 {
     int rc;
-    if (mode == WG_MODE_DENSE) {
+    if (mode == WG_MODE_DENSE && wg_pb_min_on(ctx)) {
+        if ((rc = wg_pb_min_prepare(ctx)))
+            return rc;
+        int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
+        if ((rc = wg_pb_min_pull(ctx, WG_GATE_ALWAYS)))
+            return rc;
+        wg_prof_end(ctx, WG_K_MIN_TILE, slot);
+    } else if (mode == WG_MODE_DENSE) {
         int slot = wg_prof_begin(ctx, WG_K_MIN_TILE);
         const min_tiling mt = dense_min_tiling<P>(ctx);
         if (mt.ipt == 4)
@@ -2300,10 +2245,16 @@ static int enqueue_iteration(wg_ctx* ctx)
                            ctx->root_off, ctx->csri_off, ctx->csr_dst);
     } else {
         const uint32_t grid = ctx->pr_grid > qgrid ? ctx->pr_grid : qgrid;
+        const int binned = wg_pb_min_on(ctx) ? 1 : 0; // the layout was built by wg_converge
+        if (binned) {
+            int rc = wg_pb_min_pull(ctx, WG_MODE_DENSE);
+            if (rc)
+                return rc;
+        }
 #define WG_MIN_LEVEL_IPT(P, T, IPT)                                                                                   \
     hipLaunchKernelGGL(WG_CW(ctx, k_min_level, P, IPT), dim3(grid), dim3(WG_BLOCK), 0, ctx->stream, ctx->off, ctx->src, ctx->dst, \
                        ctx->w, (T*)ctx->prop[0], ctx->od, mt.tile_v, mt.tile_e, mt.ntiles, (uint32_t)ctx->E,            \
-                       ctx->bm_next, ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx))
+                       ctx->bm_next, ctx->wedge, ctx->ctr, &ctx->ctr->overflow, min_hub_count<P>(ctx), binned)
 #define WG_MIN_LEVEL(P, T)                          \
     do {                                            \
         const min_tiling mt = dense_min_tiling<P>(ctx); \
@@ -2389,6 +2340,8 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
         return WG_E_INVAL;
     int rc;
     if (ctx->wedge_log2 >= 0 && (rc = clear_wedge_state(ctx)))
+        return rc;
+    if (wg_pb_min_on(ctx) && (rc = wg_pb_min_prepare(ctx))) // dense iterations on the binned layout
         return rc;
     if (ctx->run_wgs[ctx->app & 7]) {
         if (!ctx->run_q) {
