@@ -25,9 +25,11 @@
 //   pull     per bin: a workgroup gathers the partials of every window (one
 //            run of consecutive positions per window) into LDS accumulators,
 //            then runs the vertex phase on the bin's vertices
-// Bytes per iteration: 2 per edge, 18 per pair (the partial written and read
-// back, and its 2-byte destination), the windows' LDS copies, and the vertex
-// phase's 16 (FP64) or 12 (FP32) per vertex plus offsets-free out-degrees.
+// Bytes per iteration: 2 per edge, 18 per pair (the FP64 partial written and
+// read back, and its 2-byte destination; 10 with the FP32-stored
+// contributions, whose partials are stored in FP32), the windows' LDS copies,
+// and the vertex phase's 16 (FP64) or 12 (FP32) per vertex plus offsets-free
+// out-degrees.
 // After the degree relabelling most edges of a window share their destination
 // with other edges of that window (rmat-s24: 0.27 pairs per edge with 16K-vertex
 // windows, rmat-s27: 0.40), so an iteration moves about 7-11 bytes per edge.
@@ -788,9 +790,9 @@ __device__ __forceinline__ void pb_tile_scan2(pb_lane_tile<OP>& a, pb_lane_tile<
 // row, so that each store instruction writes 64 consecutive partials (one
 // lane's pairs are consecutive, 64 lanes' are not: stored directly, each
 // 8-byte store was its own L2 request).
-template <class OP>
-__device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lane_tile<OP>& w,
-                                              typename OP::V* __restrict__ part, typename OP::V* s_stage)
+template <class OP, typename PT>
+__device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lane_tile<OP>& w, PT* __restrict__ part,
+                                              typename OP::V* s_stage)
 {
     typedef typename OP::V V;
     const int lane = wg_lane();
@@ -822,7 +824,7 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint32_t n = all - h < WG_PB_STAGE ? all - h : WG_PB_STAGE;
         for (uint32_t k = (uint32_t)lane; k < n; k += WG_WAVE)
-            part[in.pbase + h + k] = s_stage[k];
+            part[in.pbase + h + k] = (PT)s_stage[k];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // reads done before the row is written again
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -832,10 +834,10 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
 // Two tiles of one wavefront, their phases interleaved (independent
 // instructions for the scheduler: a wavefront's tile is a chain of dependent
 // steps, and a CU holds only the 16 wavefronts of one workgroup).
-template <class OP, typename W, class LOAD>
+template <class OP, typename W, typename PT, class LOAD>
 __device__ __forceinline__ void pb_push_pair(const pb_tile_in& x, uint32_t tx, const pb_tile_in& y, uint32_t ty,
                                              bool y_valid, const W* s_win, uint32_t wmask, uint32_t wend,
-                                             typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
+                                             PT* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
     pb_lane_tile<OP> a, b;
     pb_tile_read<OP>(x, tx, true, s_win, wmask, wend, a, load);
@@ -851,10 +853,10 @@ __device__ __forceinline__ void pb_push_pair(const pb_tile_in& x, uint32_t tx, c
 // one pair's loads are in flight while the other pair runs.  s_win holds the
 // window (filled by the caller, converted by LOAD on read; wmask = its size
 // - 1).
-template <class OP, typename W, class LOAD>
+template <class OP, typename W, typename PT, class LOAD>
 __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code, const W* s_win, uint32_t wmask,
                                               const uint4 tk, const uint32_t* __restrict__ tile_pbase,
-                                              typename OP::V* __restrict__ part, typename OP::V* s_stage, LOAD load)
+                                              PT* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
     constexpr uint32_t S = WG_PB_WAVES; // tiles between one wavefront's consecutive tiles
     const int lane = wg_lane();
@@ -879,12 +881,15 @@ __device__ __forceinline__ void pb_push_tiles(const uint16_t* __restrict__ code,
     }
 }
 
-// PageRank push: the window holds contributions (FP64, or FP32 stored).
+// PageRank push: the window holds contributions (FP64, or FP32 stored).  A
+// pair's partial is summed in FP64 and stored as T: FP64, or FP32 with the
+// FP32-stored contributions (one more rounding per pair, in the error the
+// FP32 leg reports).
 template <typename T, int WS>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_push(const uint16_t* __restrict__ code, const T* __restrict__ cin,
                                                            const uint4* __restrict__ task, uint32_t ntasks,
                                                            const uint32_t* __restrict__ tile_pbase,
-                                                           double* __restrict__ part, uint32_t V)
+                                                           T* __restrict__ part, uint32_t V)
 {
     static_assert(WG_WAVE * WG_PB_IPT == WG_PB_TILE, "a tile is one wavefront's entries");
     static_assert(WS <= WG_PB_SRC + 1, "window larger than the source field");
@@ -1021,7 +1026,7 @@ __device__ __forceinline__ bool pb_join_slices(X* s_acc, uint32_t nv, uint32_t b
 // k_pb_total adds those.
 template <typename T, int MODE, int WD>
 __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
-    const double* __restrict__ part, const uint16_t* __restrict__ pdst, const uint32_t* __restrict__ chunk_block,
+    const T* __restrict__ part, const uint16_t* __restrict__ pdst, const uint32_t* __restrict__ chunk_block,
     const uint32_t* __restrict__ boff, uint32_t nblocks, uint32_t npairs, const uint4* __restrict__ slice,
     uint32_t nslices, const uint2* __restrict__ bin_slot, double* __restrict__ slot_acc,
     unsigned int* __restrict__ bin_ticket, double* __restrict__ bin_part, T* __restrict__ cout,
@@ -1053,7 +1058,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
             s_acc[i] = 0.0;
         __syncthreads();
         pb_pull_range<8>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
-                         [&](uint32_t d, double x) { atomicAdd(&s_acc[d], x); });
+                         [&](uint32_t d, T x) { atomicAdd(&s_acc[d], (double)x); });
         __syncthreads();
         if (sl.w != WG_PB_NOSLOT &&
             !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
@@ -1110,7 +1115,7 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     if (pb->ntasks) {
         const int slot = wg_prof_begin(ctx, WG_K_PB_PUSH);
         hipLaunchKernelGGL((k_pb_push<T, WS>), dim3(pb->ntasks), dim3(WG_PB_THREADS), 0, ctx->stream, pb->code, cin,
-                           pb->task, pb->ntasks, pb->tile_pbase, pb->part, V);
+                           pb->task, pb->ntasks, pb->tile_pbase, reinterpret_cast<T*>(pb->part), V);
         int rc = wg_check_launch(ctx);
         if (rc)
             return rc;
@@ -1118,11 +1123,13 @@ static int pb_launch(wg_ctx* ctx, int mode, const T* cin, T* cout, double* S_in,
     }
     const int slot = wg_prof_begin(ctx, WG_K_PB_PULL);
     if (mode == 0)
-        hipLaunchKernelGGL((k_pb_pull<T, 0, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
+        hipLaunchKernelGGL((k_pb_pull<T, 0, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const T*)reinterpret_cast<T*>(pb->part),
                            pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
                            cout, nullptr, ctx->od, V, S_in, ctx->damping, N);
     else
-        hipLaunchKernelGGL((k_pb_pull<T, 1, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream, pb->part,
+        hipLaunchKernelGGL((k_pb_pull<T, 1, WD>), dim3(pb->nslices), dim3(WG_PB_THREADS), 0, ctx->stream,
+                           (const T*)reinterpret_cast<T*>(pb->part),
                            pb->pdst, pb->chunk_block, pb->boff, pb->nblocks, (uint32_t)pb->npairs, pb->slice, pb->nslices, pb->bin_slot, pb->slot_acc, pb->bin_ticket, pb->bin_part,
                            nullptr, ctx->acc, ctx->od, V, S_in, ctx->damping, N);
     int rc = wg_check_launch(ctx);
