@@ -643,6 +643,7 @@ __device__ __forceinline__ uint4 pb_load_codes(const uint16_t* __restrict__ code
 // ORs 64-bit frontier masks.
 struct pb_sum {
     typedef double V;
+    static constexpr bool ZERO_BITS_0 = true; // the identity's bit pattern is all zeros
     static __device__ __forceinline__ V zero() { return 0.0; }
     static __device__ __forceinline__ V op(V a, V b) { return a + b; }
     static __device__ __forceinline__ unsigned long long bits(V x) { return (unsigned long long)__double_as_longlong(x); }
@@ -650,6 +651,7 @@ struct pb_sum {
 };
 struct pb_or {
     typedef unsigned long long V;
+    static constexpr bool ZERO_BITS_0 = true;
     static __device__ __forceinline__ V zero() { return 0ull; }
     static __device__ __forceinline__ V op(V a, V b) { return a | b; }
     static __device__ __forceinline__ unsigned long long bits(V x) { return x; }
@@ -660,6 +662,7 @@ struct pb_or {
 template <class P>
 struct pb_min {
     typedef typename P::T V;
+    static constexpr bool ZERO_BITS_0 = false;
     static __device__ __forceinline__ V zero() { return P::INF; }
     static __device__ __forceinline__ V op(V a, V b) { return b < a ? b : a; }
     static __device__ __forceinline__ unsigned long long bits(V x) { return (unsigned long long)x; }
@@ -676,14 +679,17 @@ struct pb_min {
 #define WG_DPP_ROW_BCAST31 0x143
 #define WG_DPP_WAVE_SHR1 0x138
 
+// With every row enabled and an all-zero identity, bound_ctrl writes the zero
+// into lanes without a source, so no old value has to be set up first.
 template <class OP, int CTRL, int ROWS>
 __device__ __forceinline__ typename OP::V pb_dpp_v(typename OP::V v)
 {
-    const unsigned long long x = OP::bits(v), z = OP::bits(OP::zero());
-    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)z, (int)(unsigned)x, CTRL, ROWS, 0xf, false);
+    constexpr bool BC = OP::ZERO_BITS_0 && ROWS == 0xf;
+    const unsigned long long x = OP::bits(v), z = BC ? 0ull : OP::bits(OP::zero());
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)z, (int)(unsigned)x, CTRL, ROWS, 0xf, BC);
     if constexpr (sizeof(typename OP::V) == 4)
         return OP::from_bits((unsigned)lo);
-    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(z >> 32), (int)(unsigned)(x >> 32), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(z >> 32), (int)(unsigned)(x >> 32), CTRL, ROWS, 0xf, BC);
     return OP::from_bits(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 // One step of the segmented inclusive scan: (v, f) of this lane combined
@@ -693,7 +699,7 @@ template <class OP, int CTRL, int ROWS>
 __device__ __forceinline__ void pb_seg_dpp(typename OP::V& v, int& f)
 {
     const typename OP::V o = pb_dpp_v<OP, CTRL, ROWS>(v);
-    const int of = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, false);
+    const int of = __builtin_amdgcn_update_dpp(0, f, CTRL, ROWS, 0xf, ROWS == 0xf);
     if (!f)
         v = OP::op(o, v);
     f |= of;
@@ -732,7 +738,7 @@ struct pb_lane_tile {
 // the window (padding entries read slot 0), so the pass has no branches and
 // two tiles' passes interleave.  An invalid tile (past the task's end) has no
 // entries.
-template <class OP, typename W, class LOAD>
+template <class OP, bool WHOLE, typename W, class LOAD>
 __device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, bool valid, const W* s_win,
                                              uint32_t wmask, uint32_t wend, pb_lane_tile<OP>& w, LOAD load)
 {
@@ -748,9 +754,10 @@ __device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, b
     bool has_end = false;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        const V x = load(s_win[c[j] & wmask], valid ? q0 + (uint32_t)j : 0u);
-        const bool real = valid && q0 + (uint32_t)j < wend;
-        w.v[j] = real ? x : OP::zero();
+        const V x = load(s_win[c[j] & wmask], WHOLE ? q0 + (uint32_t)j : (valid ? q0 + (uint32_t)j : 0u));
+        // a WHOLE tile is valid and ends at or before the window's last real entry
+        const bool real = WHOLE || (valid && q0 + (uint32_t)j < wend);
+        w.v[j] = WHOLE ? x : (real ? x : OP::zero());
         w.e[j] = real && (c[j] & WG_PB_END);
         run = OP::op(run, w.v[j]);
         run = w.e[j] ? OP::zero() : run;
@@ -805,7 +812,9 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
         all += (uint32_t)__popcll(m);
     }
     // values to the LDS row in pair order, WG_PB_STAGE pairs at a time (the
-    // whole wavefront takes the same passes), then coalesced stores
+    // whole wavefront takes the same passes), then coalesced stores; a tile
+    // with at most WG_PB_STAGE pairs (most) needs no range check
+    const bool one = all <= WG_PB_STAGE; // uniform
     for (uint32_t h = 0; h < all; h += WG_PB_STAGE) {
         V r = w.sc;
         uint32_t i = pre;
@@ -813,7 +822,7 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
         for (int j = 0; j < WG_PB_IPT; ++j) {
             r = OP::op(r, w.v[j]);
             if (w.e[j]) {
-                if (i - h < WG_PB_STAGE) // i >= h: unsigned wrap puts i < h out of range
+                if (one || i - h < WG_PB_STAGE) // i >= h: unsigned wrap puts i < h out of range
                     s_stage[i - h] = r;
                 ++i;
                 r = OP::zero();
@@ -840,8 +849,16 @@ __device__ __forceinline__ void pb_push_pair(const pb_tile_in& x, uint32_t tx, c
                                              PT* __restrict__ part, typename OP::V* s_stage, LOAD load)
 {
     pb_lane_tile<OP> a, b;
-    pb_tile_read<OP>(x, tx, true, s_win, wmask, wend, a, load);
-    pb_tile_read<OP>(y, ty, y_valid, s_win, wmask, wend, b, load);
+    // both tiles free of padding and of the task's end (all but a window's
+    // last tile): no per-entry validity selects
+    const bool whole = (tx + 1u) * WG_PB_TILE <= wend && y_valid && (ty + 1u) * WG_PB_TILE <= wend;
+    if (whole) {
+        pb_tile_read<OP, true>(x, tx, true, s_win, wmask, wend, a, load);
+        pb_tile_read<OP, true>(y, ty, true, s_win, wmask, wend, b, load);
+    } else {
+        pb_tile_read<OP, false>(x, tx, true, s_win, wmask, wend, a, load);
+        pb_tile_read<OP, false>(y, ty, y_valid, s_win, wmask, wend, b, load);
+    }
     pb_tile_scan2<OP>(a, b);
     pb_tile_store<OP>(x, a, part, s_stage);
     pb_tile_store<OP>(y, b, part, s_stage);
