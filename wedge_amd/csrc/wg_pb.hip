@@ -948,8 +948,10 @@ __device__ __forceinline__ uint32_t pb_slice_of(uint32_t i, uint32_t n)
 // (chunk c = [64c, 64c + 64)): chunk_block[c] is the block holding 64c, the
 // block-start flags in pdst count the blocks after it, and the chunk's blocks'
 // offsets come from the lanes that loaded them.  R chunks per wavefront are
-// in flight at a time (8 for the PageRank pull, 4 for the 64-source BFS, whose
-// epilogue keeps more registers).
+// in flight at a time: the pull waits on this chain of loads, so more is
+// better as far as registers allow (the PageRank pull at s27: R = 6, 8, 10,
+// 12 took 3.66, 3.60, 3.57, 3.53 ms; 12 for PageRank, 8 for the minimum, 4
+// for the 64-source BFS, whose epilogue keeps more registers).
 template <int R, typename X, class ADD>
 __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
                                               const uint32_t* __restrict__ chunk_block,
@@ -1074,7 +1076,7 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
         __syncthreads();
-        pb_pull_range<8>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
+        pb_pull_range<12>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
                          [&](uint32_t d, T x) { atomicAdd(&s_acc[d], (double)x); });
         __syncthreads();
         if (sl.w != WG_PB_NOSLOT &&
