@@ -950,8 +950,9 @@ __device__ __forceinline__ uint32_t pb_slice_of(uint32_t i, uint32_t n)
 // offsets come from the lanes that loaded them.  R chunks per wavefront are
 // in flight at a time: the pull waits on this chain of loads, so more is
 // better as far as registers allow (the PageRank pull at s27: R = 6, 8, 10,
-// 12 took 3.66, 3.60, 3.57, 3.53 ms; 12 for PageRank, 8 for the minimum, 4
-// for the 64-source BFS, whose epilogue keeps more registers).
+// 12 took 3.66, 3.60, 3.57, 3.53 ms; 12 for FP64 PageRank partials, 8 for
+// FP32 ones (12 took their pull from 3.27 to 4.45 ms) and for the minimum,
+// 4 for the 64-source BFS, whose epilogue keeps more registers).
 template <int R, typename X, class ADD>
 __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const uint16_t* __restrict__ pdst,
                                               const uint32_t* __restrict__ chunk_block,
@@ -1076,8 +1077,8 @@ __global__ __launch_bounds__(WG_PB_THREADS) void k_pb_pull(
         for (uint32_t i = threadIdx.x; i < nv; i += WG_PB_THREADS)
             s_acc[i] = 0.0;
         __syncthreads();
-        pb_pull_range<12>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
-                         [&](uint32_t d, T x) { atomicAdd(&s_acc[d], (double)x); });
+        pb_pull_range<sizeof(T) == 8 ? 12 : 8>(part, pdst, chunk_block, boff, nblocks, npairs, sl.y, sl.z,
+                                               [&](uint32_t d, T x) { atomicAdd(&s_acc[d], (double)x); });
         __syncthreads();
         if (sl.w != WG_PB_NOSLOT &&
             !pb_join_slices(s_acc, nv, b, sl.w, bin_slot, slot_acc, bin_ticket, (uint32_t)WD, &s_flag,
