@@ -55,7 +55,10 @@
 //   pdst[o]        destination - bin start of the pair at bin-order position o,
 //                  bit 15 set where a block starts
 //   chunk_block[c] the block holding position 64c (c <= positions / 64: one
-//                  past the end holds the block count)
+//                  past the end holds the block count), followed by
+//                  WG_PB_CHUNK_OFFS block offsets per chunk: boff of the
+//                  chunk's first blocks, so that the pull of a chunk with at
+//                  most that many blocks needs no dependent load for them
 //   slice[j]       pull work: {bin, first position, end position, accumulator
 //                  slot or WG_PB_NOSLOT when the slice is its bin's only one},
 //                  at most WG_PB_SLICE pairs, a contiguous range of positions
@@ -83,6 +86,7 @@
 #define WG_PB_MAX_WS_LOG 14
 #define WG_PB_BFLAG 0x8000u                     // pdst: a block starts here
 #define WG_PB_DMASK 0x7fffu                     // pdst: destination - bin start
+#define WG_PB_CHUNK_OFFS 8u                     // block offsets kept with each 64-position chunk
 
 struct wg_pb {
     int ws_log, wd_log;
@@ -504,7 +508,7 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
     std::vector<uint32_t> woff(nb ? nb : 1, 0u); // by window-order block
     std::vector<uint32_t> boff(nb ? nb : 1, 0u); // by bin-order block
     const uint64_t nchunks = ((uint64_t)npairs + 63u) / 64u;
-    std::vector<uint32_t> chunk_block(nchunks + 1, (uint32_t)nb);
+    std::vector<uint32_t> chunk_block(nchunks + 1 + nchunks * WG_PB_CHUNK_OFFS, (uint32_t)nb);
     std::vector<uint32_t> bin_out(B + 1, 0u); // first position of each bin in part[]
     const uint32_t slice_max = ctx->pb_slice ? ctx->pb_slice : WG_PB_SLICE;
     std::vector<uint4> slice;
@@ -542,6 +546,11 @@ static int pb_build(wg_ctx* ctx, wg_pb* pb, int ws_log, int wd_log)
             }
         }
         bin_out[B] = o;
+    }
+    for (uint64_t c = 0; c < nchunks; ++c) { // the chunk's first block offsets, after the block numbers
+        const uint64_t k0 = chunk_block[c];
+        for (uint32_t j = 0; j < WG_PB_CHUNK_OFFS; ++j)
+            chunk_block[nchunks + 1 + c * WG_PB_CHUNK_OFFS + j] = k0 + j < nb ? boff[k0 + j] : 0u;
     }
     pb->nslices = (uint32_t)slice.size();
     pb->nslots = nslots;
@@ -945,9 +954,12 @@ __device__ __forceinline__ uint32_t pb_slice_of(uint32_t i, uint32_t n)
 // The pull's pass over one slice, bin-order positions [b0, b1):
 // ADD(destination, partial) for every pair.  Position o of block kb holds pair
 // o - boff[kb] of part[] (window order).  Positions go 64 to a wavefront step
-// (chunk c = [64c, 64c + 64)): chunk_block[c] is the block holding 64c, the
-// block-start flags in pdst count the blocks after it, and the chunk's blocks'
-// offsets come from the lanes that loaded them.  R chunks per wavefront are
+// (chunk c = [64c, 64c + 64)): the block-start flags in pdst count the blocks
+// after the one holding 64c, and the offsets of the chunk's first
+// WG_PB_CHUNK_OFFS blocks, kept with the chunk, come from the lanes that
+// loaded them, so the destinations, the offsets and then the partials are
+// two rounds of loads.  A chunk with more blocks (rare) loads its block
+// number, chunk_block[c], then the blocks' offsets, boff[].  R chunks per wavefront are
 // in flight at a time: the pull waits on this chain of loads, so more is
 // better as far as registers allow (the PageRank pull at s27: R = 6, 8, 10,
 // 12 took 3.66, 3.60, 3.57, 3.53 ms; 12 for FP64 PageRank partials, 8 for
@@ -965,6 +977,7 @@ __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const 
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull; // lanes 0..lane
     const uint32_t wave = threadIdx.x / WG_WAVE;
     const uint32_t c1 = ((b1 - 1u) >> 6) + 1u;
+    const uint32_t* __restrict__ cbo = chunk_block + ((npairs + 63u) >> 6) + 1u;
     for (uint32_t c = (b0 >> 6) + wave; c < c1; c += WG_PB_WAVES * R) {
         uint32_t o[R], pd[R], bo[R];
         X x[R];
@@ -975,28 +988,42 @@ __device__ __forceinline__ void pb_pull_range(const X* __restrict__ part, const 
             pd[r] = 0u;
             bo[r] = 0u;
             if (cc < c1) {
-                const uint32_t k0 = chunk_block[cc], k1 = chunk_block[cc + 1u];
-                const uint32_t nbc = k1 - k0 < (uint32_t)WG_WAVE ? k1 - k0 + 1u : (uint32_t)WG_WAVE;
                 if (o[r] < npairs)
                     pd[r] = pdst[o[r]];
-                if ((uint32_t)lane < nbc && k0 + (uint32_t)lane < nblocks)
-                    bo[r] = boff[k0 + (uint32_t)lane];
+                if ((uint32_t)lane < WG_PB_CHUNK_OFFS)
+                    bo[r] = cbo[(uint64_t)cc * WG_PB_CHUNK_OFFS + (uint32_t)lane];
             }
         }
+        unsigned slow = 0u; // uniform: chunks with more blocks than kept offsets, done below
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            // blocks started after the chunk's first position, up to this one
-            const uint32_t k =
-                (uint32_t)__popcll(wg_ballot(lane > 0 && (pd[r] & WG_PB_BFLAG)) & upto);
+            // blocks started after the chunk's first position: up to this one, and in all
+            const unsigned long long starts = wg_ballot(lane > 0 && (pd[r] & WG_PB_BFLAG));
+            const uint32_t k = (uint32_t)__popcll(starts & upto);
+            if ((uint32_t)__popcll(starts) >= WG_PB_CHUNK_OFFS)
+                slow |= 1u << r;
             const uint32_t off = (uint32_t)__shfl((int)bo[r], (int)k, WG_WAVE);
             x[r] = X(0);
-            if (o[r] >= b0 && o[r] < b1)
+            if (!((slow >> r) & 1u) && o[r] >= b0 && o[r] < b1)
                 x[r] = __builtin_nontemporal_load(&part[o[r] - off]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (o[r] >= b0 && o[r] < b1)
+            if (!((slow >> r) & 1u) && o[r] >= b0 && o[r] < b1)
                 add(pd[r] & WG_PB_DMASK, x[r]);
+        while (slow) { // the block number, then every block offset the chunk needs
+            const uint32_t r = (uint32_t)__builtin_ctz(slow);
+            slow &= slow - 1u;
+            const uint32_t cc = c + r * WG_PB_WAVES;
+            const uint32_t oo = cc * 64u + (uint32_t)lane;
+            const uint32_t pdv = oo < npairs ? pdst[oo] : 0u;
+            const uint32_t k0 = chunk_block[cc];
+            const uint32_t bov = k0 + (uint32_t)lane < nblocks ? boff[k0 + (uint32_t)lane] : 0u;
+            const uint32_t k = (uint32_t)__popcll(wg_ballot(lane > 0 && (pdv & WG_PB_BFLAG)) & upto);
+            const uint32_t off = (uint32_t)__shfl((int)bov, (int)k, WG_WAVE);
+            if (oo >= b0 && oo < b1)
+                add(pdv & WG_PB_DMASK, __builtin_nontemporal_load(&part[oo - off]));
+        }
     }
 }
 
