@@ -744,14 +744,15 @@ struct pb_lane_tile {
 // Tile phase 1 (one wavefront): lane l takes the tile's entries [8l, 8l + 8)
 // and combines them in order; an entry flagged as a pair's last closes a pair,
 // and the value still open at the lane's end is its run.  Every entry reads
-// the window (padding entries read slot 0), so the pass has no branches and
-// two tiles' passes interleave.  An invalid tile (past the task's end) has no
-// entries.
+// the window (padding entries read slot 0), so the pass has no branches.  It
+// runs in two steps: pb_tile_fetch issues the entries' window reads (and the
+// loader's per-entry loads), pb_tile_combine sums them once they are in, so
+// the reads of two tiles can all be in flight before the first sum.  An
+// invalid tile (past the task's end) has no entries.
 template <class OP, bool WHOLE, typename W, class LOAD>
-__device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, bool valid, const W* s_win,
-                                             uint32_t wmask, uint32_t wend, pb_lane_tile<OP>& w, LOAD load)
+__device__ __forceinline__ void pb_tile_fetch(const pb_tile_in& in, uint32_t t, bool valid, const W* s_win,
+                                              uint32_t wmask, pb_lane_tile<OP>& w, LOAD load)
 {
-    typedef typename OP::V V;
     const uint4 raw = in.codes;
     const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
                                    raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
@@ -759,14 +760,28 @@ __device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, b
     // weights): an invalid tile's t is past the task (its codes are another
     // tile's), so its entries use index 0, which every layout has
     const uint32_t q0 = valid ? t * WG_PB_TILE + (uint32_t)wg_lane() * WG_PB_IPT : 0u;
+#pragma unroll
+    for (int j = 0; j < WG_PB_IPT; ++j)
+        w.v[j] = load(s_win[c[j] & wmask], WHOLE ? q0 + (uint32_t)j : (valid ? q0 + (uint32_t)j : 0u));
+}
+
+template <class OP, bool WHOLE>
+__device__ __forceinline__ void pb_tile_combine(const pb_tile_in& in, uint32_t t, bool valid, uint32_t wend,
+                                                pb_lane_tile<OP>& w)
+{
+    typedef typename OP::V V;
+    const uint4 raw = in.codes;
+    const uint32_t c[WG_PB_IPT] = {raw.x & 0xffffu, raw.x >> 16, raw.y & 0xffffu, raw.y >> 16,
+                                   raw.z & 0xffffu, raw.z >> 16, raw.w & 0xffffu, raw.w >> 16};
+    const uint32_t q0 = valid ? t * WG_PB_TILE + (uint32_t)wg_lane() * WG_PB_IPT : 0u;
     V run = OP::zero();
     bool has_end = false;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
-        const V x = load(s_win[c[j] & wmask], WHOLE ? q0 + (uint32_t)j : (valid ? q0 + (uint32_t)j : 0u));
         // a WHOLE tile is valid and ends at or before the window's last real entry
         const bool real = WHOLE || (valid && q0 + (uint32_t)j < wend);
-        w.v[j] = WHOLE ? x : (real ? x : OP::zero());
+        if constexpr (!WHOLE)
+            w.v[j] = real ? w.v[j] : OP::zero();
         w.e[j] = real && (c[j] & WG_PB_END);
         run = OP::op(run, w.v[j]);
         run = w.e[j] ? OP::zero() : run;
@@ -779,23 +794,43 @@ __device__ __forceinline__ void pb_tile_read(const pb_tile_in& in, uint32_t t, b
 // Tile phase 2 for two tiles at once: the segmented inclusive scan of the
 // lanes' runs (a lane with an end starts a new segment): shifts of 1, 2, 4, 8
 // within each row of 16, then the row totals into rows 1 and 3, then the
-// first two rows' total into rows 2 and 3.  Each tile's sc becomes its carry,
-// the scan one lane down (the value flowing into the lane's first entry).
+// first two rows' total into rows 2 and 3.  A lane's f is set once its value
+// reaches back to its segment's start (lane 0's segment starts with the tile),
+// and the later steps leave such a lane as it is: the scan stops when every
+// lane of both tiles has its f.  Most lanes hold a pair's end (a pair averages
+// 2.5 entries), so most tile pairs stop after the first shift.  Each tile's
+// sc becomes its carry, the scan one lane down (the value flowing into the
+// lane's first entry).
+template <class OP>
+__device__ __forceinline__ bool pb_scan_open(const pb_lane_tile<OP>& a, const pb_lane_tile<OP>& b)
+{
+    return wg_ballot(!(a.f && b.f)) != 0ull;
+}
+
 template <class OP>
 __device__ __forceinline__ void pb_tile_scan2(pb_lane_tile<OP>& a, pb_lane_tile<OP>& b)
 {
+    const int first = wg_lane() == 0 ? 1 : 0;
+    a.f |= first;
+    b.f |= first;
     pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(a.sc, a.f);
     pb_seg_dpp<OP, WG_DPP_ROW_SHR(1), 0xf>(b.sc, b.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(a.sc, a.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(b.sc, b.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(a.sc, a.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(b.sc, b.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(a.sc, a.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(b.sc, b.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(a.sc, a.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(b.sc, b.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(a.sc, a.f);
-    pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(b.sc, b.f);
+    if (pb_scan_open(a, b)) {
+        pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(a.sc, a.f);
+        pb_seg_dpp<OP, WG_DPP_ROW_SHR(2), 0xf>(b.sc, b.f);
+        if (pb_scan_open(a, b)) {
+            pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(a.sc, a.f);
+            pb_seg_dpp<OP, WG_DPP_ROW_SHR(4), 0xf>(b.sc, b.f);
+            pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(a.sc, a.f);
+            pb_seg_dpp<OP, WG_DPP_ROW_SHR(8), 0xf>(b.sc, b.f);
+            if (pb_scan_open(a, b)) {
+                pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(a.sc, a.f);
+                pb_seg_dpp<OP, WG_DPP_ROW_BCAST15, 0xa>(b.sc, b.f);
+                pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(a.sc, a.f);
+                pb_seg_dpp<OP, WG_DPP_ROW_BCAST31, 0xc>(b.sc, b.f);
+            }
+        }
+    }
     a.sc = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(a.sc);
     b.sc = pb_dpp_v<OP, WG_DPP_WAVE_SHR1, 0xf>(b.sc);
 }
@@ -861,12 +896,20 @@ __device__ __forceinline__ void pb_push_pair(const pb_tile_in& x, uint32_t tx, c
     // both tiles free of padding and of the task's end (all but a window's
     // last tile): no per-entry validity selects
     const bool whole = (tx + 1u) * WG_PB_TILE <= wend && y_valid && (ty + 1u) * WG_PB_TILE <= wend;
+    // all 16 window reads of the two tiles issued before the first sum (the
+    // scheduler left four in flight, waiting for each group)
     if (whole) {
-        pb_tile_read<OP, true>(x, tx, true, s_win, wmask, wend, a, load);
-        pb_tile_read<OP, true>(y, ty, true, s_win, wmask, wend, b, load);
+        pb_tile_fetch<OP, true>(x, tx, true, s_win, wmask, a, load);
+        pb_tile_fetch<OP, true>(y, ty, true, s_win, wmask, b, load);
+        __builtin_amdgcn_sched_barrier(0);
+        pb_tile_combine<OP, true>(x, tx, true, wend, a);
+        pb_tile_combine<OP, true>(y, ty, true, wend, b);
     } else {
-        pb_tile_read<OP, false>(x, tx, true, s_win, wmask, wend, a, load);
-        pb_tile_read<OP, false>(y, ty, y_valid, s_win, wmask, wend, b, load);
+        pb_tile_fetch<OP, false>(x, tx, true, s_win, wmask, a, load);
+        pb_tile_fetch<OP, false>(y, ty, y_valid, s_win, wmask, b, load);
+        __builtin_amdgcn_sched_barrier(0);
+        pb_tile_combine<OP, false>(x, tx, true, wend, a);
+        pb_tile_combine<OP, false>(y, ty, y_valid, wend, b);
     }
     pb_tile_scan2<OP>(a, b);
     pb_tile_store<OP>(x, a, part, s_stage);
