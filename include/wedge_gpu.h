@@ -239,7 +239,9 @@ int wg_advance_frontier(wg_ctx* ctx, uint64_t* frontier_vertices);
  * host enqueues iterations in growing batches and never waits per iteration.
  * schedule: 0 plain threshold rule, 1 learned (the model of
  * wg_set_schedule_model evaluated on the device; the plain rule if no model is
- * set), 2 always dense, 3 always frontier. */
+ * set; either way a CC / SSSP pull is dense whenever its dense pulls run on the
+ * binned layout, wg_dense_pull_binned, and the activity exceeds E / 64),
+ * 2 always dense, 3 always frontier. */
 typedef struct wg_loop_params {
     int schedule;
     uint32_t threshold_pct;  /* plain rule: dense iff activity > E * pct / 100 */
@@ -247,6 +249,11 @@ typedef struct wg_loop_params {
     uint32_t max_iterations; /* 0 = until convergence */
 } wg_loop_params;
 int wg_converge(wg_ctx* ctx, const wg_loop_params* params, uint64_t* iterations, uint64_t* frontier_iterations);
+/* *on = 1 if the initialised CC / SSSP's dense pulls run on the binned layout
+ * (WG_OPT_MIN_BINNED and the graph's size), else 0.  No reference counterpart:
+ * the learned schedule of the host-driven loop (wg_run_cc / wg_run_sssp with a
+ * trace recorded) reads it to take the dense pulls as wg_converge does. */
+int wg_dense_pull_binned(const wg_ctx* ctx, int* on);
 /* Fused small-frontier levels.  No reference counterpart: an execution
  * option of wg_converge.  While the chosen pull is frontier-filtered and a
  * level is small (at most 1024 activated vertices and 8192 out-edges per

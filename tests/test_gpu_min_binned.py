@@ -161,3 +161,36 @@ def test_binned_and_tile_dense_pulls_reach_the_same_fixpoint_in_the_phase_api():
         assert np.array_equal(out[1], ref)
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("kind", ["rmat_w", "rmat_sym"])
+def test_learned_schedule_takes_binned_dense_pulls_above_e_over_64(ctx, kind):
+    """With the dense pulls on the binned layout, the learned schedule (device
+    loop and host loop, model loaded) pulls dense whenever the activity
+    exceeds E / 64, where the plain rule keeps frontier pulls up to 20 % of E;
+    the results stay the oracle's."""
+    g = _graph(kind)
+    ctx.load(g)
+    cc = kind == "rmat_sym"
+    root = g.max_outdegree_vertex()
+    ref = oracle.Oracle.of(g, 4).cc()[0] if cc else oracle.Oracle.of(g, 4).sssp(root)[0]
+    W.mlp_load()
+    try:
+        for trace in (0, 1):
+            if cc:
+                out, st = ctx.run_cc(schedule="learned", record_trace=trace)
+                assert np.array_equal(out.astype(np.float64), ref)
+            else:
+                out, st = ctx.run_sssp(root, schedule="learned", record_trace=trace)
+                assert _sssp_ok(g, out, ref)
+            print(f"{kind} learned trace={trace}: iterations {st['iterations']} frontier {st['frontier_iterations']}")
+        assert ctx.dense_pull_binned()
+        rows = W.trace_rows()
+        floor, plain = g.num_edges >> 6, g.num_edges * 20 // 100
+        for r in rows:
+            if r["frontier_edges"] > floor:
+                assert r["mode"] == W.MODE_DENSE, r
+        # the rule decided at least one pull the plain rule leaves frontier-filtered
+        assert any(floor < r["frontier_edges"] <= plain for r in rows), [r["frontier_edges"] for r in rows]
+    finally:
+        W.mlp_unload()

@@ -165,6 +165,7 @@ GPU_SYMBOLS = [
     ("wg_converge", C.c_int, [_p, _p, _u64p, _u64p]),
     ("wg_set_fused_levels", C.c_int, [_p, C.c_uint32, C.c_uint32]),
     ("wg_fused_level_count", C.c_int, [_p, _u64p]),
+    ("wg_dense_pull_binned", C.c_int, [_p, C.POINTER(C.c_int)]),
     ("wg_profile_enable", C.c_int, [_p, C.c_int]),
     ("wg_profile_get", C.c_int, [_p, C.c_int, _u64p, _f64p]),
     ("wg_profile_reset", C.c_int, [_p]),
@@ -733,6 +734,12 @@ class Context:
         n = C.c_uint64()
         self._chk(self._lib.wg_fused_level_count(self._h, C.byref(n)), "wg_fused_level_count")
         return int(n.value)
+
+    def dense_pull_binned(self) -> bool:
+        """Whether the initialised CC / SSSP's dense pulls run on the binned layout."""
+        on = C.c_int()
+        self._chk(self._lib.wg_dense_pull_binned(self._h, C.byref(on)), "wg_dense_pull_binned")
+        return bool(on.value)
 
     def read_bfs(self) -> np.ndarray:
         out = np.empty(self.V, np.uint64)

@@ -85,7 +85,7 @@ struct wg_dev_counters {
     unsigned long long last_activated;
     unsigned long long unvisited;       // vertices not reached yet (BFS, SSSP): learned-rule feature
     unsigned int app;                   // features of the learned rule: application,
-    unsigned int pad2;
+    unsigned int min_binned;            // dense CC/SSSP pulls run on the binned layout (learned rule)
     unsigned long long num_vertices;    // graph size
     unsigned long long num_edges;
     wg_count_slot slots[WG_CSLOTS];

@@ -152,6 +152,19 @@ static inline WG_HD int wg_mlp_gated(uint64_t activity, uint64_t num_edges)
     return activity <= (num_edges >> WG_MLP_GATE_LOG2);
 }
 
+/* Dense CC / SSSP pulls on the binned layout stream every edge's 2-byte code
+ * in about the time a frontier-filtered pull, its block marking and its hub
+ * chunks take for E/64 active edges (rmat-s24 FP64 SSSP: 1.42 ms per dense
+ * iteration, 2.4-5.8 ms per frontier iteration at 10-20 % of E).  Above that
+ * activity the learned schedule takes the dense pull without evaluating the
+ * model (rmat-s24 SSSP 30.7 -> 21.5 ms; E/10, E/20, E/40, E/80, E/160:
+ * 25.1, 23.1, 21.9, 21.5, 21.9 ms). */
+#define WG_SCHED_BINNED_DENSE_LOG2 6
+static inline WG_HD int wg_sched_binned_dense(int binned, uint64_t activity, uint64_t num_edges)
+{
+    return binned && activity > (num_edges >> WG_SCHED_BINNED_DENSE_LOG2);
+}
+
 /* CC and SSSP keep the plain rule on graphs with more vertices than the model
  * was trained on (L(V) above the model's bound): their schedule also decides
  * how many pulls follow, which a per-pull model cannot extrapolate.  A BFS

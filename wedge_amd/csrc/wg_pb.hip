@@ -1513,6 +1513,14 @@ bool wg_pb_min_on(const wg_ctx* ctx)
     return opt == 1 || (opt == 2 && ctx->E >= (1ull << 24) && ctx->E >= 8ull * ctx->V);
 }
 
+extern "C" int wg_dense_pull_binned(const wg_ctx* ctx, int* on)
+{
+    if (!ctx || !on)
+        return WG_E_INVAL;
+    *on = wg_pb_min_on(ctx) ? 1 : 0;
+    return WG_OK;
+}
+
 int wg_pb_min_prepare(wg_ctx* ctx)
 {
     int rc = pb_prepare_sizes(ctx, 14, 14); // 16K values per window, 16K minima per bin

@@ -874,6 +874,7 @@ struct loop_cfg {
     unsigned int sched;
     unsigned int app;
     int plain_blog2;
+    unsigned int binned; // dense pulls on the binned layout
     unsigned long long threshold;
     unsigned long long nv, ne;
 };
@@ -884,6 +885,7 @@ __device__ __forceinline__ loop_cfg read_cfg(const wg_dev_counters* c)
     g.sched = c->sched;
     g.app = c->app;
     g.plain_blog2 = c->plain_blog2;
+    g.binned = c->min_binned;
     g.threshold = c->threshold;
     g.nv = c->num_vertices;
     g.ne = c->num_edges;
@@ -905,6 +907,10 @@ __device__ __forceinline__ void loop_choose_wave(const loop_cfg& g, const float*
         *mode = WG_MODE_FRONTIER;
         return;
     case LOOP_SCHED_LEARNED:
+        if (wg_sched_binned_dense((int)g.binned, act, g.ne)) { // dense pulls on the binned layout
+            *mode = WG_MODE_DENSE;
+            return;
+        }
         // below the gate: the plain rule's frontier pull (wg_mlp_gated)
         if (mlp && !wg_mlp_gated(act, g.ne) && !wg_mlp_out_of_range(mlp, (int)g.app, g.nv)) {
             float x[WG_MLP_IN];
@@ -932,7 +938,8 @@ __device__ __forceinline__ void loop_choose_wave(const loop_cfg& g, const float*
 __global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, unsigned int sched,
                             unsigned long long threshold, int plain_blog2, unsigned int max_iters,
                             unsigned long long initial_activity, unsigned long long initial_vertices, int dense_first,
-                            unsigned long long unvisited, unsigned int app, unsigned long long nv, unsigned long long ne)
+                            unsigned long long unvisited, unsigned int app, unsigned long long nv, unsigned long long ne,
+                            unsigned int binned)
 {
     if (blockIdx.x != 0 || threadIdx.x >= WG_WAVE)
         return;
@@ -940,6 +947,7 @@ __global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, u
     g.sched = sched;
     g.app = app;
     g.plain_blog2 = plain_blog2;
+    g.binned = binned;
     g.threshold = threshold;
     g.nv = nv;
     g.ne = ne;
@@ -951,6 +959,7 @@ __global__ void k_loop_init(wg_dev_counters* c, const float* __restrict__ mlp, u
         return;
     c->sched = sched;
     c->app = app;
+    c->min_binned = binned;
     c->num_vertices = nv;
     c->num_edges = ne;
     c->threshold = threshold;
@@ -2364,7 +2373,7 @@ extern "C" int wg_converge(wg_ctx* ctx, const wg_loop_params* p, uint64_t* itera
                        ctx->app == WG_APP_CC ? (unsigned long long)ctx->V : (unsigned long long)ctx->nroots,
                        ctx->app == WG_APP_CC ? 1 : 0,
                        ctx->app == WG_APP_CC ? 0ull : (unsigned long long)(ctx->V - ctx->nroots), (unsigned int)ctx->app,
-                       (unsigned long long)ctx->V, (unsigned long long)ctx->E);
+                       (unsigned long long)ctx->V, (unsigned long long)ctx->E, wg_pb_min_on(ctx) ? 1u : 0u);
     // A BFS that starts from its roots (at most WG_PUSH_ROOTS_MAX, whose
     // out-edge offsets the init computed) pushes its first level along the
     // roots' out-edges if that level is frontier-filtered (k_bfs_level), so

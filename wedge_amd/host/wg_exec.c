@@ -231,6 +231,9 @@ static int run_converging(wg_ctx* ctx, const wg_run_opts* o, int app, pull_fn pu
     f.app = app;
     f.num_vertices = V;
     f.num_edges = E;
+    int rcb = wg_dense_pull_binned(ctx, &f.binned_dense);
+    if (rcb)
+        return rcb;
     int mode, blog2;
     f.iteration = 0;
     f.frontier_vertices = frontier_vertices;

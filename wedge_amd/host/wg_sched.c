@@ -18,6 +18,10 @@ void wg_sched_decide(const wg_run_opts* o, uint64_t threshold, int plain_block_l
         *mode = WG_MODE_FRONTIER;
         return;
     case WG_SCHED_LEARNED:
+        if (wg_sched_binned_dense(f->binned_dense, f->frontier_edges, f->num_edges)) {
+            *mode = WG_MODE_DENSE;
+            return;
+        }
         /* below the gate: the plain rule's frontier pull (wg_mlp_gated) */
         if (!wg_mlp_gated(f->frontier_edges, f->num_edges) &&
             wg_mlp_decide(f, f->frontier_edges > threshold, mode, block_log2) == WG_OK)

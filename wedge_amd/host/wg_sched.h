@@ -18,6 +18,7 @@ typedef struct wg_sched_features {
     uint64_t frontier_vertices; /* vertices activated by the last pull */
     uint64_t frontier_edges;    /* their out-edges: the reference's num_edges_active */
     uint64_t unvisited;         /* vertices not reached yet (BFS, SSSP) */
+    int binned_dense;           /* dense pulls run on the binned layout (wg_dense_pull_binned) */
 } wg_sched_features;
 
 /* Fills *mode and *block_log2 for the next pull. */
