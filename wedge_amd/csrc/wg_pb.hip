@@ -847,12 +847,13 @@ __device__ __forceinline__ void pb_tile_store(const pb_tile_in& in, const pb_lan
 {
     typedef typename OP::V V;
     const int lane = wg_lane();
-    const unsigned long long below = (1ull << lane) - 1ull;
+    // pre: pairs closed by lower lanes (mbcnt counts the mask's bits below
+    // this lane and adds its second operand: two instructions per entry)
     uint32_t pre = 0, all = 0;
 #pragma unroll
     for (int j = 0; j < WG_PB_IPT; ++j) {
         const unsigned long long m = wg_ballot(w.e[j]);
-        pre += (uint32_t)__popcll(m & below);
+        pre = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, pre));
         all += (uint32_t)__popcll(m);
     }
     // values to the LDS row in pair order, WG_PB_STAGE pairs at a time (the
