@@ -246,10 +246,14 @@ def batched_sources(ctx, g, pr_ms, damping, reps=3):
         "gteps": round(32 * g.num_edges / (ms / iters * 1e-3) / 1e9, 2),
     }
     log(f"PPR 32: {out['ppr_32']}")
-    single_sssp = float(np.mean([ctx.run_sssp(int(r))[1]["running_ms"] for r in roots[:4]]))
+    single_sssp = {s: float(np.mean([ctx.run_sssp(int(r), schedule=s)[1]["running_ms"] for r in roots[:4]]))
+                   for s in ("plain", "learned")}
     ms, it = best_ms(lambda: ctx.mssssp(roots), 1)
-    out["mssssp_64"] = {"ms": round(ms, 1), "iterations": it, "single_ms": round(single_sssp, 2),
-                        "speedup_vs_64": round(64 * single_sssp / ms, 2)}
+    out["mssssp_64"] = {"ms": round(ms, 1), "iterations": it,
+                        "single_ms_plain": round(single_sssp["plain"], 2),
+                        "single_ms_learned": round(single_sssp["learned"], 2),
+                        "speedup_vs_64_plain": round(64 * single_sssp["plain"] / ms, 2),
+                        "speedup_vs_64_learned": round(64 * single_sssp["learned"] / ms, 2)}
     log(f"MS-SSSP 64: {out['mssssp_64']}")
     return out
 
